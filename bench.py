@@ -1,0 +1,169 @@
+#!/usr/bin/env python
+"""Fock-build throughput of the MI355X engine on the seeded (H2O)_8 6-31G**
+cluster (200 Cartesian functions, Schwarz cutoff 1e-12, FP64).
+
+A step is one direct Fock build (J and K from one density, pq_fock_jk) over
+the full screened quartet list.  Prints one JSON line.  The roofline block is
+for the dominant class kernel: algorithmic Rys flops of that kernel (the
+F_prim/F_dig operation count) over its HIP-event time on the launch stream,
+against the MI355X FP64 peak.  The cpu_baseline block times the compiled
+reference crys.c (oracle/_ref) on a seeded sample of the same workload's
+unique function quartets, single-threaded.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+FP64_PEAK_TFLOPS = 78.6
+
+
+def cpu_baseline(bfs, seconds=10.0, seed=0):
+    """Time the reference's crys.c contr_coulomb (compiled from the reference
+    sources into oracle/_ref) on random unique function quartets."""
+    import ctypes
+
+    import numpy as np
+    path = os.path.join(ROOT, "oracle", "_ref", "libref_crys.so")
+    if not os.path.exists(path):
+        return None
+    lib = ctypes.CDLL(path)
+    PD, PI = ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_int)
+    lib.pqref_crys_contr_batch.argtypes = [ctypes.c_int, PI, PI, PD, PD, PD, PD, PI, PD]
+    rng = np.random.default_rng(seed)
+    n = len(bfs)
+    nprim = np.array([len(f.pexps) for f in bfs], dtype=np.int32)
+    poff = np.concatenate([[0], np.cumsum(nprim)[:-1]]).astype(np.int32)
+    e = np.array([x for f in bfs for x in f.pexps])
+    c = np.array([x for f in bfs for x in f.pcoefs])
+    nr = np.array([x for f in bfs for x in f.pnorms])
+    org = np.array([f.origin for f in bfs])
+    pw = np.array([f.powers for f in bfs], dtype=np.int32)
+
+    def sample(m):
+        q = rng.integers(0, n, size=(m, 4))
+        return (np.ascontiguousarray(nprim[q]), np.ascontiguousarray(poff[q]),
+                np.ascontiguousarray(org[q].reshape(m, 12)), np.ascontiguousarray(pw[q].reshape(m, 12)))
+
+    done, t_used, m = 0, 0.0, 2000
+    while t_used < seconds:
+        a, b, o, p = sample(m)
+        out = np.zeros(m)
+        t0 = time.perf_counter()
+        lib.pqref_crys_contr_batch(m, a.ctypes.data_as(PI), b.ctypes.data_as(PI), e.ctypes.data_as(PD),
+                                   c.ctypes.data_as(PD), nr.ctypes.data_as(PD), o.ctypes.data_as(PD),
+                                   p.ctypes.data_as(PI), out.ctypes.data_as(PD))
+        t_used += time.perf_counter() - t0
+        done += m
+    return {"value": done / t_used, "unit": "quartets/s", "cores": 1, "kind": "reference",
+            "sample": "%d random unique function quartets of the (H2O)_8 6-31G** basis, reference "
+                      "crys.c contr_coulomb compiled -O2, %.1f s, 1 thread" % (done, t_used)}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--seed", type=int, default=0)
+    ap.add_argument("--cpu-seconds", type=float, default=10.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py runs on one GPU only: this build has no multi-GPU path "
+              "(the Fock build would shard its class work lists per GPU and all-reduce J/K)",
+              file=sys.stderr)
+        return 2
+
+    import numpy as np
+    import torch
+
+    from pyquante_amd import _abi
+    from pyquante_amd.basis import BasisSet
+    from pyquante_amd.engine import Engine
+    from pyquante_amd.hartree_fock import chol_orth, geigh, mkdens
+    from pyquante_amd.molecule import water_cluster
+
+    torch.cuda.set_device(0)
+    mol = water_cluster((2, 2, 2), seed=args.seed)
+    bs = BasisSet(mol, "6-31g**")
+    eng = Engine(bs, mol)
+    eng.build(1e-12, 0.0)
+    st = eng.stats()
+    npair = eng.nbf * (eng.nbf + 1) // 2
+    unscreened = npair * (npair + 1) // 2
+    # a representative density: a few SCF iterations from the core guess
+    S, T, V = eng.get1ints()
+    h = T + V
+    X = chol_orth(S)
+    _, C = geigh(h, S, X)
+    nocc = mol.get_closedopen()[0]
+    for _ in range(3):
+        D = mkdens(C, 0, nocc)
+        J, K = eng.fock_jk(D)
+        _, C = geigh(h + 2 * J - K, S, X)
+    D = mkdens(C, 0, nocc).contiguous()
+    print("bench: (H2O)_8 6-31G** nbf=%d, %d shell quartets, %d function quartets (unscreened %d)"
+          % (eng.nbf, st["quartets_fp64"], st["fquartets_fp64"], unscreened), file=sys.stderr, flush=True)
+    for _ in range(args.warmup):
+        eng.fock_jk(D)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        eng.fock_jk(D)
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / args.steps
+    # per-class kernel times (HIP events on the launch stream)
+    cls = eng.class_stats()
+    nt = 3
+    acc = np.zeros(43)
+    for _ in range(nt):
+        _, _, ms = eng.fock_jk_timed(D)
+        acc += np.array(ms)
+    acc /= nt
+    k = int(np.argmax(acc[:42:2]))
+    c = cls[k]
+    ktime = acc[2 * k] * 1e-3
+    achieved = c["flops_fp64"] / ktime / 1e12
+    lname = "(%s%s|%s%s)" % tuple("spd"[x] for x in c["class"])
+    total_flops = st["flops_fp64"]
+    print("bench: %.2f ms/build; dominant kernel %s %.3f ms, %.2f TFLOP/s; all kernels %.2f TFLOP/s"
+          % (dt * 1e3, lname, ktime * 1e3, achieved, total_flops / (acc[42] * 1e-3) / 1e12),
+          file=sys.stderr, flush=True)
+    cpu = None
+    if not args.no_cpu:
+        cpu = cpu_baseline(bs.bfs, seconds=args.cpu_seconds)
+    fq = st["fquartets_fp64"]
+    line = {
+        "metric": "ERI quartets/s + s per Fock build, mixed vs FP64 at <=1e-6 Eh; forces/s",
+        "value": fq / dt,
+        "unit": "quartets/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": dt * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic: seeded water cluster (numpy default_rng(%d)), published 6-31G** basis" % args.seed,
+        "config": {"workload": "(H2O)_8 6-31G** RHF Fock build (J+K), 200 Cartesian functions, "
+                               "Schwarz 1e-12, FP64", "nbf": eng.nbf, "shell_quartets": st["quartets_fp64"],
+                   "function_quartets": fq, "unscreened_function_quartets": unscreened,
+                   "s_per_fock_build": dt},
+        "roofline": {"bound": "mfma", "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+                     "frac": achieved / FP64_PEAK_TFLOPS, "traffic": None,
+                     "kernel": "k_eri_%s %s FP64 J/K" % ("lane" if c["class"] != (2, 2, 2, 2) else "wave", lname),
+                     "all_kernels_tflops": total_flops / (acc[42] * 1e-3) / 1e12},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line))
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

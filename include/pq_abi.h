@@ -1,0 +1,144 @@
+/*
+ * pq_abi.h -- C ABI of libpqhip.so, the MI355X two-electron-integral and
+ * Fock-build engine for PyQuante.
+ *
+ * Plain C types only: pointers, sizes, status codes.  Every entry point
+ * returns PQ_OK (0) or a negative PQ_E* code; pq_last_error() holds the
+ * message of the last failure on the calling thread.  Device pointers
+ * ("_dev") are HIP device addresses (e.g. a torch.cuda tensor's data_ptr());
+ * host pointers ("_host") are ordinary memory.  A stream argument of NULL
+ * means the null stream.  An engine handle is not safe to share between
+ * threads without external locking; distinct handles are independent.
+ *
+ * Each entry point names the reference interface it replaces.
+ */
+#ifndef PQ_ABI_H
+#define PQ_ABI_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+  PQ_OK = 0,
+  PQ_E_ARG = -1,      /* bad argument (null pointer, size mismatch) */
+  PQ_E_AM = -2,       /* shell angular momentum above d (l > 2) */
+  PQ_E_LTOTAL = -3,   /* L_total > 9: no Rys rule (the reference's Root6 stub, crys.c:1440-1443) */
+  PQ_E_ALLOC = -4,    /* device allocation failed */
+  PQ_E_HIP = -5,      /* HIP runtime error */
+  PQ_E_STATE = -6,    /* call out of order (e.g. Fock build before pq_build_pairs) */
+  PQ_E_NPRIM = -7     /* too many primitives for the scalar entry points */
+};
+
+enum { PQ_PREC_FP64 = 0, PQ_PREC_MIXED = 1 };
+
+typedef struct pq_engine pq_engine;
+
+/* Message of the last failure on this thread ("" if none). */
+const char *pq_last_error(void);
+/* Library version, major*10000 + minor*100 + patch. */
+int pq_version(void);
+/* Number of HIP devices visible (0 when none). */
+int pq_device_count(void);
+
+/* ---------------- scalar / batched compatibility entry points --------------- */
+
+/* cints.ijkl2intindex (Src/cints.c:413-425; PyQuante/Ints.py:16), widened to
+ * 64 bits: the reference's C int overflows for nbf >= 304. Integer arithmetic
+ * on the host; no device work. */
+int pq_ijkl2intindex(int64_t i, int64_t j, int64_t k, int64_t l, uint64_t *out);
+
+/* cints.Fgamma / chgp Fgamma (Src/cints.c:340-411): Boys function for n
+ * (m, x) pairs, evaluated on the device.  compat_nr = 1 keeps the reference's
+ * EPS = 3e-7 stopping rule; 0 converges to full double precision. */
+int pq_boys_batch(int n, const double *m_host, const double *x_host, int compat_nr, double *out_host);
+
+/* Rys roots u_i = t_i^2 and weights of order nr (1..5) at n arguments X,
+ * evaluated by the device tables (Src/crys.c:99-1443 Roots; the reference
+ * returns t^2/(1-t^2)).  u_host, w_host hold n*nr values. */
+int pq_rys_roots_batch(int n, int nr, const double *X_host, double *u_host, double *w_host);
+
+/* {crys,cints,chgp}.contr_coulomb (Src/crys.c:1556-1674, cints.c:572-690,
+ * chgp.c:701-818) for n contracted quartets at once, on the device.
+ * Quartet q uses functions 4q..4q+3: nprim[4q+f] primitives starting at
+ * poff[4q+f] in exps/coefs/norms; origins[12q+3f+d], powers[12q+3f+d].
+ * coulomb_repulsion (crys.c:1676-1711) is the case nprim = 1, coef = 1. */
+int pq_contr_coulomb_batch(int n, const int *nprim_host, const int *poff_host, int nprim_total,
+                           const double *exps_host, const double *coefs_host, const double *norms_host,
+                           const double *origins_host, const int *powers_host, double *out_host);
+
+/* ---------------- engine: basis, geometry, pair data ---------------- */
+
+/* Creates an engine for nshell shells (PyQuante/Basis/basis.py:11-50 order).
+ * Shell s: angular momentum shell_l[s] (0..2), shell_nprim[s] primitives
+ * whose exponents and raw contraction coefficients (CGBF.py:65-75; the
+ * contraction is not renormalised) start at shell_poff[s] in exps/coefs,
+ * centred on atom shell_atom[s].  Functions are numbered atom -> shell ->
+ * Cartesian component in PyQuante order (Ints.py:23-29).  atom_xyz (bohr,
+ * natom*3) and atom_z give the nuclei (for V and forces). */
+int pq_engine_create(int nshell, const int *shell_l, const int *shell_nprim, const int *shell_poff,
+                     const int *shell_atom, const double *exps, const double *coefs, int natom,
+                     const double *atom_xyz, const double *atom_z, pq_engine **out);
+int pq_engine_destroy(pq_engine *eng);
+/* Number of basis functions. */
+int pq_engine_nbf(const pq_engine *eng, int *nbf);
+/* Moves the nuclei (and the shells on them); invalidates pair data. */
+int pq_engine_set_geometry(pq_engine *eng, const double *atom_xyz_host);
+
+/* Builds shell-pair data and the class-sorted quartet work lists.  Quartets
+ * whose Schwarz bound Q_ab*Q_cd is below schwarz_cut are skipped (the
+ * reference has no screening: 0 keeps every quartet); in PQ_PREC_MIXED
+ * builds, quartets with bound below fp32_cut run in FP32. */
+int pq_build_pairs(pq_engine *eng, double schwarz_cut, double fp32_cut);
+
+/* Statistics of the current work lists: counts[0..5] = FP64 shell quartets,
+ * FP32 shell quartets, FP64 primitive quartets, FP32 primitive quartets,
+ * FP64 and FP32 unique function quartets (i>=j, k>=l, ij>=kl: the unit the
+ * reference's get2ints enumerates, Ints.py:146-167); flops[0..1] =
+ * algorithmic Rys flops (FP64 part, FP32 part) of one build. */
+int pq_engine_stats(const pq_engine *eng, uint64_t counts[6], double flops[2]);
+
+/* Per-class statistics for class index c in 0..20 (pair classes X >= Y in
+ * the order ss, ps, pp, ds, dp, dd; c enumerates (X,Y) row by row):
+ * l[0..3] = (la, lb, lc, ld), counts/flops as in pq_engine_stats.
+ * No reference counterpart (diagnostics for the roofline report). */
+int pq_engine_class_stats(const pq_engine *eng, int c, int l[4], uint64_t counts[6], double flops[2]);
+
+/* ---------------- integrals and Fock builds ---------------- */
+
+/* Ints.get2ints (PyQuante/Ints.py:146-167): writes every unique ERI (ij|kl)
+ * at index ijkl2intindex(i,j,k,l) of out_dev (len = npair(npair+1)/2,
+ * npair = nbf(nbf+1)/2), FP64, unscreened. */
+int pq_get2ints(pq_engine *eng, double *out_dev, uint64_t len, void *stream);
+
+/* Ints.getJ / getK / get2JmK (PyQuante/Ints.py:195-238) in direct mode:
+ * J_ij = sum_kl (ij|kl) D_kl and K_ij = sum_kl (ik|jl) D_kl from integrals
+ * computed on the fly over the screened work lists.  D_dev is a symmetric
+ * nbf x nbf row-major FP64 matrix; J_dev or K_dev may be NULL.
+ * precision: PQ_PREC_FP64 or PQ_PREC_MIXED (needs pq_build_pairs with
+ * fp32_cut > 0). */
+int pq_fock_jk(pq_engine *eng, const double *D_dev, double *J_dev, double *K_dev, int precision,
+               void *stream);
+
+/* pq_fock_jk with HIP events around every class launch on the given stream;
+ * ms[2c] / ms[2c+1] receive the FP64 / FP32 kernel time of class c (21
+ * classes, 42 floats), ms[42] the whole build.  Synchronises the stream.
+ * No reference counterpart (measurement). */
+int pq_fock_jk_timed(pq_engine *eng, const double *D_dev, double *J_dev, double *K_dev, int precision,
+                     void *stream, float *ms);
+
+/* getJ / getK over a stored flat integral array (the reference's Ints object,
+ * Ints.py:195-223), digesting every stored unique integral on the device. */
+int pq_digest_stored(int nbf, const double *ints_dev, uint64_t len, const double *D_dev, double *J_dev,
+                     double *K_dev, void *stream);
+
+/* Ints.get1ints (PyQuante/Ints.py:73-87): S, T and V = sum_A Z_A V_A
+ * (h = T + V), nbf x nbf row-major FP64, on the device. */
+int pq_get1ints(pq_engine *eng, double *S_dev, double *T_dev, double *V_dev, void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* PQ_ABI_H */

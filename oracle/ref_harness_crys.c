@@ -34,3 +34,20 @@ void pqref_crys_roots(int n, double X, double *r, double *w) {
     w[i] = weights[i];
   }
 }
+
+/* Batch over n contracted quartets (layout of pq_contr_coulomb_batch), for
+ * timing the reference path without per-call Python overhead. */
+void pqref_crys_contr_batch(int n, const int *nprim, const int *poff, const double *e, const double *c,
+                            const double *nrm, const double *o, const int *p, double *out) {
+  for (int q = 0; q < n; ++q) {
+    const int *np = nprim + 4 * q, *po = poff + 4 * q;
+    const double *oo = o + 12 * q;
+    const int *pp = p + 12 * q;
+    out[q] = contr_coulomb(np[0], (double *)e + po[0], (double *)c + po[0], (double *)nrm + po[0], oo[0], oo[1],
+                           oo[2], pp[0], pp[1], pp[2], np[1], (double *)e + po[1], (double *)c + po[1],
+                           (double *)nrm + po[1], oo[3], oo[4], oo[5], pp[3], pp[4], pp[5], np[2],
+                           (double *)e + po[2], (double *)c + po[2], (double *)nrm + po[2], oo[6], oo[7], oo[8],
+                           pp[6], pp[7], pp[8], np[3], (double *)e + po[3], (double *)c + po[3],
+                           (double *)nrm + po[3], oo[9], oo[10], oo[11], pp[9], pp[10], pp[11]);
+  }
+}

@@ -1,0 +1,168 @@
+"""Drop-in replacement for PyQuante/Ints.py on the MI355X engine.
+
+Same entry points and argument meanings as the reference (Ints.py:35-238):
+getbasis, getints(bfs, atoms) -> (S, h, Ints), get1ints, getS, getT, getV,
+get2ints(bfs) -> Ints, getJ / getK / get2JmK(Ints, D) -> ndarray and
+ijkl2intindex.  Differences, all deliberate:
+  - Ints is an `Integrals` object: it indexes, iterates and has len() like
+    the reference's flat array('d') (materialised once on the GPU when first
+    indexed), but getJ/getK/get2JmK on it run the direct GPU Fock build and
+    never touch a stored array;
+  - getJ/getK use the Ints they are given (the reference reads module-global
+    caches filled by the last get2ints call, Ints.py:31-33,195-223);
+  - indices are 64-bit (the reference's C int overflows at nbf >= 304).
+A plain sequence of integrals in the reference layout is also accepted by
+getJ/getK/get2JmK; it is digested on the GPU.
+"""
+import numpy as np
+import torch
+
+from . import _abi
+from .basis import BasisSet, getbasis, sym2powerlist  # noqa: F401  (re-exported)
+from .engine import Engine, digest_stored
+
+# Kept for callers that set it; the engine never keeps global caches.
+sorted = True
+# Schwarz cutoff of the direct Fock build behind getJ/getK/get2JmK
+schwarz_cut = 1e-12
+
+
+class _Nucleus(object):
+    def __init__(self, xyz):
+        self.atno = 0
+        self._xyz = tuple(float(v) for v in xyz)
+
+    def pos(self):
+        return self._xyz
+
+
+def _atoms_for(bfs, atoms):
+    if atoms is not None:
+        return list(atoms)
+    centres = []
+    for f in bfs:
+        if f.origin not in centres:
+            centres.append(f.origin)
+    return [_Nucleus(c) for c in centres]
+
+
+class Integrals(object):
+    """Two-electron integrals of one basis, held by the GPU engine."""
+
+    def __init__(self, bfs, atoms=None, engine=None):
+        self.bfs = bfs
+        self.engine = engine if engine is not None else Engine(bfs, _atoms_for(bfs, atoms))
+        self.nbf = self.engine.nbf
+        npair = self.nbf * (self.nbf + 1) // 2
+        self._len = npair * (npair + 1) // 2
+        self._dev = None
+        self._host = None
+        self._built_cut = None
+
+    def __len__(self):
+        return self._len
+
+    def device_array(self):
+        """Flat unique-ERI array (reference layout) as a device tensor."""
+        if self._dev is None:
+            self._dev = self.engine.get2ints()
+        return self._dev
+
+    def host_array(self):
+        if self._host is None:
+            self._host = self.device_array().cpu().numpy()
+        return self._host
+
+    def __getitem__(self, idx):
+        return self.host_array()[idx]
+
+    def __iter__(self):
+        return iter(self.host_array())
+
+    def __array__(self, dtype=None):
+        a = self.host_array()
+        return a if dtype is None else a.astype(dtype)
+
+    def jk(self, D, want_j=True, want_k=True, precision=_abi.PQ_PREC_FP64):
+        if self._built_cut != schwarz_cut or self.engine.fp32cut:
+            self.engine.build(schwarz_cut, 0.0)
+            self._built_cut = schwarz_cut
+        return self.engine.fock_jk(D, want_j, want_k, precision)
+
+
+def ijkl2intindex(i, j, k, l):
+    """cints.ijkl2intindex (cints.c:413-425), 64-bit."""
+    from ._compat import intindex
+    return intindex(i, j, k, l)
+
+
+def _as_out(x, like):
+    if torch.is_tensor(like):
+        return x
+    return x.cpu().numpy()
+
+
+def get1ints(bfs, atoms):
+    """S and h = T + sum_A Z_A V_A (Ints.py:73-87), computed on the GPU."""
+    eng = Engine(bfs, atoms)
+    S, T, V = eng.get1ints()
+    return S.cpu().numpy(), (T + V).cpu().numpy()
+
+
+def getS(bfs):
+    eng = Engine(bfs, _atoms_for(bfs, None))
+    return eng.get1ints()[0].cpu().numpy()
+
+
+def getT(bfs):
+    eng = Engine(bfs, _atoms_for(bfs, None))
+    return eng.get1ints()[1].cpu().numpy()
+
+
+def getV(bfs, atoms):
+    eng = Engine(bfs, atoms)
+    return eng.get1ints()[2].cpu().numpy()
+
+
+def get2ints(bfs, atoms=None):
+    """Ints.get2ints (Ints.py:146-167): the unique ERIs, reference layout."""
+    return Integrals(bfs, atoms)
+
+
+def getints(bfs, atoms):
+    """Ints.getints (Ints.py:66-71): (S, h, Ints) sharing one GPU engine."""
+    eng = Engine(bfs, atoms)
+    S, T, V = eng.get1ints()
+    return S.cpu().numpy(), (T + V).cpu().numpy(), Integrals(bfs, atoms, engine=eng)
+
+
+def _jk(Ints, D, want_j, want_k):
+    Dt = D if torch.is_tensor(D) else torch.as_tensor(np.ascontiguousarray(D, dtype=np.float64))
+    if isinstance(Ints, Integrals):
+        Dt = Dt.to(Ints.engine.device, torch.float64)
+        Ds = 0.5 * (Dt + Dt.T)
+        return Ints.jk(Ds.contiguous(), want_j, want_k)
+    _abi.require_gpu()
+    dev = torch.device("cuda", torch.cuda.current_device())
+    flat = torch.as_tensor(np.ascontiguousarray(np.asarray(Ints, dtype=np.float64)), device=dev)
+    Dt = Dt.to(dev, torch.float64)
+    Ds = (0.5 * (Dt + Dt.T)).contiguous()
+    return digest_stored(flat, Ds, want_j, want_k)
+
+
+def getJ(Ints, D):
+    """J_ij = sum_kl (ij|kl) D_kl (Ints.py:195-208)."""
+    J, _ = _jk(Ints, D, True, False)
+    return _as_out(J, D)
+
+
+def getK(Ints, D):
+    """K_ij = sum_kl (ik|jl) D_kl, D symmetric (Ints.py:210-223)."""
+    _, K = _jk(Ints, D, False, True)
+    return _as_out(K, D)
+
+
+def get2JmK(Ints, D):
+    """G = 2J - K (Ints.py:225-238)."""
+    J, K = _jk(Ints, D, True, True)
+    return _as_out(2.0 * J - K, D)

@@ -1,0 +1,12 @@
+"""Drop-in for PyQuante's C extension module `chgp` (Src/chgp.c:701-855 (contr_coulomb, coulomb_repulsion = hrr)).
+
+Same function names and argument formats; the integrals are evaluated by the
+GPU Rys engine, so every method returns the same (full double precision)
+values.  Fgamma keeps the reference's Numerical-Recipes stopping rule
+(EPS = 3e-7) so that it reproduces the reference's values.
+"""
+from ._compat import contr_coulomb, coulomb_repulsion, fgamma as _fgamma, intindex as ijkl2intindex  # noqa: F401
+
+
+def Fgamma(m, x):
+    return _fgamma(m, x, compat_nr=1)

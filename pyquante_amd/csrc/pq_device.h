@@ -1,0 +1,194 @@
+// pq_device.h -- device-side building blocks of the MI355X ERI engine:
+// Cartesian component tables, shell-pair records, Rys root evaluation and
+// the 1-D Rys recurrence + transfer, shared by every integral kernel.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "rys_table.h"
+
+namespace pq {
+
+// ---------------------------------------------------------------------
+// Cartesian components in PyQuante order (Ints.py:23-29): for a shell of
+// angular momentum L, x-power runs from L down, then y-power from L-lx down.
+// D = xx, xy, xz, yy, yz, zz.
+// ---------------------------------------------------------------------
+__host__ __device__ constexpr int ncart(int L) { return (L + 1) * (L + 2) / 2; }
+
+__host__ __device__ constexpr int cart_pow(int L, int i, int d) {
+  int idx = 0;
+  for (int lx = L; lx >= 0; --lx)
+    for (int ly = L - lx; ly >= 0; --ly) {
+      if (idx == i) return d == 0 ? lx : (d == 1 ? ly : L - lx - ly);
+      ++idx;
+    }
+  return 0;
+}
+
+__host__ __device__ constexpr int dfact(int n) { return n <= 1 ? 1 : n * dfact(n - 2); }
+
+// Per-component part of the primitive normalisation (primitive-gto.c:76-83):
+// N = Nrad(alpha, L) / sqrt((2l-1)!!(2m-1)!!(2n-1)!!).  Nrad is folded into
+// the primitive-pair prefactor, this factor is applied per component.
+__host__ __device__ constexpr double const_sqrt(double x) {
+  double r = x > 1.0 ? x : 1.0;
+  for (int it = 0; it < 64; ++it) r = 0.5 * (r + x / r);
+  return r;
+}
+__host__ __device__ constexpr double cart_norm(int L, int i) {
+  return 1.0 / const_sqrt((double)(dfact(2 * cart_pow(L, i, 0) - 1) * dfact(2 * cart_pow(L, i, 1) - 1) *
+                                   dfact(2 * cart_pow(L, i, 2) - 1)));
+}
+
+// ---------------------------------------------------------------------
+// Shell-pair records in HBM.  A pair (A,B) is stored with l_A >= l_B.
+// ---------------------------------------------------------------------
+struct __align__(16) PrimPair {
+  double p;    // alpha_a + alpha_b
+  double K;    // c_a c_b Nrad_a Nrad_b exp(-alpha_a alpha_b |AB|^2 / p)
+  double Px, Py, Pz;
+  double a;    // alpha_a (used by the derivative kernels)
+};
+
+struct __align__(16) PairInfo {
+  double Ax, Ay, Az;     // centre of shell A
+  double ABx, ABy, ABz;  // A - B
+  int pp_off, pp_n;      // primitive pairs [pp_off, pp_off + pp_n)
+  int bfA, bfB;          // first basis function of each shell
+  int shA, shB;          // shell indices
+  int atA, atB;          // atom indices
+  int pad0, pad1;
+};
+
+struct Tile {
+  int bra;   // pair id
+  int ket0;  // first entry of the ket pair list covered by this tile
+  int nket;  // number of kets
+  int pad;
+};
+
+// Everything a kernel needs about the basis and the pair data.
+struct EngineView {
+  const PairInfo* pairs;
+  const PrimPair* prims;
+  int nbf;
+};
+
+// ---------------------------------------------------------------------
+// Rys roots u_i = t_i^2 and weights w_i for order NR at argument X.
+// Piecewise polynomial in X below rys_xasym[NR], large-X limit above;
+// the large-X branch is always evaluated in FP64.
+// ---------------------------------------------------------------------
+template <int NR, typename T>
+__device__ __forceinline__ void rys_roots(double X, T* u, T* w) {
+  if (X >= rys_xasym[NR]) {
+    const double rx = 1.0 / X, rs = rsqrt(X);
+#pragma unroll
+    for (int i = 0; i < NR; ++i) {
+      u[i] = (T)(rys_asym[NR - 1][i] * rx);
+      w[i] = (T)(rys_asym[NR - 1][NR + i] * rs);
+    }
+    return;
+  }
+  const int k = (int)X;  // interval width 1
+  const T s = (T)(2.0 * (X - (double)k) - 1.0);
+  const double* c = rys_coef + rys_off[NR - 1] + k * (2 * NR * (RYS_DEG + 1));
+#pragma unroll
+  for (int f = 0; f < 2 * NR; ++f) {
+    T v = (T)c[f * (RYS_DEG + 1) + RYS_DEG];
+#pragma unroll
+    for (int j = RYS_DEG - 1; j >= 0; --j) v = v * s + (T)c[f * (RYS_DEG + 1) + j];
+    if (f < NR) u[f] = v; else w[f - NR] = v;
+  }
+}
+
+// Runtime-order variant (scalar entry points and one-electron kernels).
+template <typename T>
+__device__ inline void rys_roots_rt(int nr, double X, T* u, T* w) {
+  switch (nr) {
+    case 1: rys_roots<1, T>(X, u, w); break;
+    case 2: rys_roots<2, T>(X, u, w); break;
+    case 3: rys_roots<3, T>(X, u, w); break;
+    case 4: rys_roots<4, T>(X, u, w); break;
+    default: rys_roots<5, T>(X, u, w); break;
+  }
+}
+
+// ---------------------------------------------------------------------
+// 1-D Rys integrals for one root and one Cartesian direction.
+// G(n,m) = I(n,0,m,0) by the ABD recurrence (crys.c:1481-1516 semantics,
+// in terms of u = t^2), then the transfer to I(ia,ib,ic,id)
+// (crys.c:1518-1533), done here as two horizontal recurrences
+// I(a,b+1) = I(a+1,b) + (A-B) I(a,b) and likewise on the ket.
+// G00 is the caller's seed (1, or the weight x prefactor for one direction).
+// ---------------------------------------------------------------------
+template <int LA, int LB, int LC, int LD, typename T>
+__device__ __forceinline__ void rys_1d(T G00, T C00, T D00, T B10, T B01, T B00, T AB, T CD,
+                                       T (&I)[LA + 1][LB + 1][LC + 1][LD + 1]) {
+  constexpr int N = LA + LB, M = LC + LD;
+  T G[N + 1][M + 1];
+  G[0][0] = G00;
+  if (N > 0) G[1][0] = C00 * G00;
+#pragma unroll
+  for (int a = 2; a <= N; ++a) G[a][0] = (T)(a - 1) * B10 * G[a - 2][0] + C00 * G[a - 1][0];
+  if (M > 0) {
+    G[0][1] = D00 * G00;
+#pragma unroll
+    for (int b = 2; b <= M; ++b) G[0][b] = (T)(b - 1) * B01 * G[0][b - 2] + D00 * G[0][b - 1];
+#pragma unroll
+    for (int a = 1; a <= N; ++a) {
+      G[a][1] = (T)a * B00 * G[a - 1][0] + D00 * G[a][0];
+#pragma unroll
+      for (int b = 2; b <= M; ++b)
+        G[a][b] = (T)(b - 1) * B01 * G[a][b - 2] + (T)a * B00 * G[a - 1][b - 1] + D00 * G[a][b - 1];
+    }
+  }
+  // bra transfer: H[b][a][m], valid for a <= N - b
+  T H[LB + 1][N + 1][M + 1];
+#pragma unroll
+  for (int a = 0; a <= N; ++a)
+#pragma unroll
+    for (int m = 0; m <= M; ++m) H[0][a][m] = G[a][m];
+#pragma unroll
+  for (int b = 1; b <= LB; ++b)
+#pragma unroll
+    for (int a = 0; a <= N - b; ++a)
+#pragma unroll
+      for (int m = 0; m <= M; ++m) H[b][a][m] = H[b - 1][a + 1][m] + AB * H[b - 1][a][m];
+  // ket transfer
+#pragma unroll
+  for (int ia = 0; ia <= LA; ++ia)
+#pragma unroll
+    for (int ib = 0; ib <= LB; ++ib) {
+      T Kd[LD + 1][M + 1];
+#pragma unroll
+      for (int m = 0; m <= M; ++m) Kd[0][m] = H[ib][ia][m];
+#pragma unroll
+      for (int d = 1; d <= LD; ++d)
+#pragma unroll
+        for (int c = 0; c <= M - d; ++c) Kd[d][c] = Kd[d - 1][c + 1] + CD * Kd[d - 1][c];
+#pragma unroll
+      for (int ic = 0; ic <= LC; ++ic)
+#pragma unroll
+        for (int id = 0; id <= LD; ++id) I[ia][ib][ic][id] = Kd[id][ic];
+    }
+}
+
+// Canonical unique-ERI index (cints.c:413-425) in 64-bit arithmetic.
+__host__ __device__ inline uint64_t ijkl2intindex(int64_t i, int64_t j, int64_t k, int64_t l) {
+  if (i < j) { int64_t t = i; i = j; j = t; }
+  if (k < l) { int64_t t = k; k = l; l = t; }
+  uint64_t ij = (uint64_t)(i * (i + 1) / 2 + j), kl = (uint64_t)(k * (k + 1) / 2 + l);
+  if (ij < kl) { uint64_t t = ij; ij = kl; kl = t; }
+  return ij * (ij + 1) / 2 + kl;
+}
+
+// Wavefront-wide sum (64 lanes) by butterfly exchange.
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+  for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off, 64);
+  return v;
+}
+
+}  // namespace pq

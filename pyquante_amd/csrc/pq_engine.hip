@@ -1,0 +1,682 @@
+// pq_engine.hip -- host side of libpqhip.so: the engine handle (basis,
+// shell-pair data in HBM, Schwarz-sorted work lists) and the C ABI declared in
+// include/pq_abi.h.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <numeric>
+#include <string>
+#include <vector>
+
+#include "../../include/pq_abi.h"
+#include "pq_device.h"
+#include "pq_kernels.h"
+
+using namespace pq;
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string& msg) {
+  g_err = msg;
+  return code;
+}
+
+#define PQ_HIP(expr)                                                                     \
+  do {                                                                                   \
+    hipError_t e_ = (expr);                                                              \
+    if (e_ != hipSuccess)                                                                \
+      return fail(PQ_E_HIP, std::string(#expr) + ": " + hipGetErrorString(e_));          \
+  } while (0)
+
+template <typename T>
+hipError_t dev_alloc(T** p, size_t n) {
+  *p = nullptr;
+  if (n == 0) return hipSuccess;
+  return hipMalloc((void**)p, n * sizeof(T));
+}
+
+template <typename T>
+void dev_free(T*& p) {
+  if (p) (void)hipFree(p);
+  p = nullptr;
+}
+
+int pair_class(int l1, int l2) { return l1 * (l1 + 1) / 2 + l2; }  // l1 >= l2
+const int kPairL[6][2] = {{0, 0}, {1, 0}, {1, 1}, {2, 0}, {2, 1}, {2, 2}};
+
+double fact2(int n) { return n <= 1 ? 1.0 : n * fact2(n - 2); }
+
+// radial part of the primitive norm, primitive-gto.c:76-83 without the
+// per-component double factorials
+double nrad(double a, int L) {
+  return std::sqrt(std::pow(2.0, 2 * L + 1.5) * std::pow(a, L + 1.5) / std::pow(M_PI, 1.5));
+}
+
+// Rys operation count of one primitive quartet of class (la lb|lc ld) and of
+// digesting one contracted quartet (FMA = 2 flops).
+double rys_flops_prim(int la, int lb, int lc, int ld) {
+  const int n = la + lb, m = lc + ld, nr = (n + m) / 2 + 1;
+  const int Nc = ncart(la) * ncart(lb) * ncart(lc) * ncart(ld);
+  double sb = 0, sd = 0;
+  for (int j = 1; j <= lb; ++j) sb += n - j + 1;
+  for (int l = 1; l <= ld; ++l) sd += m - l + 1;
+  return 30.0 + nr * (60.0 + 3.0 * (4.0 * (n + 1) * (m + 1) + 2.0 * (m + 1) * sb +
+                                    2.0 * (la + 1) * (lb + 1) * sd) + 4.0 * Nc) + 2.0 * Nc;
+}
+double rys_flops_dig(int la, int lb, int lc, int ld) {
+  return 12.0 * ncart(la) * ncart(lb) * ncart(lc) * ncart(ld);
+}
+
+struct ClassWork {
+  int X, Y;  // pair classes, X >= Y
+  std::vector<Tile> t64, t32;
+  Tile* d_t64 = nullptr;
+  Tile* d_t32 = nullptr;
+  uint64_t nq64 = 0, nq32 = 0;
+  double np64 = 0, np32 = 0;
+  double nf64 = 0, nf32 = 0;  // unique function quartets
+};
+
+}  // namespace
+
+struct pq_engine {
+  int nshell = 0, natom = 0, nbf = 0;
+  std::vector<int> sh_l, sh_np, sh_off, sh_atom, sh_bf;
+  std::vector<double> exps, coefs, atom_xyz, atom_z;
+
+  // pair data
+  bool pairs_ok = false;
+  std::vector<PairInfo> pairs;
+  std::vector<PrimPair> prims;
+  std::vector<int> pcls;                 // pair class of each pair
+  std::vector<std::vector<int>> members;  // per pair class: pair ids (unsorted)
+  std::vector<double> Q;                 // Schwarz bound per pair
+  std::vector<std::vector<int>> sorted;   // per pair class: sorted by Q desc
+  PairInfo* d_pairs = nullptr;
+  PrimPair* d_prims = nullptr;
+  int* d_members[6] = {};
+  int* d_sorted[6] = {};
+  double* d_atom_xyz = nullptr;
+  double* d_atom_z = nullptr;
+
+  // work lists
+  bool work_ok = false;
+  double cut = 0, fp32cut = 0;
+  std::vector<ClassWork> work;
+  std::vector<ClassWork> store_work;  // unscreened, for get2ints
+  bool store_ok = false;
+  double* d_Jacc = nullptr;
+  double* d_Kacc = nullptr;
+
+  EngineView view() const { return EngineView{d_pairs, d_prims, nbf}; }
+
+  void free_work(std::vector<ClassWork>& w) {
+    for (auto& c : w) {
+      dev_free(c.d_t64);
+      dev_free(c.d_t32);
+    }
+    w.clear();
+  }
+  void free_pairs() {
+    dev_free(d_pairs);
+    dev_free(d_prims);
+    for (int i = 0; i < 6; ++i) {
+      dev_free(d_members[i]);
+      dev_free(d_sorted[i]);
+    }
+    free_work(work);
+    free_work(store_work);
+    pairs_ok = work_ok = store_ok = false;
+  }
+  ~pq_engine() {
+    free_pairs();
+    dev_free(d_atom_xyz);
+    dev_free(d_atom_z);
+    dev_free(d_Jacc);
+    dev_free(d_Kacc);
+  }
+};
+
+namespace {
+
+int build_pair_data(pq_engine* e) {
+  e->pairs.clear();
+  e->prims.clear();
+  e->pcls.clear();
+  e->members.assign(6, {});
+  for (int A = 0; A < e->nshell; ++A)
+    for (int B = 0; B <= A; ++B) {
+      int s1 = A, s2 = B;
+      if (e->sh_l[B] > e->sh_l[A]) std::swap(s1, s2);
+      const double* c1 = &e->atom_xyz[3 * e->sh_atom[s1]];
+      const double* c2 = &e->atom_xyz[3 * e->sh_atom[s2]];
+      PairInfo pi{};
+      pi.Ax = c1[0];
+      pi.Ay = c1[1];
+      pi.Az = c1[2];
+      pi.ABx = c1[0] - c2[0];
+      pi.ABy = c1[1] - c2[1];
+      pi.ABz = c1[2] - c2[2];
+      pi.pp_off = (int)e->prims.size();
+      pi.bfA = e->sh_bf[s1];
+      pi.bfB = e->sh_bf[s2];
+      pi.shA = s1;
+      pi.shB = s2;
+      pi.atA = e->sh_atom[s1];
+      pi.atB = e->sh_atom[s2];
+      const double ab2 = pi.ABx * pi.ABx + pi.ABy * pi.ABy + pi.ABz * pi.ABz;
+      const int l1 = e->sh_l[s1], l2 = e->sh_l[s2];
+      for (int i = 0; i < e->sh_np[s1]; ++i)
+        for (int j = 0; j < e->sh_np[s2]; ++j) {
+          const double a = e->exps[e->sh_off[s1] + i], b = e->exps[e->sh_off[s2] + j];
+          const double ca = e->coefs[e->sh_off[s1] + i], cb = e->coefs[e->sh_off[s2] + j];
+          PrimPair pp{};
+          pp.p = a + b;
+          pp.a = a;
+          pp.Px = (a * c1[0] + b * c2[0]) / pp.p;
+          pp.Py = (a * c1[1] + b * c2[1]) / pp.p;
+          pp.Pz = (a * c1[2] + b * c2[2]) / pp.p;
+          pp.K = ca * cb * nrad(a, l1) * nrad(b, l2) * std::exp(-a * b / pp.p * ab2);
+          e->prims.push_back(pp);
+        }
+      pi.pp_n = (int)e->prims.size() - pi.pp_off;
+      const int pc = pair_class(l1, l2);
+      e->members[pc].push_back((int)e->pairs.size());
+      e->pcls.push_back(pc);
+      e->pairs.push_back(pi);
+    }
+  dev_free(e->d_pairs);
+  dev_free(e->d_prims);
+  PQ_HIP(dev_alloc(&e->d_pairs, e->pairs.size()));
+  PQ_HIP(dev_alloc(&e->d_prims, e->prims.size()));
+  PQ_HIP(hipMemcpy(e->d_pairs, e->pairs.data(), e->pairs.size() * sizeof(PairInfo), hipMemcpyHostToDevice));
+  PQ_HIP(hipMemcpy(e->d_prims, e->prims.data(), e->prims.size() * sizeof(PrimPair), hipMemcpyHostToDevice));
+  for (int c = 0; c < 6; ++c) {
+    dev_free(e->d_members[c]);
+    PQ_HIP(dev_alloc(&e->d_members[c], e->members[c].size()));
+    if (!e->members[c].empty())
+      PQ_HIP(hipMemcpy(e->d_members[c], e->members[c].data(), e->members[c].size() * sizeof(int),
+                       hipMemcpyHostToDevice));
+  }
+  // Schwarz bounds from the diagonal quartets (ab|ab)
+  double* d_q = nullptr;
+  PQ_HIP(dev_alloc(&d_q, e->pairs.size()));
+  for (int c = 0; c < 6; ++c) {
+    const int np = (int)e->members[c].size();
+    if (!np) continue;
+    std::vector<Tile> tiles;
+    for (int k = 0; k < np; k += 64) tiles.push_back(Tile{-1, k, std::min(64, np - k), 0});
+    Tile* d_tiles = nullptr;
+    PQ_HIP(dev_alloc(&d_tiles, tiles.size()));
+    PQ_HIP(hipMemcpy(d_tiles, tiles.data(), tiles.size() * sizeof(Tile), hipMemcpyHostToDevice));
+    // the heavy-class kernel walks a tile's kets sequentially: same tiles work
+    hipError_t err = launch_eri_class(kPairL[c][0], kPairL[c][1], kPairL[c][0], kPairL[c][1], MODE_DIAG, 0,
+                                      e->view(), d_tiles, (int)tiles.size(), e->d_members[c], nullptr,
+                                      nullptr, nullptr, nullptr, d_q, 0);
+    PQ_HIP(err);
+    PQ_HIP(hipDeviceSynchronize());
+    dev_free(d_tiles);
+  }
+  e->Q.assign(e->pairs.size(), 0.0);
+  PQ_HIP(hipMemcpy(e->Q.data(), d_q, e->pairs.size() * sizeof(double), hipMemcpyDeviceToHost));
+  dev_free(d_q);
+  // per class, sorted by Q descending (ties by pair id)
+  e->sorted.assign(6, {});
+  for (int c = 0; c < 6; ++c) {
+    auto v = e->members[c];
+    std::stable_sort(v.begin(), v.end(), [&](int a, int b) { return e->Q[a] > e->Q[b]; });
+    e->sorted[c] = v;
+    dev_free(e->d_sorted[c]);
+    PQ_HIP(dev_alloc(&e->d_sorted[c], v.size()));
+    if (!v.empty())
+      PQ_HIP(hipMemcpy(e->d_sorted[c], v.data(), v.size() * sizeof(int), hipMemcpyHostToDevice));
+  }
+  e->pairs_ok = true;
+  return PQ_OK;
+}
+
+int build_work(pq_engine* e, std::vector<ClassWork>& out, double cut, double fp32cut) {
+  for (auto& c : out) {
+    dev_free(c.d_t64);
+    dev_free(c.d_t32);
+  }
+  out.clear();
+  for (int X = 0; X < 6; ++X)
+    for (int Y = 0; Y <= X; ++Y) {
+      ClassWork cw;
+      cw.X = X;
+      cw.Y = Y;
+      const auto& bras = e->sorted[X];
+      const auto& kets = e->sorted[Y];
+      if (bras.empty() || kets.empty()) {
+        out.push_back(cw);
+        continue;
+      }
+      const int la = kPairL[X][0], lb = kPairL[X][1], lc = kPairL[Y][0], ld = kPairL[Y][1];
+      const bool heavy = ncart(la) * ncart(lb) * ncart(lc) * ncart(ld) > kLaneMaxComp;
+      const int tile = heavy ? 4 : 64;
+      std::vector<double> kq(kets.size());
+      std::vector<double> kpre(kets.size() + 1, 0.0), fpre(kets.size() + 1, 0.0);
+      auto ufun = [&](int pid, int l1, int l2) -> double {  // unique function pairs of a shell pair
+        const PairInfo& p = e->pairs[pid];
+        return p.shA == p.shB ? ncart(l1) * (ncart(l1) + 1) / 2.0 : (double)ncart(l1) * ncart(l2);
+      };
+      for (size_t j = 0; j < kets.size(); ++j) {
+        kq[j] = e->Q[kets[j]];
+        kpre[j + 1] = kpre[j] + e->pairs[kets[j]].pp_n;
+        fpre[j + 1] = fpre[j] + ufun(kets[j], lc, ld);
+      }
+      auto count_ge = [&](double thr) -> int {
+        // kets sorted by Q descending: number with Q >= thr
+        return (int)(std::upper_bound(kq.begin(), kq.end(), thr, [](double t, double q) { return t > q; }) -
+                     kq.begin());
+      };
+      for (size_t i = 0; i < bras.size(); ++i) {
+        const double qi = e->Q[bras[i]];
+        int nk;
+        if (cut <= 0.0) nk = (int)kets.size();
+        else if (qi <= 0.0) nk = 0;
+        else nk = count_ge(cut / qi);
+        if (X == Y) nk = std::min(nk, (int)i + 1);
+        int n64 = nk;
+        if (fp32cut > 0.0) n64 = qi <= 0.0 ? 0 : std::min(nk, count_ge(fp32cut / qi));
+        const int bpn = e->pairs[bras[i]].pp_n;
+        for (int k = 0; k < n64; k += tile) cw.t64.push_back(Tile{bras[i], k, std::min(tile, n64 - k), 0});
+        for (int k = n64; k < nk; k += tile) cw.t32.push_back(Tile{bras[i], k, std::min(tile, nk - k), 0});
+        cw.nq64 += n64;
+        cw.nq32 += nk - n64;
+        cw.np64 += bpn * (kpre[n64] - kpre[0]);
+        cw.np32 += bpn * (kpre[nk] - kpre[n64]);
+        // unique function quartets; the diagonal quartet (bra == ket, the last
+        // ket of a same-class row) has ua(ua+1)/2 instead of ua^2
+        const double ua = ufun(bras[i], la, lb);
+        double f64 = ua * fpre[n64], f32 = ua * (fpre[nk] - fpre[n64]);
+        if (X == Y && nk == (int)i + 1) {
+          const double corr = ua * ua - ua * (ua + 1) / 2.0;
+          if (n64 == nk) f64 -= corr; else f32 -= corr;
+        }
+        cw.nf64 += f64;
+        cw.nf32 += f32;
+      }
+      PQ_HIP(dev_alloc(&cw.d_t64, cw.t64.size()));
+      PQ_HIP(dev_alloc(&cw.d_t32, cw.t32.size()));
+      if (!cw.t64.empty())
+        PQ_HIP(hipMemcpy(cw.d_t64, cw.t64.data(), cw.t64.size() * sizeof(Tile), hipMemcpyHostToDevice));
+      if (!cw.t32.empty())
+        PQ_HIP(hipMemcpy(cw.d_t32, cw.t32.data(), cw.t32.size() * sizeof(Tile), hipMemcpyHostToDevice));
+      out.push_back(std::move(cw));
+    }
+  return PQ_OK;
+}
+
+struct DevBuf {
+  void* p = nullptr;
+  ~DevBuf() {
+    if (p) (void)hipFree(p);
+  }
+};
+
+}  // namespace
+
+extern "C" {
+
+const char* pq_last_error(void) { return g_err.c_str(); }
+int pq_version(void) { return 100; }
+
+int pq_device_count(void) {
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess) return 0;
+  return n;
+}
+
+int pq_ijkl2intindex(int64_t i, int64_t j, int64_t k, int64_t l, uint64_t* out) {
+  if (!out) return fail(PQ_E_ARG, "null output");
+  if (i < 0 || j < 0 || k < 0 || l < 0) return fail(PQ_E_ARG, "negative index");
+  *out = ijkl2intindex(i, j, k, l);
+  return PQ_OK;
+}
+
+int pq_boys_batch(int n, const double* m_host, const double* x_host, int compat_nr, double* out_host) {
+  if (n < 0 || (n > 0 && (!m_host || !x_host || !out_host))) return fail(PQ_E_ARG, "bad arguments");
+  if (n == 0) return PQ_OK;
+  for (int i = 0; i < n; ++i)
+    if (!(m_host[i] >= 0.0) || !(x_host[i] >= 0.0)) return fail(PQ_E_ARG, "Boys function needs m, x >= 0");
+  DevBuf bm, bx, bo;
+  PQ_HIP(hipMalloc(&bm.p, n * sizeof(double)));
+  PQ_HIP(hipMalloc(&bx.p, n * sizeof(double)));
+  PQ_HIP(hipMalloc(&bo.p, n * sizeof(double)));
+  PQ_HIP(hipMemcpy(bm.p, m_host, n * sizeof(double), hipMemcpyHostToDevice));
+  PQ_HIP(hipMemcpy(bx.p, x_host, n * sizeof(double), hipMemcpyHostToDevice));
+  PQ_HIP(launch_boys(n, (double*)bm.p, (double*)bx.p, compat_nr ? 3.0e-7 : 1e-17, (double*)bo.p, 0));
+  PQ_HIP(hipMemcpy(out_host, bo.p, n * sizeof(double), hipMemcpyDeviceToHost));
+  return PQ_OK;
+}
+
+int pq_rys_roots_batch(int n, int nr, const double* X_host, double* u_host, double* w_host) {
+  if (n < 0 || (n > 0 && (!X_host || !u_host || !w_host))) return fail(PQ_E_ARG, "bad arguments");
+  if (nr < 1 || nr > 5) return fail(PQ_E_LTOTAL, "Rys order must be 1..5");
+  if (n == 0) return PQ_OK;
+  for (int i = 0; i < n; ++i)
+    if (!(X_host[i] >= 0.0)) return fail(PQ_E_ARG, "X must be >= 0");
+  DevBuf bx, bu, bw;
+  PQ_HIP(hipMalloc(&bx.p, n * sizeof(double)));
+  PQ_HIP(hipMalloc(&bu.p, (size_t)n * nr * sizeof(double)));
+  PQ_HIP(hipMalloc(&bw.p, (size_t)n * nr * sizeof(double)));
+  PQ_HIP(hipMemcpy(bx.p, X_host, n * sizeof(double), hipMemcpyHostToDevice));
+  PQ_HIP(launch_rys_roots(n, nr, (double*)bx.p, (double*)bu.p, (double*)bw.p, 0));
+  PQ_HIP(hipMemcpy(u_host, bu.p, (size_t)n * nr * sizeof(double), hipMemcpyDeviceToHost));
+  PQ_HIP(hipMemcpy(w_host, bw.p, (size_t)n * nr * sizeof(double), hipMemcpyDeviceToHost));
+  return PQ_OK;
+}
+
+int pq_contr_coulomb_batch(int n, const int* nprim_host, const int* poff_host, int nprim_total,
+                           const double* exps_host, const double* coefs_host, const double* norms_host,
+                           const double* origins_host, const int* powers_host, double* out_host) {
+  if (n < 0 || nprim_total < 0) return fail(PQ_E_ARG, "negative size");
+  if (n == 0) return PQ_OK;
+  if (!nprim_host || !poff_host || !exps_host || !coefs_host || !norms_host || !origins_host || !powers_host ||
+      !out_host)
+    return fail(PQ_E_ARG, "null pointer");
+  for (int q = 0; q < n; ++q) {
+    int L = 0;
+    for (int f = 0; f < 4; ++f) {
+      const int np = nprim_host[4 * q + f], po = poff_host[4 * q + f];
+      if (np < 1 || po < 0 || po + np > nprim_total) return fail(PQ_E_ARG, "primitive range out of bounds");
+      for (int d = 0; d < 3; ++d) {
+        const int pw = powers_host[12 * q + 3 * f + d];
+        if (pw < 0) return fail(PQ_E_ARG, "negative power");
+        L += pw;
+      }
+    }
+    if (L > 9) return fail(PQ_E_LTOTAL, "L_total > 9 has no Rys rule here (reference: Root6 stub)");
+  }
+  DevBuf b_np, b_po, b_e, b_c, b_n, b_o, b_p, b_out;
+  PQ_HIP(hipMalloc(&b_np.p, 4 * n * sizeof(int)));
+  PQ_HIP(hipMalloc(&b_po.p, 4 * n * sizeof(int)));
+  PQ_HIP(hipMalloc(&b_e.p, nprim_total * sizeof(double)));
+  PQ_HIP(hipMalloc(&b_c.p, nprim_total * sizeof(double)));
+  PQ_HIP(hipMalloc(&b_n.p, nprim_total * sizeof(double)));
+  PQ_HIP(hipMalloc(&b_o.p, 12 * n * sizeof(double)));
+  PQ_HIP(hipMalloc(&b_p.p, 12 * n * sizeof(int)));
+  PQ_HIP(hipMalloc(&b_out.p, n * sizeof(double)));
+  PQ_HIP(hipMemcpy(b_np.p, nprim_host, 4 * n * sizeof(int), hipMemcpyHostToDevice));
+  PQ_HIP(hipMemcpy(b_po.p, poff_host, 4 * n * sizeof(int), hipMemcpyHostToDevice));
+  PQ_HIP(hipMemcpy(b_e.p, exps_host, nprim_total * sizeof(double), hipMemcpyHostToDevice));
+  PQ_HIP(hipMemcpy(b_c.p, coefs_host, nprim_total * sizeof(double), hipMemcpyHostToDevice));
+  PQ_HIP(hipMemcpy(b_n.p, norms_host, nprim_total * sizeof(double), hipMemcpyHostToDevice));
+  PQ_HIP(hipMemcpy(b_o.p, origins_host, 12 * n * sizeof(double), hipMemcpyHostToDevice));
+  PQ_HIP(hipMemcpy(b_p.p, powers_host, 12 * n * sizeof(int), hipMemcpyHostToDevice));
+  PQ_HIP(launch_contr_eri(n, (int*)b_np.p, (int*)b_po.p, (double*)b_e.p, (double*)b_c.p, (double*)b_n.p,
+                          (double*)b_o.p, (int*)b_p.p, (double*)b_out.p, 0));
+  PQ_HIP(hipMemcpy(out_host, b_out.p, n * sizeof(double), hipMemcpyDeviceToHost));
+  return PQ_OK;
+}
+
+int pq_engine_create(int nshell, const int* shell_l, const int* shell_nprim, const int* shell_poff,
+                     const int* shell_atom, const double* exps, const double* coefs, int natom,
+                     const double* atom_xyz, const double* atom_z, pq_engine** out) {
+  if (!out || nshell <= 0 || natom <= 0 || !shell_l || !shell_nprim || !shell_poff || !shell_atom || !exps ||
+      !coefs || !atom_xyz || !atom_z)
+    return fail(PQ_E_ARG, "bad engine arguments");
+  *out = nullptr;
+  int nprim_total = 0;
+  for (int s = 0; s < nshell; ++s) {
+    if (shell_l[s] < 0 || shell_l[s] > 2) return fail(PQ_E_AM, "shell angular momentum must be 0..2");
+    if (shell_nprim[s] < 1 || shell_poff[s] < 0) return fail(PQ_E_ARG, "bad primitive range");
+    if (shell_atom[s] < 0 || shell_atom[s] >= natom) return fail(PQ_E_ARG, "bad shell atom");
+    nprim_total = std::max(nprim_total, shell_poff[s] + shell_nprim[s]);
+  }
+  pq_engine* e = new pq_engine();
+  e->nshell = nshell;
+  e->natom = natom;
+  e->sh_l.assign(shell_l, shell_l + nshell);
+  e->sh_np.assign(shell_nprim, shell_nprim + nshell);
+  e->sh_off.assign(shell_poff, shell_poff + nshell);
+  e->sh_atom.assign(shell_atom, shell_atom + nshell);
+  e->exps.assign(exps, exps + nprim_total);
+  e->coefs.assign(coefs, coefs + nprim_total);
+  e->atom_xyz.assign(atom_xyz, atom_xyz + 3 * natom);
+  e->atom_z.assign(atom_z, atom_z + natom);
+  e->sh_bf.resize(nshell);
+  int nbf = 0;
+  for (int s = 0; s < nshell; ++s) {
+    e->sh_bf[s] = nbf;
+    nbf += ncart(shell_l[s]);
+  }
+  e->nbf = nbf;
+  if (dev_alloc(&e->d_atom_xyz, 3 * natom) != hipSuccess || dev_alloc(&e->d_atom_z, natom) != hipSuccess ||
+      dev_alloc(&e->d_Jacc, (size_t)nbf * nbf) != hipSuccess || dev_alloc(&e->d_Kacc, (size_t)nbf * nbf) != hipSuccess) {
+    delete e;
+    return fail(PQ_E_ALLOC, "device allocation failed");
+  }
+  if (hipMemcpy(e->d_atom_xyz, atom_xyz, 3 * natom * sizeof(double), hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(e->d_atom_z, atom_z, natom * sizeof(double), hipMemcpyHostToDevice) != hipSuccess) {
+    delete e;
+    return fail(PQ_E_HIP, "copy of nuclei failed");
+  }
+  *out = e;
+  return PQ_OK;
+}
+
+int pq_engine_destroy(pq_engine* eng) {
+  delete eng;
+  return PQ_OK;
+}
+
+int pq_engine_nbf(const pq_engine* eng, int* nbf) {
+  if (!eng || !nbf) return fail(PQ_E_ARG, "null argument");
+  *nbf = eng->nbf;
+  return PQ_OK;
+}
+
+int pq_engine_set_geometry(pq_engine* eng, const double* atom_xyz_host) {
+  if (!eng || !atom_xyz_host) return fail(PQ_E_ARG, "null argument");
+  eng->atom_xyz.assign(atom_xyz_host, atom_xyz_host + 3 * eng->natom);
+  PQ_HIP(hipMemcpy(eng->d_atom_xyz, atom_xyz_host, 3 * eng->natom * sizeof(double), hipMemcpyHostToDevice));
+  eng->free_pairs();
+  return PQ_OK;
+}
+
+int pq_build_pairs(pq_engine* eng, double schwarz_cut, double fp32_cut) {
+  if (!eng) return fail(PQ_E_ARG, "null engine");
+  if (!(schwarz_cut >= 0.0) || !(fp32_cut >= 0.0)) return fail(PQ_E_ARG, "cutoffs must be >= 0");
+  if (!eng->pairs_ok) {
+    int rc = build_pair_data(eng);
+    if (rc) return rc;
+  }
+  int rc = build_work(eng, eng->work, schwarz_cut, fp32_cut);
+  if (rc) return rc;
+  eng->cut = schwarz_cut;
+  eng->fp32cut = fp32_cut;
+  eng->work_ok = true;
+  return PQ_OK;
+}
+
+int pq_engine_stats(const pq_engine* eng, uint64_t counts[6], double flops[2]) {
+  if (!eng || !counts || !flops) return fail(PQ_E_ARG, "null argument");
+  if (!eng->work_ok) return fail(PQ_E_STATE, "pq_build_pairs not called");
+  for (int i = 0; i < 6; ++i) counts[i] = 0;
+  flops[0] = flops[1] = 0;
+  for (const auto& c : eng->work) {
+    const int la = kPairL[c.X][0], lb = kPairL[c.X][1], lc = kPairL[c.Y][0], ld = kPairL[c.Y][1];
+    counts[0] += c.nq64;
+    counts[1] += c.nq32;
+    counts[2] += (uint64_t)c.np64;
+    counts[3] += (uint64_t)c.np32;
+    counts[4] += (uint64_t)c.nf64;
+    counts[5] += (uint64_t)c.nf32;
+    flops[0] += c.np64 * rys_flops_prim(la, lb, lc, ld) + c.nq64 * rys_flops_dig(la, lb, lc, ld);
+    flops[1] += c.np32 * rys_flops_prim(la, lb, lc, ld) + c.nq32 * rys_flops_dig(la, lb, lc, ld);
+  }
+  return PQ_OK;
+}
+
+int pq_get2ints(pq_engine* eng, double* out_dev, uint64_t len, void* stream) {
+  if (!eng || !out_dev) return fail(PQ_E_ARG, "null argument");
+  const uint64_t n = eng->nbf, npair = n * (n + 1) / 2;
+  if (len != npair * (npair + 1) / 2) return fail(PQ_E_ARG, "output length must be npair*(npair+1)/2");
+  if (!eng->pairs_ok) {
+    int rc = build_pair_data(eng);
+    if (rc) return rc;
+  }
+  if (!eng->store_ok) {
+    int rc = build_work(eng, eng->store_work, 0.0, 0.0);
+    if (rc) return rc;
+    eng->store_ok = true;
+  }
+  hipStream_t st = (hipStream_t)stream;
+  PQ_HIP(launch_fill(out_dev, (int64_t)len, 0.0, st));
+  for (const auto& c : eng->store_work) {
+    if (c.t64.empty()) continue;
+    PQ_HIP(launch_eri_class(kPairL[c.X][0], kPairL[c.X][1], kPairL[c.Y][0], kPairL[c.Y][1], MODE_STORE, 0,
+                            eng->view(), c.d_t64, (int)c.t64.size(), eng->d_sorted[c.Y], nullptr, nullptr,
+                            nullptr, out_dev, nullptr, st));
+  }
+  return PQ_OK;
+}
+
+int pq_engine_class_stats(const pq_engine* eng, int c, int l[4], uint64_t counts[6], double flops[2]) {
+  if (!eng || !l || !counts || !flops) return fail(PQ_E_ARG, "null argument");
+  if (!eng->work_ok) return fail(PQ_E_STATE, "pq_build_pairs not called");
+  if (c < 0 || c >= (int)eng->work.size()) return fail(PQ_E_ARG, "class index out of range");
+  const ClassWork& w = eng->work[c];
+  l[0] = kPairL[w.X][0];
+  l[1] = kPairL[w.X][1];
+  l[2] = kPairL[w.Y][0];
+  l[3] = kPairL[w.Y][1];
+  counts[0] = w.nq64;
+  counts[1] = w.nq32;
+  counts[2] = (uint64_t)w.np64;
+  counts[3] = (uint64_t)w.np32;
+  counts[4] = (uint64_t)w.nf64;
+  counts[5] = (uint64_t)w.nf32;
+  flops[0] = w.np64 * rys_flops_prim(l[0], l[1], l[2], l[3]) + w.nq64 * rys_flops_dig(l[0], l[1], l[2], l[3]);
+  flops[1] = w.np32 * rys_flops_prim(l[0], l[1], l[2], l[3]) + w.nq32 * rys_flops_dig(l[0], l[1], l[2], l[3]);
+  return PQ_OK;
+}
+
+static int fock_jk_impl(pq_engine* eng, const double* D_dev, double* J_dev, double* K_dev, int precision,
+                        void* stream, float* ms);
+
+int pq_fock_jk(pq_engine* eng, const double* D_dev, double* J_dev, double* K_dev, int precision, void* stream) {
+  return fock_jk_impl(eng, D_dev, J_dev, K_dev, precision, stream, nullptr);
+}
+
+int pq_fock_jk_timed(pq_engine* eng, const double* D_dev, double* J_dev, double* K_dev, int precision,
+                     void* stream, float* ms) {
+  if (!ms) return fail(PQ_E_ARG, "null timing array");
+  return fock_jk_impl(eng, D_dev, J_dev, K_dev, precision, stream, ms);
+}
+
+static int fock_jk_impl(pq_engine* eng, const double* D_dev, double* J_dev, double* K_dev, int precision,
+                        void* stream, float* ms) {
+  if (!eng || !D_dev) return fail(PQ_E_ARG, "null argument");
+  if (!eng->work_ok) return fail(PQ_E_STATE, "pq_build_pairs must be called before pq_fock_jk");
+  if (precision != PQ_PREC_FP64 && precision != PQ_PREC_MIXED) return fail(PQ_E_ARG, "bad precision");
+  if (precision == PQ_PREC_MIXED && !(eng->fp32cut > 0.0))
+    return fail(PQ_E_STATE, "mixed precision needs pq_build_pairs with fp32_cut > 0");
+  if (precision == PQ_PREC_FP64 && eng->fp32cut > 0.0)
+    return fail(PQ_E_STATE, "work lists were built for mixed precision; rebuild with fp32_cut = 0");
+  hipStream_t st = (hipStream_t)stream;
+  const int64_t nn = (int64_t)eng->nbf * eng->nbf;
+  std::vector<hipEvent_t> ev;
+  auto mark = [&]() -> hipError_t {
+    if (!ms) return hipSuccess;
+    hipEvent_t e;
+    hipError_t r = hipEventCreate(&e);
+    if (r != hipSuccess) return r;
+    ev.push_back(e);
+    return hipEventRecord(e, st);
+  };
+  auto cleanup = [&]() {
+    for (auto e : ev) (void)hipEventDestroy(e);
+  };
+  int rc = PQ_OK;
+  std::vector<int> slot;  // which ms[] entry each consecutive event pair measures
+  do {
+    hipError_t err;
+    if ((err = mark()) != hipSuccess) { rc = fail(PQ_E_HIP, hipGetErrorString(err)); break; }
+    if ((err = launch_fill(eng->d_Jacc, nn, 0.0, st)) != hipSuccess ||
+        (err = launch_fill(eng->d_Kacc, nn, 0.0, st)) != hipSuccess) {
+      rc = fail(PQ_E_HIP, hipGetErrorString(err));
+      break;
+    }
+    for (size_t ci = 0; ci < eng->work.size() && rc == PQ_OK; ++ci) {
+      const auto& c = eng->work[ci];
+      const int la = kPairL[c.X][0], lb = kPairL[c.X][1], lc = kPairL[c.Y][0], ld = kPairL[c.Y][1];
+      for (int prec = 0; prec < 2; ++prec) {
+        const auto& tl = prec ? c.t32 : c.t64;
+        if (tl.empty()) continue;
+        if ((err = mark()) != hipSuccess) { rc = fail(PQ_E_HIP, hipGetErrorString(err)); break; }
+        err = launch_eri_class(la, lb, lc, ld, MODE_JK, prec, eng->view(), prec ? c.d_t32 : c.d_t64,
+                               (int)tl.size(), eng->d_sorted[c.Y], D_dev, eng->d_Jacc, eng->d_Kacc, nullptr,
+                               nullptr, st);
+        if (err != hipSuccess) { rc = fail(PQ_E_HIP, hipGetErrorString(err)); break; }
+        if ((err = mark()) != hipSuccess) { rc = fail(PQ_E_HIP, hipGetErrorString(err)); break; }
+        slot.push_back((int)(2 * ci + prec));
+      }
+    }
+    if (rc != PQ_OK) break;
+    if ((err = launch_symmetrize(eng->nbf, eng->d_Jacc, eng->d_Kacc, J_dev, K_dev, 1.0, 1.0, st)) != hipSuccess) {
+      rc = fail(PQ_E_HIP, hipGetErrorString(err));
+      break;
+    }
+    if ((err = mark()) != hipSuccess) { rc = fail(PQ_E_HIP, hipGetErrorString(err)); break; }
+    if (ms) {
+      if ((err = hipStreamSynchronize(st)) != hipSuccess) { rc = fail(PQ_E_HIP, hipGetErrorString(err)); break; }
+      for (int i = 0; i < 43; ++i) ms[i] = 0.f;
+      for (size_t k = 0; k < slot.size(); ++k) {
+        float t = 0.f;
+        (void)hipEventElapsedTime(&t, ev[1 + 2 * k], ev[2 + 2 * k]);
+        ms[slot[k]] += t;
+      }
+      float tot = 0.f;
+      (void)hipEventElapsedTime(&tot, ev.front(), ev.back());
+      ms[42] = tot;
+    }
+  } while (0);
+  cleanup();
+  return rc;
+}
+
+int pq_digest_stored(int nbf, const double* ints_dev, uint64_t len, const double* D_dev, double* J_dev,
+                     double* K_dev, void* stream) {
+  if (nbf <= 0 || !ints_dev || !D_dev) return fail(PQ_E_ARG, "bad arguments");
+  const uint64_t n = nbf, npair = n * (n + 1) / 2;
+  if (len != npair * (npair + 1) / 2) return fail(PQ_E_ARG, "integral array length mismatch");
+  hipStream_t st = (hipStream_t)stream;
+  double *Jacc = nullptr, *Kacc = nullptr;
+  PQ_HIP(dev_alloc(&Jacc, (size_t)nbf * nbf));
+  PQ_HIP(dev_alloc(&Kacc, (size_t)nbf * nbf));
+  hipError_t err = launch_fill(Jacc, (int64_t)nbf * nbf, 0.0, st);
+  if (err == hipSuccess) err = launch_fill(Kacc, (int64_t)nbf * nbf, 0.0, st);
+  if (err == hipSuccess) err = launch_digest_stored(nbf, ints_dev, len, D_dev, Jacc, Kacc, st);
+  if (err == hipSuccess) err = launch_symmetrize(nbf, Jacc, Kacc, J_dev, K_dev, 1.0, 1.0, st);
+  if (err == hipSuccess) err = hipStreamSynchronize(st);
+  dev_free(Jacc);
+  dev_free(Kacc);
+  PQ_HIP(err);
+  return PQ_OK;
+}
+
+int pq_get1ints(pq_engine* eng, double* S_dev, double* T_dev, double* V_dev, void* stream) {
+  if (!eng || !S_dev || !T_dev || !V_dev) return fail(PQ_E_ARG, "null argument");
+  if (!eng->pairs_ok) {
+    int rc = build_pair_data(eng);
+    if (rc) return rc;
+  }
+  hipStream_t st = (hipStream_t)stream;
+  for (int c = 0; c < 6; ++c) {
+    if (eng->members[c].empty()) continue;
+    PQ_HIP(launch_onee(kPairL[c][0], kPairL[c][1], eng->view(), eng->d_members[c], (int)eng->members[c].size(),
+                       eng->d_atom_xyz, eng->d_atom_z, eng->natom, S_dev, T_dev, V_dev, st));
+  }
+  return PQ_OK;
+}
+
+}  // extern "C"

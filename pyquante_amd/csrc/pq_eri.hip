@@ -1,0 +1,133 @@
+// pq_eri.hip -- dispatch of the per-class Rys ERI kernels (each class is
+// compiled in its own translation unit from pq_eri_cls.hip), the stored-array
+// digestion kernel and the J/K symmetrisation.
+#include "pq_device.h"
+#include "pq_kernels.h"
+
+namespace pq {
+
+#define PQ_DECL(A, B, C_, D_)                                                                              \
+  hipError_t launch_cls_##A##B##C_##D_(int mode, int fp32, const EngineView& ev, const Tile* tiles, int ntiles, \
+                                       const int* kets, const double* D, double* Jacc, double* Kacc,       \
+                                       double* ints, double* qout, hipStream_t st);
+PQ_DECL(0, 0, 0, 0)
+PQ_DECL(1, 0, 0, 0)
+PQ_DECL(1, 0, 1, 0)
+PQ_DECL(1, 1, 0, 0)
+PQ_DECL(1, 1, 1, 0)
+PQ_DECL(1, 1, 1, 1)
+PQ_DECL(2, 0, 0, 0)
+PQ_DECL(2, 0, 1, 0)
+PQ_DECL(2, 0, 1, 1)
+PQ_DECL(2, 0, 2, 0)
+PQ_DECL(2, 1, 0, 0)
+PQ_DECL(2, 1, 1, 0)
+PQ_DECL(2, 1, 1, 1)
+PQ_DECL(2, 1, 2, 0)
+PQ_DECL(2, 1, 2, 1)
+PQ_DECL(2, 2, 0, 0)
+PQ_DECL(2, 2, 1, 0)
+PQ_DECL(2, 2, 1, 1)
+PQ_DECL(2, 2, 2, 0)
+PQ_DECL(2, 2, 2, 1)
+PQ_DECL(2, 2, 2, 2)
+#undef PQ_DECL
+
+// ---------------------------------------------------------------------
+// J = Jacc + Jacc^T, K = Kacc + Kacc^T (outputs may be null).
+// ---------------------------------------------------------------------
+__global__ void k_symmetrize(int n, const double* __restrict__ Jacc, const double* __restrict__ Kacc,
+                             double* J, double* K, double jscale, double kscale) {
+  const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= (int64_t)n * n) return;
+  const int i = (int)(idx / n), j = (int)(idx % n);
+  if (J) J[idx] = jscale * (Jacc[idx] + Jacc[(int64_t)j * n + i]);
+  if (K) K[idx] = kscale * (Kacc[idx] + Kacc[(int64_t)j * n + i]);
+}
+
+// ---------------------------------------------------------------------
+// Digestion of a stored flat array Ints[ijkl2intindex(i,j,k,l)]: entry t
+// is decoded back to (ij, kl) with ij >= kl and then to i >= j, k >= l; the
+// same accumulation rule as the direct path applies with the function-level
+// degeneracy factor.
+// ---------------------------------------------------------------------
+__device__ __forceinline__ int64_t tri_root(uint64_t t) {
+  int64_t r = (int64_t)((sqrt(8.0 * (double)t + 1.0) - 1.0) * 0.5);
+  while ((uint64_t)(r + 1) * (uint64_t)(r + 2) / 2 <= t) ++r;
+  while (r > 0 && (uint64_t)r * (uint64_t)(r + 1) / 2 > t) --r;
+  return r;
+}
+
+__global__ void k_digest_stored(int n, const double* __restrict__ ints, uint64_t len, const double* __restrict__ D,
+                                double* Jacc, double* Kacc) {
+  uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  for (; t < len; t += stride) {
+    const double v = ints[t];
+    if (v == 0.0) continue;
+    const int64_t ij = tri_root(t), kl = (int64_t)(t - (uint64_t)ij * (ij + 1) / 2);
+    const int64_t i = tri_root(ij), j = ij - i * (i + 1) / 2;
+    const int64_t k = tri_root(kl), l = kl - k * (k + 1) / 2;
+    double f = 1.0;
+    if (i == j) f *= 0.5;
+    if (k == l) f *= 0.5;
+    if (ij == kl) f *= 0.5;
+    const double fv = f * v;
+    unsafeAtomicAdd(&Jacc[i * n + j], 2.0 * fv * D[k * n + l]);
+    unsafeAtomicAdd(&Jacc[k * n + l], 2.0 * fv * D[i * n + j]);
+    unsafeAtomicAdd(&Kacc[i * n + k], fv * D[j * n + l]);
+    unsafeAtomicAdd(&Kacc[i * n + l], fv * D[j * n + k]);
+    unsafeAtomicAdd(&Kacc[j * n + k], fv * D[i * n + l]);
+    unsafeAtomicAdd(&Kacc[j * n + l], fv * D[i * n + k]);
+  }
+}
+
+hipError_t launch_digest_stored(int n, const double* ints, uint64_t len, const double* D, double* Jacc,
+                                double* Kacc, hipStream_t st) {
+  if (len == 0) return hipSuccess;
+  uint64_t blocks = (len + 255) / 256;
+  if (blocks > 8192) blocks = 8192;
+  hipLaunchKernelGGL(k_digest_stored, dim3((unsigned)blocks), dim3(256), 0, st, n, ints, len, D, Jacc, Kacc);
+  return hipGetLastError();
+}
+
+hipError_t launch_eri_class(int la, int lb, int lc, int ld, int mode, int fp32, const EngineView& ev,
+                            const Tile* tiles, int ntiles, const int* kets, const double* D, double* Jacc,
+                            double* Kacc, double* ints, double* qout, hipStream_t st) {
+#define PQ_CASE(A, B, C_, D_)                                                                       \
+  if (la == A && lb == B && lc == C_ && ld == D_)                                                   \
+    return launch_cls_##A##B##C_##D_(mode, fp32, ev, tiles, ntiles, kets, D, Jacc, Kacc, ints, qout, st);
+  PQ_CASE(0, 0, 0, 0)
+  PQ_CASE(1, 0, 0, 0)
+  PQ_CASE(1, 0, 1, 0)
+  PQ_CASE(1, 1, 0, 0)
+  PQ_CASE(1, 1, 1, 0)
+  PQ_CASE(1, 1, 1, 1)
+  PQ_CASE(2, 0, 0, 0)
+  PQ_CASE(2, 0, 1, 0)
+  PQ_CASE(2, 0, 1, 1)
+  PQ_CASE(2, 0, 2, 0)
+  PQ_CASE(2, 1, 0, 0)
+  PQ_CASE(2, 1, 1, 0)
+  PQ_CASE(2, 1, 1, 1)
+  PQ_CASE(2, 1, 2, 0)
+  PQ_CASE(2, 1, 2, 1)
+  PQ_CASE(2, 2, 0, 0)
+  PQ_CASE(2, 2, 1, 0)
+  PQ_CASE(2, 2, 1, 1)
+  PQ_CASE(2, 2, 2, 0)
+  PQ_CASE(2, 2, 2, 1)
+  PQ_CASE(2, 2, 2, 2)
+#undef PQ_CASE
+  return hipErrorInvalidValue;
+}
+
+hipError_t launch_symmetrize(int n, const double* Jacc, const double* Kacc, double* J, double* K, double jscale,
+                             double kscale, hipStream_t st) {
+  const int64_t tot = (int64_t)n * n;
+  hipLaunchKernelGGL(k_symmetrize, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, st, n, Jacc, Kacc, J, K,
+                     jscale, kscale);
+  return hipGetLastError();
+}
+
+}  // namespace pq

@@ -1,0 +1,39 @@
+// pq_kernels.h -- launch interface between the host engine and the kernels.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "pq_device.h"
+
+namespace pq {
+
+enum { MODE_JK = 0, MODE_STORE = 1, MODE_DIAG = 2 };
+
+// classes with at most this many Cartesian integrals run one lane per quartet
+constexpr int kLaneMaxComp = 81;
+
+hipError_t launch_eri_class(int la, int lb, int lc, int ld, int mode, int fp32, const EngineView& ev,
+                            const Tile* tiles, int ntiles, const int* kets, const double* D, double* Jacc,
+                            double* Kacc, double* ints, double* qout, hipStream_t st);
+
+hipError_t launch_symmetrize(int n, const double* Jacc, const double* Kacc, double* J, double* K,
+                             double jscale, double kscale, hipStream_t st);
+
+// J/K from a stored flat unique-integral array (Ints.py:195-223 over Ints)
+hipError_t launch_digest_stored(int n, const double* ints, uint64_t len, const double* D, double* Jacc,
+                                double* Kacc, hipStream_t st);
+
+hipError_t launch_fill(double* p, int64_t n, double v, hipStream_t st);
+
+// one-electron S, T, V (full matrices, row-major n x n)
+hipError_t launch_onee(int la, int lb, const EngineView& ev, const int* pair_ids, int npairs,
+                       const double* atom_xyz, const double* atom_z, int natom, double* S, double* T,
+                       double* V, hipStream_t st);
+
+// scalar-compatibility batches
+hipError_t launch_boys(int n, const double* m, const double* x, double eps, double* out, hipStream_t st);
+hipError_t launch_rys_roots(int n, int nr, const double* X, double* u, double* w, hipStream_t st);
+hipError_t launch_contr_eri(int n, const int* nprim, const int* poff, const double* exps,
+                            const double* coefs, const double* norms, const double* origins,
+                            const int* powers, double* out, hipStream_t st);
+
+}  // namespace pq

@@ -1,0 +1,362 @@
+// pq_onee.hip -- one-electron integrals (S, T, V) and the scalar-compatibility
+// kernels behind the drop-in cints/crys/chgp functions.
+#include <math.h>
+
+#include "pq_device.h"
+#include "pq_kernels.h"
+
+namespace pq {
+
+// ---------------------------------------------------------------------
+// Fill (graph-capturable replacement for hipMemsetAsync)
+// ---------------------------------------------------------------------
+__global__ void k_fill(double* p, int64_t n, double v) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (; i < n; i += stride) p[i] = v;
+}
+
+hipError_t launch_fill(double* p, int64_t n, double v, hipStream_t st) {
+  if (n <= 0) return hipSuccess;
+  int64_t blocks = (n + 255) / 256;
+  if (blocks > 65536) blocks = 65536;
+  hipLaunchKernelGGL(k_fill, dim3((unsigned)blocks), dim3(256), 0, st, p, n, v);
+  return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------
+// One-electron integrals over one shell pair (l_A >= l_B), all components:
+//   S_ab (THO eq. 2.12, cints.c:214-239) by the Obara-Saika overlap
+//   recurrence; T_ab by the reference's expression in shifted overlaps
+//   (cints.c:188-212); V_ab = sum_C Z_C <a|-1/r_C|b> (cints.c:241-275) by Rys
+//   quadrature with X = p |P - C|^2.  Full matrices are written (both
+//   triangles).
+// ---------------------------------------------------------------------
+template <int LA, int LB>
+__global__ __launch_bounds__(256) void k_onee(EngineView ev, const int* __restrict__ pair_ids, int npairs,
+                                              const double* __restrict__ atom_xyz,
+                                              const double* __restrict__ atom_z, int natom, double* S,
+                                              double* T, double* V) {
+  constexpr int NA = ncart(LA), NB = ncart(LB);
+  constexpr int NRV = (LA + LB) / 2 + 1;
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= npairs) return;
+  const PairInfo bp = ev.pairs[pair_ids[t]];
+  const double A[3] = {bp.Ax, bp.Ay, bp.Az};
+  const double B[3] = {bp.Ax - bp.ABx, bp.Ay - bp.ABy, bp.Az - bp.ABz};
+  double s[NA][NB], tk[NA][NB], v[NA][NB];
+#pragma unroll
+  for (int a = 0; a < NA; ++a)
+#pragma unroll
+    for (int b = 0; b < NB; ++b) s[a][b] = tk[a][b] = v[a][b] = 0.0;
+
+  for (int ip = 0; ip < bp.pp_n; ++ip) {
+    const PrimPair pp = ev.prims[bp.pp_off + ip];
+    const double p = pp.p, beta = pp.p - pp.a;
+    const double P[3] = {pp.Px, pp.Py, pp.Pz};
+    const double i2p = 0.5 / p;
+    // 1-D overlaps O[d][i][j], i <= LA, j <= LB + 2
+    double O[3][LA + 1][LB + 3];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      const double PA = P[d] - A[d], PB = P[d] - B[d];
+      O[d][0][0] = 1.0;
+#pragma unroll
+      for (int i = 0; i <= LA; ++i) {
+        if (i > 0) {
+          O[d][i][0] = PA * O[d][i - 1][0] + (i > 1 ? (i - 1) * i2p * O[d][i - 2][0] : 0.0);
+        }
+#pragma unroll
+        for (int j = 1; j <= LB + 2; ++j)
+          O[d][i][j] = PB * O[d][i][j - 1] + i2p * ((i > 0 ? i * O[d][i - 1][j - 1] : 0.0) +
+                                                    (j > 1 ? (j - 1) * O[d][i][j - 2] : 0.0));
+      }
+    }
+    const double pre = pp.K * pow(M_PI / p, 1.5);
+#pragma unroll
+    for (int a = 0; a < NA; ++a)
+#pragma unroll
+      for (int b = 0; b < NB; ++b) {
+        const int ax = cart_pow(LA, a, 0), ay = cart_pow(LA, a, 1), az = cart_pow(LA, a, 2);
+        const int bx = cart_pow(LB, b, 0), by = cart_pow(LB, b, 1), bz = cart_pow(LB, b, 2);
+        const double sxyz = O[0][ax][bx] * O[1][ay][by] * O[2][az][bz];
+        s[a][b] += pre * sxyz;
+        double t0 = beta * (2 * (bx + by + bz) + 3) * sxyz;
+        double t1 = O[0][ax][bx + 2] * O[1][ay][by] * O[2][az][bz] +
+                    O[0][ax][bx] * O[1][ay][by + 2] * O[2][az][bz] +
+                    O[0][ax][bx] * O[1][ay][by] * O[2][az][bz + 2];
+        double t2 = 0.0;
+        if (bx >= 2) t2 += bx * (bx - 1) * O[0][ax][bx - 2] * O[1][ay][by] * O[2][az][bz];
+        if (by >= 2) t2 += by * (by - 1) * O[0][ax][bx] * O[1][ay][by - 2] * O[2][az][bz];
+        if (bz >= 2) t2 += bz * (bz - 1) * O[0][ax][bx] * O[1][ay][by] * O[2][az][bz - 2];
+        tk[a][b] += pre * (t0 - 2.0 * beta * beta * t1 - 0.5 * t2);
+      }
+    // nuclear attraction
+    const double vpre = -2.0 * M_PI / p * pp.K;
+    for (int c = 0; c < natom; ++c) {
+      const double Cc[3] = {atom_xyz[3 * c], atom_xyz[3 * c + 1], atom_xyz[3 * c + 2]};
+      const double PC[3] = {P[0] - Cc[0], P[1] - Cc[1], P[2] - Cc[2]};
+      const double X = p * (PC[0] * PC[0] + PC[1] * PC[1] + PC[2] * PC[2]);
+      double u[NRV], w[NRV];
+      rys_roots<NRV, double>(X, u, w);
+      const double zc = atom_z[c] * vpre;
+#pragma unroll
+      for (int r = 0; r < NRV; ++r) {
+        const double B10 = i2p * (1.0 - u[r]);
+        double I[3][LA + 1][LB + 1];
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+          const double C00 = (P[d] - A[d]) - u[r] * PC[d];
+          const double AB = A[d] - B[d];
+          double G[LA + LB + 1];
+          G[0] = 1.0;
+          if (LA + LB > 0) G[1] = C00;
+#pragma unroll
+          for (int n = 2; n <= LA + LB; ++n) G[n] = C00 * G[n - 1] + (n - 1) * B10 * G[n - 2];
+          double H[LB + 1][LA + LB + 1];
+#pragma unroll
+          for (int n = 0; n <= LA + LB; ++n) H[0][n] = G[n];
+#pragma unroll
+          for (int j = 1; j <= LB; ++j)
+#pragma unroll
+            for (int n = 0; n <= LA + LB - j; ++n) H[j][n] = H[j - 1][n + 1] + AB * H[j - 1][n];
+#pragma unroll
+          for (int i = 0; i <= LA; ++i)
+#pragma unroll
+            for (int j = 0; j <= LB; ++j) I[d][i][j] = H[j][i];
+        }
+        const double wz = w[r] * zc;
+#pragma unroll
+        for (int a = 0; a < NA; ++a)
+#pragma unroll
+          for (int b = 0; b < NB; ++b)
+            v[a][b] += wz * I[0][cart_pow(LA, a, 0)][cart_pow(LB, b, 0)] *
+                       I[1][cart_pow(LA, a, 1)][cart_pow(LB, b, 1)] *
+                       I[2][cart_pow(LA, a, 2)][cart_pow(LB, b, 2)];
+      }
+    }
+  }
+  const int n = ev.nbf;
+#pragma unroll
+  for (int a = 0; a < NA; ++a)
+#pragma unroll
+    for (int b = 0; b < NB; ++b) {
+      const double nf = cart_norm(LA, a) * cart_norm(LB, b);
+      const int i = bp.bfA + a, j = bp.bfB + b;
+      S[i * n + j] = S[j * n + i] = s[a][b] * nf;
+      T[i * n + j] = T[j * n + i] = tk[a][b] * nf;
+      V[i * n + j] = V[j * n + i] = v[a][b] * nf;
+    }
+}
+
+hipError_t launch_onee(int la, int lb, const EngineView& ev, const int* pair_ids, int npairs,
+                       const double* atom_xyz, const double* atom_z, int natom, double* S, double* T,
+                       double* V, hipStream_t st) {
+  if (npairs <= 0) return hipSuccess;
+  dim3 g((npairs + 127) / 128), b(128);
+#define PQ_OCASE(A, B_)                                                                                 \
+  if (la == A && lb == B_) {                                                                            \
+    hipLaunchKernelGGL((k_onee<A, B_>), g, b, 0, st, ev, pair_ids, npairs, atom_xyz, atom_z, natom, S, T, \
+                       V);                                                                              \
+    return hipGetLastError();                                                                           \
+  }
+  PQ_OCASE(0, 0)
+  PQ_OCASE(1, 0)
+  PQ_OCASE(1, 1)
+  PQ_OCASE(2, 0)
+  PQ_OCASE(2, 1)
+  PQ_OCASE(2, 2)
+#undef PQ_OCASE
+  return hipErrorInvalidValue;
+}
+
+// ---------------------------------------------------------------------
+// Boys function F_m(x) = 1/2 x^(-m-1/2) gamma(m+1/2, x) by the incomplete
+// gamma series (x < m + 3/2) or Lentz continued fraction, as the
+// reference's Fgamma does (cints.c:340-411); eps is the relative stopping
+// tolerance (3e-7 reproduces the reference, 1e-16 gives full precision).
+// ---------------------------------------------------------------------
+__global__ void k_boys(int n, const double* m_, const double* x_, double eps, int itmax, double* out) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  const double m = m_[t];
+  double x = x_[t];
+  if (fabs(x) < 1e-8) x = 1e-8;
+  if (eps < 1e-10 && (x < 50.0 || m == floor(m))) {
+    // full-precision mode: F_m(x) = e^-x sum_k (2x)^k / prod_{j<=k}(2m+2j+1)
+    // below x = 50; upward recursion from F_0 = sqrt(pi/x) erf(sqrt x)/2 above
+    // (stable there: (2m+1)/(2x) < 1 for the orders in use)
+    if (x < 50.0) {
+      double term = 1.0 / (2.0 * m + 1.0), sum = term;
+      for (int k = 1; k < 400; ++k) {
+        term *= 2.0 * x / (2.0 * m + 2.0 * k + 1.0);
+        sum += term;
+        if (term < sum * 1e-17) break;
+      }
+      out[t] = exp(-x) * sum;
+    } else {
+      const double ex = exp(-x);
+      double f = 0.5 * sqrt(M_PI / x) * erf(sqrt(x));
+      for (int k = 0; k < (int)m; ++k) f = ((2.0 * k + 1.0) * f - ex) / (2.0 * x);
+      out[t] = f;
+    }
+    return;
+  }
+  const double a = m + 0.5;
+  const double gln = lgamma(a);
+  double gam;
+  if (x < a + 1.0) {
+    double ap = a, del = 1.0 / a, sum = del;
+    for (int k = 1; k <= itmax; ++k) {
+      ap += 1.0;
+      del *= x / ap;
+      sum += del;
+      if (fabs(del) < fabs(sum) * eps) break;
+    }
+    gam = exp(gln) * sum * exp(-x + a * log(x) - gln);
+  } else {
+    const double fpmin = 1e-30;
+    double b = x + 1.0 - a, c = 1.0 / fpmin, d = 1.0 / b, h = d;
+    for (int i = 1; i <= itmax; ++i) {
+      const double an = -i * (i - a);
+      b += 2.0;
+      d = an * d + b;
+      if (fabs(d) < fpmin) d = fpmin;
+      c = b + an / c;
+      if (fabs(c) < fpmin) c = fpmin;
+      d = 1.0 / d;
+      const double del = d * c;
+      h *= del;
+      if (fabs(del - 1.0) < eps) break;
+    }
+    gam = exp(gln) * (1.0 - exp(-x + a * log(x) - gln) * h);
+  }
+  out[t] = 0.5 * pow(x, -m - 0.5) * gam;
+}
+
+hipError_t launch_boys(int n, const double* m, const double* x, double eps, double* out, hipStream_t st) {
+  if (n <= 0) return hipSuccess;
+  const int itmax = eps < 1e-10 ? 2000 : 100;
+  hipLaunchKernelGGL(k_boys, dim3((n + 127) / 128), dim3(128), 0, st, n, m, x, eps, itmax, out);
+  return hipGetLastError();
+}
+
+__global__ void k_rys_roots(int n, int nr, const double* X, double* u, double* w) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  double uu[5], ww[5];
+  rys_roots_rt<double>(nr, X[t], uu, ww);
+  for (int i = 0; i < nr; ++i) {
+    u[t * nr + i] = uu[i];
+    w[t * nr + i] = ww[i];
+  }
+}
+
+hipError_t launch_rys_roots(int n, int nr, const double* X, double* u, double* w, hipStream_t st) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_rys_roots, dim3((n + 127) / 128), dim3(128), 0, st, n, nr, X, u, w);
+  return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------
+// Contracted ERI of four arbitrary Cartesian functions (crys.c:34-97
+// semantics): the batch behind crys/cints/chgp contr_coulomb and
+// coulomb_repulsion.  Runtime angular momentum, L_total <= 9.
+// ---------------------------------------------------------------------
+__device__ double rys_1d_rt(double u, int i, int j, int k, int l, double xi, double xj, double xk, double xl,
+                            double ai, double aj, double ak, double al) {
+  double G[10][10];
+  const int n = i + j, m = k + l;
+  const double A = ai + aj, B = ak + al;
+  const double Px = (ai * xi + aj * xj) / A, Qx = (ak * xk + al * xl) / B;
+  const double fff = u / (A + B);
+  const double B00 = 0.5 * fff, B1 = (1 - B * fff) / (2 * A), B1p = (1 - A * fff) / (2 * B);
+  const double C = (Px - xi) + B * (Qx - Px) * fff, Cp = (Qx - xk) + A * (Px - Qx) * fff;
+  G[0][0] = 1.0;
+  if (n > 0) G[1][0] = C;
+  if (m > 0) G[0][1] = Cp;
+  for (int a = 2; a <= n; a++) G[a][0] = B1 * (a - 1) * G[a - 2][0] + C * G[a - 1][0];
+  for (int b = 2; b <= m; b++) G[0][b] = B1p * (b - 1) * G[0][b - 2] + Cp * G[0][b - 1];
+  if (m > 0 && n > 0)
+    for (int a = 1; a <= n; a++) {
+      G[a][1] = a * B00 * G[a - 1][0] + Cp * G[a][0];
+      for (int b = 2; b <= m; b++) G[a][b] = B1p * (b - 1) * G[a][b - 2] + a * B00 * G[a - 1][b - 1] + Cp * G[a][b - 1];
+    }
+  // transfer I(i,j,k,l) = sum binom(j,n) (xi-xj)^(j-n) binom(l,m) (xk-xl)^(l-m) G(n+i, m+k)
+  const double xij = xi - xj, xkl = xk - xl;
+  double res = 0.0;
+  for (int mm = l; mm >= 0; --mm) {
+    // binomial(l, mm) * xkl^(l-mm)
+    double bl = 1.0;
+    for (int q = 0; q < l - mm; ++q) bl *= xkl * (double)(l - q) / (double)(q + 1);
+    double ijm0 = 0.0;
+    for (int nn = j; nn >= 0; --nn) {
+      double bj = 1.0;
+      for (int q = 0; q < j - nn; ++q) bj *= xij * (double)(j - q) / (double)(q + 1);
+      ijm0 += bj * G[nn + i][mm + k];
+    }
+    res += bl * ijm0;
+  }
+  return res;
+}
+
+__global__ void k_contr_eri(int n, const int* __restrict__ nprim, const int* __restrict__ poff,
+                            const double* __restrict__ exps, const double* __restrict__ coefs,
+                            const double* __restrict__ norms, const double* __restrict__ org,
+                            const int* __restrict__ pw, double* out) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  int np[4], po[4], p[4][3];
+  double o[4][3];
+  int L = 0;
+  for (int f = 0; f < 4; ++f) {
+    np[f] = nprim[4 * t + f];
+    po[f] = poff[4 * t + f];
+    for (int d = 0; d < 3; ++d) {
+      p[f][d] = pw[12 * t + 3 * f + d];
+      o[f][d] = org[12 * t + 3 * f + d];
+      L += p[f][d];
+    }
+  }
+  const int nr = L / 2 + 1;
+  double val = 0.0;
+  for (int i = 0; i < np[0]; ++i)
+    for (int j = 0; j < np[1]; ++j)
+      for (int k = 0; k < np[2]; ++k)
+        for (int l = 0; l < np[3]; ++l) {
+          const double a0 = exps[po[0] + i], a1 = exps[po[1] + j], a2 = exps[po[2] + k], a3 = exps[po[3] + l];
+          const double A = a0 + a1, B = a2 + a3, rho = A * B / (A + B);
+          double rpq2 = 0.0, kab = 0.0;
+          for (int d = 0; d < 3; ++d) {
+            const double P = (a0 * o[0][d] + a1 * o[1][d]) / A, Q = (a2 * o[2][d] + a3 * o[3][d]) / B;
+            rpq2 += (P - Q) * (P - Q);
+            kab += a0 * a1 / A * (o[0][d] - o[1][d]) * (o[0][d] - o[1][d]) +
+                   a2 * a3 / B * (o[2][d] - o[3][d]) * (o[2][d] - o[3][d]);
+          }
+          double u[5], w[5];
+          rys_roots_rt<double>(nr, rho * rpq2, u, w);
+          double sum = 0.0;
+          for (int r = 0; r < nr; ++r) {
+            double prod = w[r];
+            for (int d = 0; d < 3; ++d)
+              prod *= rys_1d_rt(u[r], p[0][d], p[1][d], p[2][d], p[3][d], o[0][d], o[1][d], o[2][d], o[3][d],
+                                a0, a1, a2, a3);
+            sum += prod;
+          }
+          const double pre = 2.0 * pow(M_PI, 2.5) / (A * B * sqrt(A + B)) * exp(-kab);
+          val += coefs[po[0] + i] * coefs[po[1] + j] * coefs[po[2] + k] * coefs[po[3] + l] * norms[po[0] + i] *
+                 norms[po[1] + j] * norms[po[2] + k] * norms[po[3] + l] * pre * sum;
+        }
+  out[t] = val;
+}
+
+hipError_t launch_contr_eri(int n, const int* nprim, const int* poff, const double* exps, const double* coefs,
+                            const double* norms, const double* origins, const int* powers, double* out,
+                            hipStream_t st) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_contr_eri, dim3((n + 63) / 64), dim3(64), 0, st, n, nprim, poff, exps, coefs, norms,
+                     origins, powers, out);
+  return hipGetLastError();
+}
+
+}  // namespace pq

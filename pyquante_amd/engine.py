@@ -1,0 +1,195 @@
+"""Engine: one basis on one molecule, resident on the GPU.
+
+Holds the libpqhip engine handle (shell data, shell-pair SoA in HBM and the
+Schwarz-sorted class work lists).  Density, J, K, S, T, V and forces are
+torch tensors on the current HIP device; the engine reads and writes them
+through their device pointers.
+"""
+import ctypes
+
+import numpy as np
+import torch
+
+from . import _abi
+from ._abi import check, lib, pd, pi
+from .basis import sym2l
+
+
+def shells_of(bfs):
+    """Shell list from a BasisSet, or rebuilt from a flat list of CGBFs by
+    grouping runs of functions with the same centre and primitives that form
+    a complete Cartesian set in PyQuante order."""
+    if hasattr(bfs, "shells"):
+        return [(sh.L, sh.exps, sh.coefs, sh.atid, sh.origin) for sh in bfs.shells]
+    from .basis import sym2powerlist
+    order = {L: sym2powerlist[s] for s, L in sym2l.items()}
+    out = []
+    i = 0
+    bl = list(bfs)
+    while i < len(bl):
+        f = bl[i]
+        L = sum(f.powers)
+        comps = order.get(L)
+        if comps is None:
+            raise ValueError("only s, p, d functions are supported")
+        group = bl[i:i + len(comps)]
+        if (len(group) != len(comps) or [tuple(g.powers) for g in group] != comps or
+                any(g.origin != f.origin or g.pexps != f.pexps or g.pcoefs != f.pcoefs for g in group)):
+            raise ValueError("basis functions %d.. do not form a complete shell in PyQuante order" % i)
+        out.append((L, list(f.pexps), list(f.pcoefs), getattr(f, "atid", 0), f.origin))
+        i += len(comps)
+    return out
+
+
+def _device():
+    _abi.require_gpu()
+    return torch.device("cuda", torch.cuda.current_device())
+
+
+def _stream():
+    return ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+
+
+class Engine(object):
+    """Engine(bfs, atoms): bfs is a BasisSet (or CGBF list), atoms a Molecule
+    or a list of Atom."""
+
+    def __init__(self, bfs, atoms):
+        self.device = _device()
+        atoms = list(atoms)
+        self.natom = len(atoms)
+        self.atom_z = np.array([a.atno for a in atoms], dtype=np.float64)
+        xyz = np.array([a.pos() for a in atoms], dtype=np.float64)
+        shells = shells_of(bfs)
+        # map each shell to an atom by its centre
+        self._shell_atom = []
+        for (L, e, c, atid, origin) in shells:
+            d = np.sum((xyz - np.array(origin)) ** 2, axis=1)
+            k = int(np.argmin(d))
+            if d[k] > 1e-20:
+                raise ValueError("shell centre does not coincide with an atom")
+            self._shell_atom.append(k)
+        self.shell_l = np.array([s[0] for s in shells], dtype=np.int32)
+        nprim = [len(s[1]) for s in shells]
+        self.shell_nprim = np.array(nprim, dtype=np.int32)
+        self.shell_poff = np.concatenate([[0], np.cumsum(nprim)[:-1]]).astype(np.int32)
+        self.shell_atom = np.array(self._shell_atom, dtype=np.int32)
+        self.exps = np.array([e for s in shells for e in s[1]], dtype=np.float64)
+        self.coefs = np.array([c for s in shells for c in s[2]], dtype=np.float64)
+        h = ctypes.c_void_p()
+        check(lib.pq_engine_create(len(shells), pi(self.shell_l), pi(self.shell_nprim), pi(self.shell_poff),
+                                   pi(self.shell_atom), pd(self.exps), pd(self.coefs), self.natom,
+                                   pd(np.ascontiguousarray(xyz.reshape(-1))), pd(self.atom_z),
+                                   ctypes.byref(h)))
+        self.handle = h
+        n = ctypes.c_int()
+        check(lib.pq_engine_nbf(h, ctypes.byref(n)))
+        self.nbf = n.value
+        self.xyz = xyz
+        self.cut = None
+        self.fp32cut = None
+
+    def __del__(self):
+        h = getattr(self, "handle", None)
+        if h is not None and h.value:
+            lib.pq_engine_destroy(h)
+            self.handle = None
+
+    # ------------------------------------------------------------------
+    def set_geometry(self, xyz):
+        xyz = np.ascontiguousarray(np.asarray(xyz, dtype=np.float64).reshape(self.natom, 3))
+        check(lib.pq_engine_set_geometry(self.handle, pd(xyz.reshape(-1))))
+        self.xyz = xyz
+        self.cut = self.fp32cut = None
+
+    def build(self, schwarz_cut=1e-12, fp32_cut=0.0):
+        """Shell pairs, Schwarz bounds and work lists (pq_build_pairs)."""
+        check(lib.pq_build_pairs(self.handle, float(schwarz_cut), float(fp32_cut)))
+        self.cut = schwarz_cut
+        self.fp32cut = fp32_cut
+        return self
+
+    def stats(self):
+        counts = (ctypes.c_uint64 * 6)()
+        flops = (ctypes.c_double * 2)()
+        check(lib.pq_engine_stats(self.handle, counts, flops))
+        return {"quartets_fp64": counts[0], "quartets_fp32": counts[1], "prim_quartets_fp64": counts[2],
+                "prim_quartets_fp32": counts[3], "fquartets_fp64": counts[4], "fquartets_fp32": counts[5],
+                "flops_fp64": flops[0], "flops_fp32": flops[1]}
+
+    def class_stats(self):
+        """Per-class counts and algorithmic flops (pq_engine_class_stats)."""
+        out = []
+        for c in range(21):
+            l = (ctypes.c_int * 4)()
+            counts = (ctypes.c_uint64 * 6)()
+            flops = (ctypes.c_double * 2)()
+            check(lib.pq_engine_class_stats(self.handle, c, l, counts, flops))
+            out.append({"class": tuple(l), "quartets_fp64": counts[0], "quartets_fp32": counts[1],
+                        "prim_quartets_fp64": counts[2], "prim_quartets_fp32": counts[3],
+                        "fquartets_fp64": counts[4], "fquartets_fp32": counts[5],
+                        "flops_fp64": flops[0], "flops_fp32": flops[1]})
+        return out
+
+    def fock_jk_timed(self, D, precision=_abi.PQ_PREC_FP64):
+        """fock_jk with per-class kernel times in ms (HIP events on the
+        current stream); returns (J, K, ms[43])."""
+        D = self._dtensor(D)
+        J = torch.empty_like(D)
+        K = torch.empty_like(D)
+        ms = (ctypes.c_float * 43)()
+        check(lib.pq_fock_jk_timed(self.handle, ctypes.c_void_p(D.data_ptr()), ctypes.c_void_p(J.data_ptr()),
+                                   ctypes.c_void_p(K.data_ptr()), int(precision), _stream(), ms))
+        return J, K, list(ms)
+
+    def _dtensor(self, D):
+        if not torch.is_tensor(D):
+            D = torch.as_tensor(np.asarray(D, dtype=np.float64))
+        D = D.to(device=self.device, dtype=torch.float64).contiguous()
+        if D.shape != (self.nbf, self.nbf):
+            raise ValueError("density must be %d x %d" % (self.nbf, self.nbf))
+        return D
+
+    def fock_jk(self, D, want_j=True, want_k=True, precision=_abi.PQ_PREC_FP64):
+        """(J, K) for a symmetric density D in direct mode (pq_fock_jk)."""
+        if self.cut is None:
+            self.build()
+        D = self._dtensor(D)
+        J = torch.empty_like(D) if want_j else None
+        K = torch.empty_like(D) if want_k else None
+        check(lib.pq_fock_jk(self.handle, ctypes.c_void_p(D.data_ptr()),
+                             ctypes.c_void_p(J.data_ptr() if J is not None else 0),
+                             ctypes.c_void_p(K.data_ptr() if K is not None else 0), int(precision),
+                             _stream()))
+        return J, K
+
+    def get2ints(self):
+        """The reference's flat unique-ERI array as a device tensor (pq_get2ints)."""
+        n = self.nbf
+        npair = n * (n + 1) // 2
+        length = npair * (npair + 1) // 2
+        out = torch.empty(length, dtype=torch.float64, device=self.device)
+        check(lib.pq_get2ints(self.handle, ctypes.c_void_p(out.data_ptr()), ctypes.c_uint64(length), _stream()))
+        return out
+
+    def get1ints(self):
+        """(S, T, V) device tensors (pq_get1ints); h = T + V."""
+        n = self.nbf
+        S = torch.empty((n, n), dtype=torch.float64, device=self.device)
+        T = torch.empty_like(S)
+        V = torch.empty_like(S)
+        check(lib.pq_get1ints(self.handle, ctypes.c_void_p(S.data_ptr()), ctypes.c_void_p(T.data_ptr()),
+                              ctypes.c_void_p(V.data_ptr()), _stream()))
+        return S, T, V
+
+
+def digest_stored(ints, D, want_j=True, want_k=True):
+    """J, K from a flat unique-ERI array (device tensor) and density D."""
+    n = D.shape[0]
+    J = torch.empty_like(D) if want_j else None
+    K = torch.empty_like(D) if want_k else None
+    check(lib.pq_digest_stored(n, ctypes.c_void_p(ints.data_ptr()), ctypes.c_uint64(ints.numel()),
+                               ctypes.c_void_p(D.data_ptr()),
+                               ctypes.c_void_p(J.data_ptr() if J is not None else 0),
+                               ctypes.c_void_p(K.data_ptr() if K is not None else 0), _stream()))
+    return J, K

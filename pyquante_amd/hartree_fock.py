@@ -1,0 +1,143 @@
+"""Closed-shell Hartree-Fock driver on the GPU engine.
+
+Mirrors PyQuante/hartree_fock.py:66-149 (rhf), LA2.py:47-131 (geigh with
+Cholesky orthogonalisation, mkdens, trace2) and Convergence.py:73-136
+(Pulay DIIS): the same options (ConvCriteria, MaxIter, DoAveraging, bfs,
+basis_data, integrals, orbs), the same iteration (D = C_occ C_occ^T,
+F = h + 2J - K, E = tr(D h) + tr(D F) + E_nuc, stop when |E - E_old| <
+ConvCriteria with E_old starting at 0) and the same defaults (DIIS off).
+All matrices are FP64 torch tensors on the GPU; the two-electron part is
+the engine's direct Fock build.
+"""
+import logging
+
+import numpy as np
+import torch
+
+from . import _abi
+from .Ints import Integrals, getbasis
+from .engine import Engine
+
+logger = logging.getLogger("pyquante")
+
+
+def _t(x, dev):
+    if torch.is_tensor(x):
+        return x.to(dev, torch.float64)
+    return torch.as_tensor(np.asarray(x, dtype=np.float64), device=dev)
+
+
+def chol_orth(S):
+    """X = inv(L)^T with L L^T = S (LA2.py:118-121)."""
+    L = torch.linalg.cholesky(S)
+    return torch.linalg.inv(L).T
+
+
+def geigh(H, S, X=None):
+    """Generalised eigenproblem H C = S C e by Cholesky orthogonalisation
+    (LA2.py:47-75).  Returns (eigenvalues, eigenvectors as columns)."""
+    if X is None:
+        X = chol_orth(S)
+    e, v = torch.linalg.eigh(X.T @ H @ X)
+    return e, X @ v
+
+
+def mkdens(c, nstart, nstop):
+    """D = C[:, nstart:nstop] C[:, nstart:nstop]^T (LA2.py:127-131)."""
+    d = c[:, nstart:nstop]
+    return d @ d.T
+
+
+def trace2(A, B):
+    return float(torch.sum(A * B))
+
+
+def get_energy(h, F, D, enuke=0.0):
+    """E = tr(D h) + tr(D F) + E_nuc (hartree_fock.py:33-38)."""
+    return trace2(D, h) + trace2(D, F) + enuke
+
+
+class DIIS(object):
+    """Pulay DIIS on the commutator error FDS - SDF (Convergence.py:73-136)."""
+
+    def __init__(self, S):
+        self.Fs, self.Errs, self.S = [], [], S
+        self.maxerr = None
+
+    def getF(self, F, D):
+        err = (F @ D @ self.S - self.S @ D @ F).reshape(-1)
+        self.maxerr = float(err.abs().max())
+        self.Fs.append(F)
+        self.Errs.append(err)
+        nit = len(self.Errs)
+        E = torch.stack(self.Errs)
+        a = torch.zeros((nit + 1, nit + 1), dtype=torch.float64, device=F.device)
+        a[:nit, :nit] = E @ E.T
+        a[nit, :nit] = -1.0
+        a[:nit, nit] = -1.0
+        b = torch.zeros(nit + 1, dtype=torch.float64, device=F.device)
+        b[nit] = -1.0
+        try:
+            c = torch.linalg.solve(a, b)
+            if not bool(torch.isfinite(c).all()):
+                raise RuntimeError("singular DIIS system")
+        except RuntimeError:
+            return F
+        return torch.einsum("i,ijk->ijk", c[:nit], torch.stack(self.Fs)).sum(0)
+
+
+def rhf(atoms, **opts):
+    """rhf(atoms, **opts) -> (energy, orbe, orbs) (hartree_fock.py:66-149)."""
+    ConvCriteria = opts.get("ConvCriteria", 1e-4)
+    MaxIter = opts.get("MaxIter", 20)
+    DoAveraging = opts.get("DoAveraging", False)
+    precision = opts.get("precision", _abi.PQ_PREC_FP64)
+    bfs = opts.get("bfs", None)
+    if not bfs:
+        bfs = getbasis(atoms, opts.get("basis_data", None))
+    nclosed, nopen = atoms.get_closedopen()
+    if nopen:
+        raise ValueError("SCF currently only works for closed-shell systems")
+    nocc = nclosed
+    integrals = opts.get("integrals", None)
+    engine = None
+    if integrals:
+        S, h, Ints = integrals
+        dev = Ints.engine.device if isinstance(Ints, Integrals) else torch.device("cuda")
+    else:
+        engine = Engine(bfs, atoms)
+        dev = engine.device
+        Sd, Td, Vd = engine.get1ints()
+        S, h = Sd, Td + Vd
+        Ints = Integrals(bfs, atoms, engine=engine)
+    S = _t(S, dev)
+    h = _t(h, dev)
+    X = chol_orth(S)
+    orbs = opts.get("orbs", None)
+    if orbs is None:
+        orbe, orbs = geigh(h, S, X)
+    else:
+        orbs = _t(orbs, dev)
+    enuke = atoms.get_enuke()
+    eold = 0.0
+    avg = DIIS(S) if DoAveraging else None
+    energy = 0.0
+    for i in range(MaxIter):
+        D = mkdens(orbs, 0, nocc)
+        if precision == _abi.PQ_PREC_FP64:
+            from .Ints import get2JmK
+            G = get2JmK(Ints, D)
+        else:
+            Ints.engine.build(opts.get("schwarz_cut", 1e-12), opts.get("fp32_cut", 1e-4))
+            J, K = Ints.engine.fock_jk(0.5 * (D + D.T), precision=precision)
+            G = 2.0 * J - K
+        F = h + G
+        if avg is not None:
+            F = avg.getF(F, D)
+        orbe, orbs = geigh(F, S, X)
+        energy = get_energy(h, F, D, enuke)
+        logger.debug("%d %f", i, energy)
+        if abs(energy - eold) < ConvCriteria:
+            break
+        eold = energy
+    return energy, orbe.cpu().numpy(), orbs.cpu().numpy()

@@ -1,0 +1,238 @@
+"""GPU parity: the HIP engine against the oracle and the compiled reference's
+golden vectors.  Tolerances: FP64 integrals 1e-12 absolute (the reference's
+own roots are ~1e-13 relative), Boys 1e-13 relative, J/K 1e-10 absolute."""
+import math
+
+import numpy as np
+import pytest
+import torch
+
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+def _h2o_631ss():
+    from pyquante_amd.molecule import Molecule
+    r = 1 / 0.52918
+    return Molecule("h2o", [(8, (0, 0, 0)), (1, (r, 0, 0)), (1, (0, r, 0))])
+
+
+def test_rys_roots_table_vs_oracle():
+    from pyquante_amd import _abi
+    rng = np.random.default_rng(0)
+    X = np.concatenate([[0.0, 1e-9, 0.5, 1.0, 36.999, 37.0, 63.99, 64.0, 1e3, 1e6],
+                        rng.uniform(0, 80, 400)])
+    for nr in range(1, 6):
+        u = np.zeros(len(X) * nr)
+        w = np.zeros(len(X) * nr)
+        _abi.check(_abi.lib.pq_rys_roots_batch(len(X), nr, _abi.pd(X), _abi.pd(u), _abi.pd(w)))
+        u = u.reshape(-1, nr)
+        w = w.reshape(-1, nr)
+        for t, x in enumerate(X):
+            uo, wo = O.rys_roots(nr, x)
+            assert np.allclose(u[t], uo, rtol=5e-14, atol=0), (nr, x)
+            assert np.allclose(w[t], wo, rtol=5e-14, atol=0), (nr, x)
+
+
+def test_boys_batch(golden):
+    from pyquante_amd import _abi
+    g = golden("boys_nr")
+    m, x = np.meshgrid(g["m"], g["x"], indexing="ij")
+    m = np.ascontiguousarray(m.ravel())
+    x = np.ascontiguousarray(x.ravel())
+    out = np.zeros_like(m)
+    _abi.check(_abi.lib.pq_boys_batch(len(m), _abi.pd(m), _abi.pd(x), 1, _abi.pd(out)))
+    assert np.allclose(out, g["F"].ravel(), rtol=1e-12, atol=0)
+    _abi.check(_abi.lib.pq_boys_batch(len(m), _abi.pd(m), _abi.pd(x), 0, _abi.pd(out)))
+    ref = np.array([O.boys(a, b) for a, b in zip(m, x)])
+    mask = x > 1e-8  # the reference clamps x to 1e-8
+    assert np.allclose(out[mask], ref[mask], rtol=2e-14, atol=0)
+    from pyquante_amd import cints
+    assert abs(cints.Fgamma(0, 1) - 0.7468241328124271) < 1e-8      # TestMath.c:19
+    assert abs(cints.Fgamma(0.5, 6.8) - 0.073447516533246701) < 1e-8  # TestMath.c:20
+
+
+def test_prim_classes_vs_compiled_crys(golden):
+    from pyquante_amd._compat import contr_coulomb_batch
+    g = golden("prim_classes")
+    P, Org, A = g["powers"], g["origins"], g["alphas"]
+    quartets = [tuple(([A[t, q]], [1.0], [1.0], Org[t, 3 * q:3 * q + 3], P[t, 3 * q:3 * q + 3])
+                      for q in range(4)) for t in range(len(P))]
+    v = contr_coulomb_batch(quartets)
+    assert np.allclose(v, g["crys"], rtol=1e-12, atol=1e-14)
+
+
+def test_get2ints_h2o_sto3g(golden):
+    from pyquante_amd.Ints import get2ints
+    from pyquante_amd.basis import BasisSet
+    from pyquante_amd.molecule import h2o_sto3g_config
+    mol = h2o_sto3g_config()
+    ints = get2ints(BasisSet(mol, "sto-3g"), mol)
+    assert len(ints) == 406
+    assert np.max(np.abs(np.asarray(ints) - golden("h2o_sto3g")["crys"])) < 1e-12
+
+
+def test_get2ints_h2o_631ss(golden):
+    from pyquante_amd.Ints import get2ints
+    from pyquante_amd.basis import BasisSet
+    mol = _h2o_631ss()
+    ints = get2ints(BasisSet(mol, "6-31g**"), mol)
+    assert len(ints) == 52975
+    ref = golden("h2o_631ss")["crys"]
+    assert np.max(np.abs(np.asarray(ints) - ref)) < 1e-12
+
+
+def test_direct_and_stored_jk_vs_oracle(golden):
+    from pyquante_amd import Ints
+    from pyquante_amd.basis import BasisSet
+    mol = _h2o_631ss()
+    bs = BasisSet(mol, "6-31g**")
+    ref_ints = golden("h2o_631ss")["crys"]
+    n = len(bs)
+    rng = np.random.default_rng(4)
+    D = rng.standard_normal((n, n)) * 0.1
+    D = D + D.T
+    Jo, Ko, Go = O.getJK(ref_ints, D)
+    old = Ints.schwarz_cut
+    Ints.schwarz_cut = 0.0
+    try:
+        S, h, I = Ints.getints(bs, mol)
+        assert np.max(np.abs(Ints.getJ(I, D) - Jo)) < 1e-10
+        assert np.max(np.abs(Ints.getK(I, D) - Ko)) < 1e-10
+        assert np.max(np.abs(Ints.get2JmK(I, D) - Go)) < 1e-10
+    finally:
+        Ints.schwarz_cut = old
+    # a plain flat array in the reference layout is digested on the GPU
+    assert np.max(np.abs(Ints.getJ(ref_ints, D) - Jo)) < 1e-10
+    assert np.max(np.abs(Ints.getK(list(ref_ints), D) - Ko)) < 1e-10
+
+
+def test_get1ints_vs_oracle(golden):
+    from pyquante_amd.Ints import get1ints
+    from pyquante_amd.basis import BasisSet
+    from pyquante_amd.molecule import h2o_sto3g_config
+    for mol, name in ((h2o_sto3g_config(), "sto-3g"), (_h2o_631ss(), "6-31g**")):
+        bs = BasisSet(mol, name)
+        S, h = get1ints(bs, mol)
+        So, ho = O.get1ints(bs.bfs, mol.atoms)
+        assert np.max(np.abs(S - So)) < 1e-12
+        assert np.max(np.abs(h - ho)) < 1e-11
+    g = golden("h2o_sto3g")
+    mol = h2o_sto3g_config()
+    S, h = get1ints(BasisSet(mol, "sto-3g"), mol)
+    assert np.max(np.abs(S - g["S"])) < 1e-12
+    # the reference's V carries the NR Boys error (EPS = 3e-7)
+    assert np.max(np.abs(h - (g["T"] + g["V"]))) < 1e-6
+
+
+def test_rhf_h2o_sto3g_vs_oracle():
+    from oracle import scf
+    from pyquante_amd.basis import BasisSet
+    from pyquante_amd.hartree_fock import rhf
+    from pyquante_amd.molecule import h2o_sto3g_config
+    mol = h2o_sto3g_config()
+    bs = BasisSet(mol, "sto-3g")
+    e_ref = scf.rhf(bs.bfs, mol, ConvCriteria=1e-10)[0]
+    e = rhf(mol, basis_data="sto-3g", ConvCriteria=1e-10, MaxIter=60)[0]
+    assert abs(e - e_ref) < 1e-9
+
+
+def test_rhf_h2o_631ss_reference_energy():
+    from pyquante_amd.hartree_fock import rhf
+    # Tests/h2o.py:16-23 (rhf defaults, 6-31G**): -76.011751 +- 1e-4
+    e = rhf(_h2o_631ss(), ConvCriteria=1e-8, MaxIter=60)[0]
+    assert abs(e - (-76.011751)) < 1e-4
+    # Tests/WaterSweet.py:16 (DIIS): -76.011755864850628 to 4 places
+    e2 = rhf(_h2o_631ss(), ConvCriteria=1e-8, MaxIter=60, DoAveraging=True)[0]
+    assert abs(e2 - (-76.011755864850628)) < 1e-4
+
+
+def test_water_dimer_direct_vs_stored_vs_oracle():
+    from pyquante_amd.basis import BasisSet
+    from pyquante_amd.engine import Engine, digest_stored
+    from pyquante_amd.molecule import water_cluster
+    mol = water_cluster((2, 1, 1), seed=1)
+    bs = BasisSet(mol, "6-31g**")
+    eng = Engine(bs, mol)
+    n = eng.nbf
+    rng = np.random.default_rng(5)
+    D = rng.standard_normal((n, n)) * 0.05
+    D = D + D.T
+    ints = eng.get2ints()
+    Dt = torch.as_tensor(D, device=eng.device)
+    Js, Ks = digest_stored(ints, Dt)
+    eng.build(0.0)
+    Jd, Kd = eng.fock_jk(Dt)
+    assert torch.max(torch.abs(Js - Jd)).item() < 1e-11
+    assert torch.max(torch.abs(Ks - Kd)).item() < 1e-11
+    Jo, Ko, _ = O.getJK(ints.cpu().numpy(), D)
+    assert np.max(np.abs(Jd.cpu().numpy() - Jo)) < 1e-10
+    assert np.max(np.abs(Kd.cpu().numpy() - Ko)) < 1e-10
+    # unscreened work lists cover every unique shell quartet exactly once
+    st = eng.stats()
+    npair = len(bs.shells) * (len(bs.shells) + 1) // 2
+    assert st["quartets_fp64"] == npair * (npair + 1) // 2
+
+
+def test_shim_modules_match_oracle():
+    from pyquante_amd import chgp, cints, crys
+    from pyquante_amd.basis import BasisSet
+    bfs = BasisSet(_h2o_631ss(), "6-31g**").bfs
+    rng = np.random.default_rng(6)
+    for _ in range(20):
+        q = [bfs[i] for i in rng.integers(0, len(bfs), 4)]
+        ref = O.contr_coulomb(*q)
+        args = []
+        for f in q:
+            args += [f.pexps, f.pcoefs, f.pnorms, f.origin, f.powers]
+        for mod in (crys, cints, chgp):
+            assert abs(mod.contr_coulomb(*args) - ref) < 1e-12
+    a = bfs[5]
+    v = crys.coulomb_repulsion(a.origin, a.pnorms[0], a.powers, a.pexps[0], a.origin, a.pnorms[0], a.powers,
+                               a.pexps[0], a.origin, a.pnorms[0], a.powers, a.pexps[0], a.origin,
+                               a.pnorms[0], a.powers, a.pexps[0])
+    z = O.prim_eri(O.RYS, a.origin, a.pnorms[0], a.powers, a.pexps[0], a.origin, a.pnorms[0], a.powers,
+                   a.pexps[0], a.origin, a.pnorms[0], a.powers, a.pexps[0], a.origin, a.pnorms[0],
+                   a.powers, a.pexps[0])
+    assert abs(v - z) < 1e-12 * abs(z)
+    with pytest.raises(SystemError):  # > 40 primitives in total (crys.c:1633)
+        crys.contr_coulomb(*([[1.0] * 11, [1.0] * 11, [1.0] * 11, (0, 0, 0), (0, 0, 0)] * 4))
+    with pytest.raises(ValueError):  # L_total > 9 has no Rys rule
+        crys.coulomb_repulsion((0, 0, 0), 1, (3, 0, 0), 1, (0, 0, 0), 1, (3, 0, 0), 1, (0, 0, 0), 1, (2, 0, 0), 1,
+                               (0, 0, 0), 1, (2, 0, 0), 1)
+
+
+def test_mixed_precision_close_to_fp64():
+    from pyquante_amd import _abi
+    from pyquante_amd.basis import BasisSet
+    from pyquante_amd.engine import Engine
+    from pyquante_amd.molecule import water_cluster
+    mol = water_cluster((2, 2, 1), seed=2)
+    eng = Engine(BasisSet(mol, "6-31g**"), mol)
+    n = eng.nbf
+    rng = np.random.default_rng(7)
+    C = rng.standard_normal((n, 20)) * 0.1
+    D = torch.as_tensor(C @ C.T, device=eng.device)
+    eng.build(1e-12, 0.0)
+    J64, K64 = eng.fock_jk(D)
+    eng.build(1e-12, 1e-4)
+    st = eng.stats()
+    assert st["quartets_fp32"] > 0
+    J32, K32 = eng.fock_jk(D, precision=_abi.PQ_PREC_MIXED)
+    e64 = float(torch.sum(D * (2 * J64 - K64)))
+    e32 = float(torch.sum(D * (2 * J32 - K32)))
+    assert abs(e64 - e32) < 1e-6
+    assert torch.max(torch.abs(J64 - J32)).item() < 1e-6
+
+
+def test_function_quartet_count_unscreened():
+    from pyquante_amd.basis import BasisSet
+    from pyquante_amd.engine import Engine
+    from pyquante_amd.molecule import water_cluster
+    mol = water_cluster((2, 1, 1), seed=3)
+    bs = BasisSet(mol, "6-31g**")
+    eng = Engine(bs, mol).build(0.0)
+    n = eng.nbf
+    npair = n * (n + 1) // 2
+    assert eng.stats()["fquartets_fp64"] == npair * (npair + 1) // 2
