@@ -72,6 +72,7 @@ def main():
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--classes", action="store_true", help="print per-class kernel times")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py runs on one GPU only: this build has no multi-GPU path "
@@ -125,6 +126,15 @@ def main():
         _, _, ms = eng.fock_jk_timed(D)
         acc += np.array(ms)
     acc /= nt
+    if args.classes:
+        for ci, cc in enumerate(cls):
+            if cc["quartets_fp64"] == 0:
+                continue
+            t = acc[2 * ci] * 1e-3
+            print("  %s quartets %9d prim %11d  %.3f ms  %7.3f TFLOP/s  %6.1f Gquartet/s"
+                  % ("(%s%s|%s%s)" % tuple("spd"[x] for x in cc["class"]), cc["quartets_fp64"],
+                     cc["prim_quartets_fp64"], t * 1e3, cc["flops_fp64"] / t / 1e12 if t > 0 else 0,
+                     cc["fquartets_fp64"] / t / 1e9 if t > 0 else 0), file=sys.stderr)
     k = int(np.argmax(acc[:42:2]))
     c = cls[k]
     ktime = acc[2 * k] * 1e-3

@@ -46,9 +46,9 @@ __host__ __device__ constexpr double cart_norm(int L, int i) {
 // ---------------------------------------------------------------------
 struct __align__(16) PrimPair {
   double p;    // alpha_a + alpha_b
-  double K;    // c_a c_b Nrad_a Nrad_b exp(-alpha_a alpha_b |AB|^2 / p)
+  double rp;   // 1 / p
+  double Kp;   // K / p, K = c_a c_b Nrad_a Nrad_b exp(-alpha_a alpha_b |AB|^2 / p)
   double Px, Py, Pz;
-  double a;    // alpha_a (used by the derivative kernels)
 };
 
 struct __align__(16) PairInfo {
@@ -72,6 +72,7 @@ struct Tile {
 struct EngineView {
   const PairInfo* pairs;
   const PrimPair* prims;
+  const double* prim_a;  // alpha_a of each primitive pair (one-electron and derivative kernels)
   int nbf;
 };
 

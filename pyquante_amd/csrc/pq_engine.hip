@@ -93,12 +93,14 @@ struct pq_engine {
   bool pairs_ok = false;
   std::vector<PairInfo> pairs;
   std::vector<PrimPair> prims;
+  std::vector<double> prim_a;
   std::vector<int> pcls;                 // pair class of each pair
   std::vector<std::vector<int>> members;  // per pair class: pair ids (unsorted)
   std::vector<double> Q;                 // Schwarz bound per pair
   std::vector<std::vector<int>> sorted;   // per pair class: sorted by Q desc
   PairInfo* d_pairs = nullptr;
   PrimPair* d_prims = nullptr;
+  double* d_prim_a = nullptr;
   int* d_members[6] = {};
   int* d_sorted[6] = {};
   double* d_atom_xyz = nullptr;
@@ -113,7 +115,7 @@ struct pq_engine {
   double* d_Jacc = nullptr;
   double* d_Kacc = nullptr;
 
-  EngineView view() const { return EngineView{d_pairs, d_prims, nbf}; }
+  EngineView view() const { return EngineView{d_pairs, d_prims, d_prim_a, nbf}; }
 
   void free_work(std::vector<ClassWork>& w) {
     for (auto& c : w) {
@@ -125,6 +127,7 @@ struct pq_engine {
   void free_pairs() {
     dev_free(d_pairs);
     dev_free(d_prims);
+    dev_free(d_prim_a);
     for (int i = 0; i < 6; ++i) {
       dev_free(d_members[i]);
       dev_free(d_sorted[i]);
@@ -147,6 +150,7 @@ namespace {
 int build_pair_data(pq_engine* e) {
   e->pairs.clear();
   e->prims.clear();
+  e->prim_a.clear();
   e->pcls.clear();
   e->members.assign(6, {});
   for (int A = 0; A < e->nshell; ++A)
@@ -177,12 +181,13 @@ int build_pair_data(pq_engine* e) {
           const double ca = e->coefs[e->sh_off[s1] + i], cb = e->coefs[e->sh_off[s2] + j];
           PrimPair pp{};
           pp.p = a + b;
-          pp.a = a;
+          pp.rp = 1.0 / pp.p;
           pp.Px = (a * c1[0] + b * c2[0]) / pp.p;
           pp.Py = (a * c1[1] + b * c2[1]) / pp.p;
           pp.Pz = (a * c1[2] + b * c2[2]) / pp.p;
-          pp.K = ca * cb * nrad(a, l1) * nrad(b, l2) * std::exp(-a * b / pp.p * ab2);
+          pp.Kp = ca * cb * nrad(a, l1) * nrad(b, l2) * std::exp(-a * b / pp.p * ab2) / pp.p;
           e->prims.push_back(pp);
+          e->prim_a.push_back(a);
         }
       pi.pp_n = (int)e->prims.size() - pi.pp_off;
       const int pc = pair_class(l1, l2);
@@ -192,8 +197,11 @@ int build_pair_data(pq_engine* e) {
     }
   dev_free(e->d_pairs);
   dev_free(e->d_prims);
+  dev_free(e->d_prim_a);
   PQ_HIP(dev_alloc(&e->d_pairs, e->pairs.size()));
   PQ_HIP(dev_alloc(&e->d_prims, e->prims.size()));
+  PQ_HIP(dev_alloc(&e->d_prim_a, e->prim_a.size()));
+  PQ_HIP(hipMemcpy(e->d_prim_a, e->prim_a.data(), e->prim_a.size() * sizeof(double), hipMemcpyHostToDevice));
   PQ_HIP(hipMemcpy(e->d_pairs, e->pairs.data(), e->pairs.size() * sizeof(PairInfo), hipMemcpyHostToDevice));
   PQ_HIP(hipMemcpy(e->d_prims, e->prims.data(), e->prims.size() * sizeof(PrimPair), hipMemcpyHostToDevice));
   for (int c = 0; c < 6; ++c) {
@@ -206,6 +214,7 @@ int build_pair_data(pq_engine* e) {
   // Schwarz bounds from the diagonal quartets (ab|ab)
   double* d_q = nullptr;
   PQ_HIP(dev_alloc(&d_q, e->pairs.size()));
+  PQ_HIP(launch_fill(d_q, (int64_t)e->pairs.size(), 0.0, 0));
   for (int c = 0; c < 6; ++c) {
     const int np = (int)e->members[c].size();
     if (!np) continue;
@@ -214,7 +223,6 @@ int build_pair_data(pq_engine* e) {
     Tile* d_tiles = nullptr;
     PQ_HIP(dev_alloc(&d_tiles, tiles.size()));
     PQ_HIP(hipMemcpy(d_tiles, tiles.data(), tiles.size() * sizeof(Tile), hipMemcpyHostToDevice));
-    // the heavy-class kernel walks a tile's kets sequentially: same tiles work
     hipError_t err = launch_eri_class(kPairL[c][0], kPairL[c][1], kPairL[c][0], kPairL[c][1], MODE_DIAG, 0,
                                       e->view(), d_tiles, (int)tiles.size(), e->d_members[c], nullptr,
                                       nullptr, nullptr, nullptr, d_q, 0);
@@ -229,7 +237,11 @@ int build_pair_data(pq_engine* e) {
   e->sorted.assign(6, {});
   for (int c = 0; c < 6; ++c) {
     auto v = e->members[c];
-    std::stable_sort(v.begin(), v.end(), [&](int a, int b) { return e->Q[a] > e->Q[b]; });
+    // contraction length first (uniform primitive loops within a tile), then Q
+    std::stable_sort(v.begin(), v.end(), [&](int a, int b) {
+      if (e->pairs[a].pp_n != e->pairs[b].pp_n) return e->pairs[a].pp_n > e->pairs[b].pp_n;
+      return e->Q[a] > e->Q[b];
+    });
     e->sorted[c] = v;
     dev_free(e->d_sorted[c]);
     PQ_HIP(dev_alloc(&e->d_sorted[c], v.size()));
@@ -258,8 +270,7 @@ int build_work(pq_engine* e, std::vector<ClassWork>& out, double cut, double fp3
         continue;
       }
       const int la = kPairL[X][0], lb = kPairL[X][1], lc = kPairL[Y][0], ld = kPairL[Y][1];
-      const bool heavy = ncart(la) * ncart(lb) * ncart(lc) * ncart(ld) > kLaneMaxComp;
-      const int tile = heavy ? 4 : 64;
+      const int tile = 64;
       std::vector<double> kq(kets.size());
       std::vector<double> kpre(kets.size() + 1, 0.0), fpre(kets.size() + 1, 0.0);
       auto ufun = [&](int pid, int l1, int l2) -> double {  // unique function pairs of a shell pair
@@ -271,37 +282,47 @@ int build_work(pq_engine* e, std::vector<ClassWork>& out, double cut, double fp3
         kpre[j + 1] = kpre[j] + e->pairs[kets[j]].pp_n;
         fpre[j + 1] = fpre[j] + ufun(kets[j], lc, ld);
       }
-      auto count_ge = [&](double thr) -> int {
-        // kets sorted by Q descending: number with Q >= thr
-        return (int)(std::upper_bound(kq.begin(), kq.end(), thr, [](double t, double q) { return t > q; }) -
-                     kq.begin());
+      // ket groups: runs of equal contraction length, each sorted by Q descending
+      std::vector<int> gs;
+      for (size_t j = 0; j < kets.size(); ++j)
+        if (j == 0 || e->pairs[kets[j]].pp_n != e->pairs[kets[j - 1]].pp_n) gs.push_back((int)j);
+      gs.push_back((int)kets.size());
+      auto count_ge = [&](int g0, int g1, double thr) -> int {  // entries of [g0,g1) with Q >= thr
+        return (int)(std::upper_bound(kq.begin() + g0, kq.begin() + g1, thr,
+                                      [](double t, double q) { return t > q; }) -
+                     (kq.begin() + g0));
       };
       for (size_t i = 0; i < bras.size(); ++i) {
         const double qi = e->Q[bras[i]];
-        int nk;
-        if (cut <= 0.0) nk = (int)kets.size();
-        else if (qi <= 0.0) nk = 0;
-        else nk = count_ge(cut / qi);
-        if (X == Y) nk = std::min(nk, (int)i + 1);
-        int n64 = nk;
-        if (fp32cut > 0.0) n64 = qi <= 0.0 ? 0 : std::min(nk, count_ge(fp32cut / qi));
         const int bpn = e->pairs[bras[i]].pp_n;
-        for (int k = 0; k < n64; k += tile) cw.t64.push_back(Tile{bras[i], k, std::min(tile, n64 - k), 0});
-        for (int k = n64; k < nk; k += tile) cw.t32.push_back(Tile{bras[i], k, std::min(tile, nk - k), 0});
-        cw.nq64 += n64;
-        cw.nq32 += nk - n64;
-        cw.np64 += bpn * (kpre[n64] - kpre[0]);
-        cw.np32 += bpn * (kpre[nk] - kpre[n64]);
-        // unique function quartets; the diagonal quartet (bra == ket, the last
-        // ket of a same-class row) has ua(ua+1)/2 instead of ua^2
         const double ua = ufun(bras[i], la, lb);
-        double f64 = ua * fpre[n64], f32 = ua * (fpre[nk] - fpre[n64]);
-        if (X == Y && nk == (int)i + 1) {
-          const double corr = ua * ua - ua * (ua + 1) / 2.0;
-          if (n64 == nk) f64 -= corr; else f32 -= corr;
+        for (size_t g = 0; g + 1 < gs.size(); ++g) {
+          const int g0 = gs[g], g1 = gs[g + 1];
+          if (X == Y && g0 > (int)i) break;
+          int c;
+          if (cut <= 0.0) c = g1 - g0;
+          else if (qi <= 0.0) c = 0;
+          else c = count_ge(g0, g1, cut / qi);
+          int end = g0 + c;
+          if (X == Y) end = std::min(end, (int)i + 1);
+          int e64 = end;
+          if (fp32cut > 0.0) e64 = qi <= 0.0 ? g0 : std::min(end, g0 + count_ge(g0, g1, fp32cut / qi));
+          for (int k = g0; k < e64; k += tile) cw.t64.push_back(Tile{bras[i], k, std::min(tile, e64 - k), 0});
+          for (int k = e64; k < end; k += tile) cw.t32.push_back(Tile{bras[i], k, std::min(tile, end - k), 0});
+          cw.nq64 += e64 - g0;
+          cw.nq32 += end - e64;
+          cw.np64 += bpn * (kpre[e64] - kpre[g0]);
+          cw.np32 += bpn * (kpre[end] - kpre[e64]);
+          // unique function quartets; the diagonal quartet (bra == ket) has
+          // ua(ua+1)/2 instead of ua^2
+          double f64 = ua * (fpre[e64] - fpre[g0]), f32 = ua * (fpre[end] - fpre[e64]);
+          if (X == Y && end == (int)i + 1 && (int)i >= g0) {
+            const double corr = ua * ua - ua * (ua + 1) / 2.0;
+            if ((int)i < e64) f64 -= corr; else f32 -= corr;
+          }
+          cw.nf64 += f64;
+          cw.nf32 += f32;
         }
-        cw.nf64 += f64;
-        cw.nf32 += f32;
       }
       PQ_HIP(dev_alloc(&cw.d_t64, cw.t64.size()));
       PQ_HIP(dev_alloc(&cw.d_t32, cw.t32.size()));

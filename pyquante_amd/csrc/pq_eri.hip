@@ -6,10 +6,11 @@
 
 namespace pq {
 
-#define PQ_DECL(A, B, C_, D_)                                                                              \
-  hipError_t launch_cls_##A##B##C_##D_(int mode, int fp32, const EngineView& ev, const Tile* tiles, int ntiles, \
-                                       const int* kets, const double* D, double* Jacc, double* Kacc,       \
-                                       double* ints, double* qout, hipStream_t st);
+#define PQ_DECL1(A, B, C_, D_, M)                                                                    \
+  hipError_t launch_cls_##A##B##C_##D_##_m##M(const EngineView& ev, const Tile* tiles, int ntiles,         \
+                                              const int* kets, const double* D, double* Jacc, double* Kacc, \
+                                              double* ints, double* qout, hipStream_t st);
+#define PQ_DECL(A, B, C_, D_) PQ_DECL1(A, B, C_, D_, 0) PQ_DECL1(A, B, C_, D_, 1) PQ_DECL1(A, B, C_, D_, 2)
 PQ_DECL(0, 0, 0, 0)
 PQ_DECL(1, 0, 0, 0)
 PQ_DECL(1, 0, 1, 0)
@@ -31,7 +32,14 @@ PQ_DECL(2, 2, 1, 1)
 PQ_DECL(2, 2, 2, 0)
 PQ_DECL(2, 2, 2, 1)
 PQ_DECL(2, 2, 2, 2)
+PQ_DECL1(0, 0, 0, 0, 3)
+PQ_DECL1(1, 0, 1, 0, 3)
+PQ_DECL1(1, 1, 1, 1, 3)
+PQ_DECL1(2, 0, 2, 0, 3)
+PQ_DECL1(2, 1, 2, 1, 3)
+PQ_DECL1(2, 2, 2, 2, 3)
 #undef PQ_DECL
+#undef PQ_DECL1
 
 // ---------------------------------------------------------------------
 // J = Jacc + Jacc^T, K = Kacc + Kacc^T (outputs may be null).
@@ -91,12 +99,104 @@ hipError_t launch_digest_stored(int n, const double* ints, uint64_t len, const d
   return hipGetLastError();
 }
 
+static hipError_t launch_diag_0000(const EngineView& ev, const Tile* tiles, int ntiles, const int* kets, const double* D,
+                                   double* Jacc, double* Kacc, double* ints, double* qout, hipStream_t st) {
+  return launch_cls_0000_m3(ev, tiles, ntiles, kets, D, Jacc, Kacc, ints, qout, st);
+}
+static hipError_t launch_diag_1000(const EngineView&, const Tile*, int, const int*, const double*, double*, double*,
+                                   double*, double*, hipStream_t) {
+  return hipErrorInvalidValue;
+}
+static hipError_t launch_diag_1010(const EngineView& ev, const Tile* tiles, int ntiles, const int* kets, const double* D,
+                                   double* Jacc, double* Kacc, double* ints, double* qout, hipStream_t st) {
+  return launch_cls_1010_m3(ev, tiles, ntiles, kets, D, Jacc, Kacc, ints, qout, st);
+}
+static hipError_t launch_diag_1100(const EngineView&, const Tile*, int, const int*, const double*, double*, double*,
+                                   double*, double*, hipStream_t) {
+  return hipErrorInvalidValue;
+}
+static hipError_t launch_diag_1110(const EngineView&, const Tile*, int, const int*, const double*, double*, double*,
+                                   double*, double*, hipStream_t) {
+  return hipErrorInvalidValue;
+}
+static hipError_t launch_diag_1111(const EngineView& ev, const Tile* tiles, int ntiles, const int* kets, const double* D,
+                                   double* Jacc, double* Kacc, double* ints, double* qout, hipStream_t st) {
+  return launch_cls_1111_m3(ev, tiles, ntiles, kets, D, Jacc, Kacc, ints, qout, st);
+}
+static hipError_t launch_diag_2000(const EngineView&, const Tile*, int, const int*, const double*, double*, double*,
+                                   double*, double*, hipStream_t) {
+  return hipErrorInvalidValue;
+}
+static hipError_t launch_diag_2010(const EngineView&, const Tile*, int, const int*, const double*, double*, double*,
+                                   double*, double*, hipStream_t) {
+  return hipErrorInvalidValue;
+}
+static hipError_t launch_diag_2011(const EngineView&, const Tile*, int, const int*, const double*, double*, double*,
+                                   double*, double*, hipStream_t) {
+  return hipErrorInvalidValue;
+}
+static hipError_t launch_diag_2020(const EngineView& ev, const Tile* tiles, int ntiles, const int* kets, const double* D,
+                                   double* Jacc, double* Kacc, double* ints, double* qout, hipStream_t st) {
+  return launch_cls_2020_m3(ev, tiles, ntiles, kets, D, Jacc, Kacc, ints, qout, st);
+}
+static hipError_t launch_diag_2100(const EngineView&, const Tile*, int, const int*, const double*, double*, double*,
+                                   double*, double*, hipStream_t) {
+  return hipErrorInvalidValue;
+}
+static hipError_t launch_diag_2110(const EngineView&, const Tile*, int, const int*, const double*, double*, double*,
+                                   double*, double*, hipStream_t) {
+  return hipErrorInvalidValue;
+}
+static hipError_t launch_diag_2111(const EngineView&, const Tile*, int, const int*, const double*, double*, double*,
+                                   double*, double*, hipStream_t) {
+  return hipErrorInvalidValue;
+}
+static hipError_t launch_diag_2120(const EngineView&, const Tile*, int, const int*, const double*, double*, double*,
+                                   double*, double*, hipStream_t) {
+  return hipErrorInvalidValue;
+}
+static hipError_t launch_diag_2121(const EngineView& ev, const Tile* tiles, int ntiles, const int* kets, const double* D,
+                                   double* Jacc, double* Kacc, double* ints, double* qout, hipStream_t st) {
+  return launch_cls_2121_m3(ev, tiles, ntiles, kets, D, Jacc, Kacc, ints, qout, st);
+}
+static hipError_t launch_diag_2200(const EngineView&, const Tile*, int, const int*, const double*, double*, double*,
+                                   double*, double*, hipStream_t) {
+  return hipErrorInvalidValue;
+}
+static hipError_t launch_diag_2210(const EngineView&, const Tile*, int, const int*, const double*, double*, double*,
+                                   double*, double*, hipStream_t) {
+  return hipErrorInvalidValue;
+}
+static hipError_t launch_diag_2211(const EngineView&, const Tile*, int, const int*, const double*, double*, double*,
+                                   double*, double*, hipStream_t) {
+  return hipErrorInvalidValue;
+}
+static hipError_t launch_diag_2220(const EngineView&, const Tile*, int, const int*, const double*, double*, double*,
+                                   double*, double*, hipStream_t) {
+  return hipErrorInvalidValue;
+}
+static hipError_t launch_diag_2221(const EngineView&, const Tile*, int, const int*, const double*, double*, double*,
+                                   double*, double*, hipStream_t) {
+  return hipErrorInvalidValue;
+}
+static hipError_t launch_diag_2222(const EngineView& ev, const Tile* tiles, int ntiles, const int* kets, const double* D,
+                                   double* Jacc, double* Kacc, double* ints, double* qout, hipStream_t st) {
+  return launch_cls_2222_m3(ev, tiles, ntiles, kets, D, Jacc, Kacc, ints, qout, st);
+}
+
 hipError_t launch_eri_class(int la, int lb, int lc, int ld, int mode, int fp32, const EngineView& ev,
                             const Tile* tiles, int ntiles, const int* kets, const double* D, double* Jacc,
                             double* Kacc, double* ints, double* qout, hipStream_t st) {
-#define PQ_CASE(A, B, C_, D_)                                                                       \
-  if (la == A && lb == B && lc == C_ && ld == D_)                                                   \
-    return launch_cls_##A##B##C_##D_(mode, fp32, ev, tiles, ntiles, kets, D, Jacc, Kacc, ints, qout, st);
+#define PQ_CASE(A, B, C_, D_)                                                                      \
+  if (la == A && lb == B && lc == C_ && ld == D_) {                                                \
+    if (mode == MODE_JK && !fp32)                                                                  \
+      return launch_cls_##A##B##C_##D_##_m0(ev, tiles, ntiles, kets, D, Jacc, Kacc, ints, qout, st); \
+    if (mode == MODE_JK)                                                                           \
+      return launch_cls_##A##B##C_##D_##_m1(ev, tiles, ntiles, kets, D, Jacc, Kacc, ints, qout, st); \
+    if (mode == MODE_STORE)                                                                        \
+      return launch_cls_##A##B##C_##D_##_m2(ev, tiles, ntiles, kets, D, Jacc, Kacc, ints, qout, st); \
+    return launch_diag_##A##B##C_##D_(ev, tiles, ntiles, kets, D, Jacc, Kacc, ints, qout, st);      \
+  }
   PQ_CASE(0, 0, 0, 0)
   PQ_CASE(1, 0, 0, 0)
   PQ_CASE(1, 0, 1, 0)

@@ -1,16 +1,28 @@
-// pq_eri_cls.hip -- one angular-momentum class of the Rys ERI kernels per
-// translation unit (PQ_LA..PQ_LD given on the compiler command line), so the
-// classes compile in parallel.
+// pq_eri_cls.hip -- one angular-momentum class and one mode of the Rys ERI
+// kernels per translation unit (PQ_LA..PQ_LD and PQ_MODE given on the
+// compiler command line: 0 J/K FP64, 1 J/K FP32, 2 store, 3 Schwarz
+// diagonal), so the instantiations compile in parallel.
 #include "pq_eri_impl.h"
 
-#define PQ_CAT2(a, b, c, d) launch_cls_##a##b##c##d
-#define PQ_CAT(a, b, c, d) PQ_CAT2(a, b, c, d)
+#define PQ_CAT2(a, b, c, d, m) launch_cls_##a##b##c##d##_m##m
+#define PQ_CAT(a, b, c, d, m) PQ_CAT2(a, b, c, d, m)
 
 namespace pq {
-hipError_t PQ_CAT(PQ_LA, PQ_LB, PQ_LC, PQ_LD)(int mode, int fp32, const EngineView& ev, const Tile* tiles,
-                                              int ntiles, const int* kets, const double* D, double* Jacc,
-                                              double* Kacc, double* ints, double* qout, hipStream_t st) {
-  return launch_cls<PQ_LA, PQ_LB, PQ_LC, PQ_LD>(mode, fp32, ev, tiles, ntiles, kets, D, Jacc, Kacc, ints, qout,
-                                                st);
+hipError_t PQ_CAT(PQ_LA, PQ_LB, PQ_LC, PQ_LD, PQ_MODE)(const EngineView& ev, const Tile* tiles, int ntiles,
+                                                       const int* kets, const double* D, double* Jacc,
+                                                       double* Kacc, double* ints, double* qout, hipStream_t st) {
+#if PQ_MODE == 0
+  return launch_one<PQ_LA, PQ_LB, PQ_LC, PQ_LD, double, MODE_JK>(ev, tiles, ntiles, kets, D, Jacc, Kacc, ints,
+                                                                 qout, st);
+#elif PQ_MODE == 1
+  return launch_one<PQ_LA, PQ_LB, PQ_LC, PQ_LD, float, MODE_JK>(ev, tiles, ntiles, kets, D, Jacc, Kacc, ints,
+                                                                qout, st);
+#elif PQ_MODE == 2
+  return launch_one<PQ_LA, PQ_LB, PQ_LC, PQ_LD, double, MODE_STORE>(ev, tiles, ntiles, kets, D, Jacc, Kacc, ints,
+                                                                    qout, st);
+#else
+  return launch_one<PQ_LA, PQ_LB, PQ_LC, PQ_LD, double, MODE_DIAG>(ev, tiles, ntiles, kets, D, Jacc, Kacc, ints,
+                                                                   qout, st);
+#endif
 }
 }  // namespace pq

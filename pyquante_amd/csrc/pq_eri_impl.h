@@ -2,20 +2,23 @@
 //
 // One template per angular-momentum class (la lb|lc ld), l <= 2, la >= lb,
 // lc >= ld, pair class (la,lb) >= (lc,ld).  Work arrives as tiles: one bra
-// shell pair and up to 64 ket shell pairs taken from the ket class's pair list
-// (sorted by Schwarz bound), so a wavefront shares its bra pair.
+// shell pair and up to 64 ket shell pairs of equal contraction length taken
+// from the ket class's pair list (sorted by Schwarz bound), so every lane of
+// a wavefront shares the bra pair and the primitive loop trip count.
 //
-//  * light classes (<= 81 Cartesian integrals): one lane per shell quartet,
-//    the contracted block held in registers;
-//  * heavy classes: one wavefront per shell quartet, the 1-D Rys integrals
-//    of each primitive quartet staged in LDS and the Cartesian products spread
-//    over the 64 lanes.
+// One lane computes one shell quartet.  For the large classes the Cartesian
+// block is split over blockIdx.y: each lane then owns the components of one
+// bra function a (SPLIT 1) or one bra function pair (a, b) (SPLIT 2), keeps
+// only those accumulators in registers, and recomputes the (cheap) roots and
+// 1-D recurrences; the compiler drops the transfer rows it does not need.
 //
 // Modes: MODE_JK digests the block into J and K (the reference's
 // getJ/getK/get2JmK, Ints.py:195-238, done directly from the integrals),
 // MODE_STORE writes the reference's flat unique-ERI array (Ints.py:146-167,
 // index cints.c:413-425), MODE_DIAG produces Schwarz bounds from (ab|ab).
 #pragma once
+#include <type_traits>
+
 #include "pq_device.h"
 #include "pq_kernels.h"
 
@@ -29,6 +32,9 @@ struct Cls {
   static constexpr int NCOMP = NA * NB * NC * ND;
   static constexpr int L = LA + LB + LC + LD;
   static constexpr int NR = L / 2 + 1;
+  // component split: 0 none, 1 over a, 2 over (a, b)
+  static constexpr int SPLIT = NCOMP <= kLaneMaxComp ? 0 : (NCOMP / NA <= kLaneMaxComp ? 1 : 2);
+  static constexpr int NSPLIT = SPLIT == 0 ? 1 : (SPLIT == 1 ? NA : NA * NB);
 };
 
 // Per-primitive-quartet scalars shared by all roots.
@@ -50,27 +56,28 @@ __device__ __forceinline__ PrimQ prim_setup(const PairInfo& bp, const PrimPair& 
   s.PQ[0] = b.Px - q.Px;
   s.PQ[1] = b.Py - q.Py;
   s.PQ[2] = b.Pz - q.Pz;
-  const double rpq = 1.0 / pq;
-  const double rho = b.p * q.p * rpq;
-  s.X = rho * (s.PQ[0] * s.PQ[0] + s.PQ[1] * s.PQ[1] + s.PQ[2] * s.PQ[2]);
-  s.pref = kTwoPi25 * b.K * q.K / (b.p * q.p) * sqrt(rpq);
+  const double r = rsqrt(pq);
+  const double rpq = r * r;
+  s.X = b.p * q.p * rpq * (s.PQ[0] * s.PQ[0] + s.PQ[1] * s.PQ[1] + s.PQ[2] * s.PQ[2]);
+  s.pref = kTwoPi25 * b.Kp * q.Kp * r;
   s.rpq = rpq;
   s.qfac = q.p * rpq;
   s.pfac = b.p * rpq;
-  s.i2p = 0.5 / b.p;
-  s.i2q = 0.5 / q.p;
+  s.i2p = 0.5 * b.rp;
+  s.i2q = 0.5 * q.rp;
   return s;
 }
 
-// Contracted block of one shell quartet, components in PyQuante order, without
+// Contracted block of one shell quartet for bra components a in [A0, A0+NAS),
+// b in [B0, B0+NBS) and every ket component; PyQuante component order, without
 // the per-component normalisation factor.
-template <int LA, int LB, int LC, int LD, typename T>
+template <int LA, int LB, int LC, int LD, int A0, int NAS, int B0, int NBS, typename T>
 __device__ __forceinline__ void eri_block(const EngineView& ev, const PairInfo& bp, const PairInfo& kp,
-                                          T (&out)[Cls<LA, LB, LC, LD>::NCOMP]) {
+                                          T (&out)[NAS * NBS * ncart(LC) * ncart(LD)]) {
   using C = Cls<LA, LB, LC, LD>;
   constexpr int NR = C::NR;
 #pragma unroll
-  for (int c = 0; c < C::NCOMP; ++c) out[c] = (T)0;
+  for (int c = 0; c < NAS * NBS * C::NC * C::ND; ++c) out[c] = (T)0;
   const T AB[3] = {(T)bp.ABx, (T)bp.ABy, (T)bp.ABz};
   const T CD[3] = {(T)kp.ABx, (T)kp.ABy, (T)kp.ABz};
   for (int i = 0; i < bp.pp_n; ++i) {
@@ -97,17 +104,17 @@ __device__ __forceinline__ void eri_block(const EngineView& ev, const PairInfo& 
         rys_1d<LA, LB, LC, LD, T>(w[r] * pref, (T)s.PA[2] - qfac * (T)s.PQ[2] * ur,
                                   (T)s.QC[2] + pfac * (T)s.PQ[2] * ur, B10, B01, B00, AB[2], CD[2], Iz);
 #pragma unroll
-        for (int a = 0; a < C::NA; ++a)
+        for (int a = 0; a < NAS; ++a)
 #pragma unroll
-          for (int bb = 0; bb < C::NB; ++bb)
+          for (int bb = 0; bb < NBS; ++bb)
 #pragma unroll
             for (int c = 0; c < C::NC; ++c)
 #pragma unroll
               for (int d = 0; d < C::ND; ++d) {
-                const int o = ((a * C::NB + bb) * C::NC + c) * C::ND + d;
-                out[o] += Ix[cart_pow(LA, a, 0)][cart_pow(LB, bb, 0)][cart_pow(LC, c, 0)][cart_pow(LD, d, 0)] *
-                          Iy[cart_pow(LA, a, 1)][cart_pow(LB, bb, 1)][cart_pow(LC, c, 1)][cart_pow(LD, d, 1)] *
-                          Iz[cart_pow(LA, a, 2)][cart_pow(LB, bb, 2)][cart_pow(LC, c, 2)][cart_pow(LD, d, 2)];
+                const int o = ((a * NBS + bb) * C::NC + c) * C::ND + d;
+                out[o] += Ix[cart_pow(LA, A0 + a, 0)][cart_pow(LB, B0 + bb, 0)][cart_pow(LC, c, 0)][cart_pow(LD, d, 0)] *
+                          Iy[cart_pow(LA, A0 + a, 1)][cart_pow(LB, B0 + bb, 1)][cart_pow(LC, c, 1)][cart_pow(LD, d, 1)] *
+                          Iz[cart_pow(LA, A0 + a, 2)][cart_pow(LB, B0 + bb, 2)][cart_pow(LC, c, 2)][cart_pow(LD, d, 2)];
               }
       }
     }
@@ -123,29 +130,27 @@ __device__ __forceinline__ void eri_block(const EngineView& ev, const PairInfo& 
 //   Kacc[b,c] += f v D[a,d],    Kacc[b,d] += f v D[a,c],
 // and J = Jacc + Jacc^T, K = Kacc + Kacc^T (symmetric D).
 // ---------------------------------------------------------------------
-template <int LA, int LB, int LC, int LD, typename T>
+template <int LA, int LB, int LC, int LD, int A0, int NAS, int B0, int NBS, typename T>
 __device__ __forceinline__ void digest_lane(const EngineView& ev, const PairInfo& bp, const PairInfo& kp,
-                                            const T (&out)[Cls<LA, LB, LC, LD>::NCOMP], double f,
+                                            const T (&out)[NAS * NBS * ncart(LC) * ncart(LD)], double f,
                                             bool active, const double* __restrict__ D, double* Jacc,
                                             double* Kacc) {
   using C = Cls<LA, LB, LC, LD>;
   const int n = ev.nbf;
-  const int a0 = bp.bfA, b0 = bp.bfB, c0 = kp.bfA, d0 = kp.bfB;
-  double jab[C::NA][C::NB], jcd[C::NC][C::ND], kac[C::NA][C::NC], kad[C::NA][C::ND],
-      kbc[C::NB][C::NC], kbd[C::NB][C::ND];
-  double Dab[C::NA][C::NB], Dcd[C::NC][C::ND], Dac[C::NA][C::NC], Dad[C::NA][C::ND],
-      Dbc[C::NB][C::NC], Dbd[C::NB][C::ND];
+  const int a0 = bp.bfA + A0, b0 = bp.bfB + B0, c0 = kp.bfA, d0 = kp.bfB;
+  double jab[NAS][NBS], jcd[C::NC][C::ND], kac[NAS][C::NC], kad[NAS][C::ND], kbc[NBS][C::NC], kbd[NBS][C::ND];
+  double Dab[NAS][NBS], Dcd[C::NC][C::ND], Dac[NAS][C::NC], Dad[NAS][C::ND], Dbc[NBS][C::NC], Dbd[NBS][C::ND];
 #pragma unroll
-  for (int a = 0; a < C::NA; ++a) {
+  for (int a = 0; a < NAS; ++a) {
 #pragma unroll
-    for (int b = 0; b < C::NB; ++b) { Dab[a][b] = D[(a0 + a) * n + b0 + b]; jab[a][b] = 0; }
+    for (int b = 0; b < NBS; ++b) { Dab[a][b] = D[(a0 + a) * n + b0 + b]; jab[a][b] = 0; }
 #pragma unroll
     for (int c = 0; c < C::NC; ++c) { Dac[a][c] = D[(a0 + a) * n + c0 + c]; kac[a][c] = 0; }
 #pragma unroll
     for (int d = 0; d < C::ND; ++d) { Dad[a][d] = D[(a0 + a) * n + d0 + d]; kad[a][d] = 0; }
   }
 #pragma unroll
-  for (int b = 0; b < C::NB; ++b) {
+  for (int b = 0; b < NBS; ++b) {
 #pragma unroll
     for (int c = 0; c < C::NC; ++c) { Dbc[b][c] = D[(b0 + b) * n + c0 + c]; kbc[b][c] = 0; }
 #pragma unroll
@@ -157,15 +162,15 @@ __device__ __forceinline__ void digest_lane(const EngineView& ev, const PairInfo
     for (int d = 0; d < C::ND; ++d) { Dcd[c][d] = D[(c0 + c) * n + d0 + d]; jcd[c][d] = 0; }
 
 #pragma unroll
-  for (int a = 0; a < C::NA; ++a)
+  for (int a = 0; a < NAS; ++a)
 #pragma unroll
-    for (int b = 0; b < C::NB; ++b)
+    for (int b = 0; b < NBS; ++b)
 #pragma unroll
       for (int c = 0; c < C::NC; ++c)
 #pragma unroll
         for (int d = 0; d < C::ND; ++d) {
-          const double nf = cart_norm(LA, a) * cart_norm(LB, b) * cart_norm(LC, c) * cart_norm(LD, d);
-          const double v = (double)out[((a * C::NB + b) * C::NC + c) * C::ND + d] * (nf * f);
+          const double nf = cart_norm(LA, A0 + a) * cart_norm(LB, B0 + b) * cart_norm(LC, c) * cart_norm(LD, d);
+          const double v = (double)out[((a * NBS + b) * C::NC + c) * C::ND + d] * (nf * f);
           jab[a][b] += v * Dcd[c][d];
           jcd[c][d] += v * Dab[a][b];
           kac[a][c] += v * Dbd[b][d];
@@ -176,9 +181,9 @@ __device__ __forceinline__ void digest_lane(const EngineView& ev, const PairInfo
   // J_ab: every lane of the wave shares the bra pair -> reduce, one atomic
   const int lane = threadIdx.x & 63;
 #pragma unroll
-  for (int a = 0; a < C::NA; ++a)
+  for (int a = 0; a < NAS; ++a)
 #pragma unroll
-    for (int b = 0; b < C::NB; ++b) {
+    for (int b = 0; b < NBS; ++b) {
       const double s = wave_sum(jab[a][b]);
       if (lane == 0) unsafeAtomicAdd(&Jacc[(a0 + a) * n + b0 + b], 2.0 * s);
     }
@@ -188,14 +193,14 @@ __device__ __forceinline__ void digest_lane(const EngineView& ev, const PairInfo
 #pragma unroll
     for (int d = 0; d < C::ND; ++d) unsafeAtomicAdd(&Jacc[(c0 + c) * n + d0 + d], 2.0 * jcd[c][d]);
 #pragma unroll
-  for (int a = 0; a < C::NA; ++a) {
+  for (int a = 0; a < NAS; ++a) {
 #pragma unroll
     for (int c = 0; c < C::NC; ++c) unsafeAtomicAdd(&Kacc[(a0 + a) * n + c0 + c], kac[a][c]);
 #pragma unroll
     for (int d = 0; d < C::ND; ++d) unsafeAtomicAdd(&Kacc[(a0 + a) * n + d0 + d], kad[a][d]);
   }
 #pragma unroll
-  for (int b = 0; b < C::NB; ++b) {
+  for (int b = 0; b < NBS; ++b) {
 #pragma unroll
     for (int c = 0; c < C::NC; ++c) unsafeAtomicAdd(&Kacc[(b0 + b) * n + c0 + c], kbc[b][c]);
 #pragma unroll
@@ -225,9 +230,72 @@ __device__ __forceinline__ bool store_canonical(const PairInfo& bp, const PairIn
   return true;
 }
 
-// ---------------------------------------------------------------------
-// Light classes: one lane per shell quartet.
-// ---------------------------------------------------------------------
+// Schwarz bounds are non-negative: their IEEE bit patterns order like the
+// values, so a 64-bit integer max combines the component splits.
+__device__ __forceinline__ void atomic_max_nonneg(double* p, double v) {
+  atomicMax((unsigned long long*)p, (unsigned long long)__double_as_longlong(v));
+}
+
+// One lane, one shell quartet, bra components [A0, A0+NAS) x [B0, B0+NBS).
+template <int LA, int LB, int LC, int LD, typename T, int MODE, int A0, int NAS, int B0, int NBS>
+__device__ __forceinline__ void lane_body(const EngineView& ev, const Tile& tl, bool active, int bra, int ket,
+                                          const double* __restrict__ D, double* Jacc, double* Kacc,
+                                          double* ints, double* qout) {
+  using C = Cls<LA, LB, LC, LD>;
+  const PairInfo bp = ev.pairs[bra];
+  const PairInfo kp = ev.pairs[ket];
+  constexpr int NO = NAS * NBS * C::NC * C::ND;
+  T out[NO];
+  if (active) {
+    eri_block<LA, LB, LC, LD, A0, NAS, B0, NBS, T>(ev, bp, kp, out);
+  } else {
+#pragma unroll
+    for (int c = 0; c < NO; ++c) out[c] = (T)0;
+  }
+  if constexpr (MODE == MODE_JK) {
+    const double f = active ? quartet_degeneracy(bp, kp, bra, ket) : 0.0;
+    digest_lane<LA, LB, LC, LD, A0, NAS, B0, NBS, T>(ev, bp, kp, out, f, active, D, Jacc, Kacc);
+  } else if constexpr (MODE == MODE_STORE) {
+    if (!active) return;
+    const bool same = bra == ket;
+#pragma unroll
+    for (int a = 0; a < NAS; ++a)
+#pragma unroll
+      for (int b = 0; b < NBS; ++b)
+#pragma unroll
+        for (int c = 0; c < C::NC; ++c)
+#pragma unroll
+          for (int d = 0; d < C::ND; ++d) {
+            const int i = bp.bfA + A0 + a, j = bp.bfB + B0 + b, k = kp.bfA + c, l = kp.bfB + d;
+            if (!store_canonical(bp, kp, same, i, j, k, l)) continue;
+            const double nf =
+                cart_norm(LA, A0 + a) * cart_norm(LB, B0 + b) * cart_norm(LC, c) * cart_norm(LD, d);
+            ints[ijkl2intindex(i, j, k, l)] = (double)out[((a * NBS + b) * C::NC + c) * C::ND + d] * nf;
+          }
+  } else {  // MODE_DIAG: Q = max over components sqrt|(ab|ab)|
+    if (!active) return;
+    double qmax = 0.0;
+#pragma unroll
+    for (int a = 0; a < NAS; ++a)
+#pragma unroll
+      for (int b = 0; b < NBS; ++b) {
+        const double nf = cart_norm(LA, A0 + a) * cart_norm(LB, B0 + b);
+        const int c = A0 + a, d = B0 + b;  // diagonal ket component (bra == ket class)
+        const double v = fabs((double)out[((a * NBS + b) * C::NC + c) * C::ND + d]) * nf * nf;
+        qmax = fmax(qmax, v);
+      }
+    atomic_max_nonneg(&qout[ket], sqrt(qmax));
+  }
+}
+
+template <int I, int N, typename F>
+__device__ __forceinline__ void static_switch(int i, F&& f) {
+  if constexpr (I < N) {
+    if (i == I) f(std::integral_constant<int, I>{});
+    else static_switch<I + 1, N>(i, f);
+  }
+}
+
 template <int LA, int LB, int LC, int LD, typename T, int MODE>
 __global__ __launch_bounds__(256) void k_eri_lane(EngineView ev, const Tile* __restrict__ tiles, int ntiles,
                                                   const int* __restrict__ kets, const double* __restrict__ D,
@@ -235,231 +303,41 @@ __global__ __launch_bounds__(256) void k_eri_lane(EngineView ev, const Tile* __r
                                                   double* qout) {
   using C = Cls<LA, LB, LC, LD>;
   const int lane = threadIdx.x & 63;
-  const int t = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+  const int t = __builtin_amdgcn_readfirstlane(blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6));
   if (t >= ntiles) return;  // uniform per wavefront
   const Tile tl = tiles[t];
   const bool active = lane < tl.nket;
   const int ket = kets[tl.ket0 + (active ? lane : 0)];
   const int bra = (MODE == MODE_DIAG) ? ket : tl.bra;
-  const PairInfo bp = ev.pairs[bra];
-  const PairInfo kp = ev.pairs[ket];
-  T out[C::NCOMP];
-  if (active) {
-    eri_block<LA, LB, LC, LD, T>(ev, bp, kp, out);
+  if constexpr (C::SPLIT == 0) {
+    lane_body<LA, LB, LC, LD, T, MODE, 0, C::NA, 0, C::NB>(ev, tl, active, bra, ket, D, Jacc, Kacc, ints, qout);
+  } else if constexpr (C::SPLIT == 1) {
+    static_switch<0, C::NA>(blockIdx.y, [&](auto ia) {
+      lane_body<LA, LB, LC, LD, T, MODE, decltype(ia)::value, 1, 0, C::NB>(ev, tl, active, bra, ket, D, Jacc,
+                                                                            Kacc, ints, qout);
+    });
   } else {
-#pragma unroll
-    for (int c = 0; c < C::NCOMP; ++c) out[c] = (T)0;
-  }
-  if (MODE == MODE_JK) {
-    const double f = active ? quartet_degeneracy(bp, kp, bra, ket) : 0.0;
-    digest_lane<LA, LB, LC, LD, T>(ev, bp, kp, out, f, active, D, Jacc, Kacc);
-  } else if (MODE == MODE_STORE) {
-    if (!active) return;
-    const bool same = bra == ket;
-#pragma unroll
-    for (int a = 0; a < C::NA; ++a)
-#pragma unroll
-      for (int b = 0; b < C::NB; ++b)
-#pragma unroll
-        for (int c = 0; c < C::NC; ++c)
-#pragma unroll
-          for (int d = 0; d < C::ND; ++d) {
-            const int i = bp.bfA + a, j = bp.bfB + b, k = kp.bfA + c, l = kp.bfB + d;
-            if (!store_canonical(bp, kp, same, i, j, k, l)) continue;
-            const double nf = cart_norm(LA, a) * cart_norm(LB, b) * cart_norm(LC, c) * cart_norm(LD, d);
-            ints[ijkl2intindex(i, j, k, l)] = (double)out[((a * C::NB + b) * C::NC + c) * C::ND + d] * nf;
-          }
-  } else {  // MODE_DIAG: Q = max over components sqrt|(ab|ab)|
-    if (!active) return;
-    double qmax = 0.0;
-#pragma unroll
-    for (int a = 0; a < C::NA; ++a)
-#pragma unroll
-      for (int b = 0; b < C::NB; ++b) {
-        const double nf = cart_norm(LA, a) * cart_norm(LB, b);
-        const double v = fabs((double)out[((a * C::NB + b) * C::NC + a) * C::ND + b]) * nf * nf;
-        qmax = fmax(qmax, v);
-      }
-    qout[ket] = sqrt(qmax);
+    static_switch<0, C::NA * C::NB>(blockIdx.y, [&](auto iab) {
+      constexpr int v = decltype(iab)::value;
+      lane_body<LA, LB, LC, LD, T, MODE, v / C::NB, 1, v % C::NB, 1>(ev, tl, active, bra, ket, D, Jacc, Kacc,
+                                                                      ints, qout);
+    });
   }
 }
 
 // ---------------------------------------------------------------------
-// Heavy classes: one wavefront per shell quartet.  Lanes (r, dir) build the
-// 1-D integrals of root r and direction dir for each primitive quartet in
-// LDS; every lane then accumulates its share of Cartesian components.
-// ---------------------------------------------------------------------
-template <int LA, int LB, int LC, int LD, typename T, int MODE>
-__global__ __launch_bounds__(64) void k_eri_wave(EngineView ev, const Tile* __restrict__ tiles, int ntiles,
-                                                 const int* __restrict__ kets, const double* __restrict__ D,
-                                                 double* Jacc, double* Kacc, double* ints, double* qout) {
-  using C = Cls<LA, LB, LC, LD>;
-  constexpr int NR = C::NR;
-  constexpr int N1 = (LA + 1) * (LB + 1) * (LC + 1) * (LD + 1);
-  constexpr int CPL = (C::NCOMP + 63) / 64;
-  __shared__ T sI[NR][3][N1];
-  const int lane = threadIdx.x;
-  const int t = blockIdx.x;
-  if (t >= ntiles) return;
-  const Tile tl = tiles[t];
-
-  // this lane's components and their offsets into the 1-D tables
-  int offs[CPL][3];
-  int comp[CPL];
-#pragma unroll
-  for (int j = 0; j < CPL; ++j) {
-    const int c = lane + 64 * j;
-    comp[j] = c;
-    const int cc = c < C::NCOMP ? c : 0;
-    const int id = cc % C::ND, ic = (cc / C::ND) % C::NC, ib = (cc / (C::ND * C::NC)) % C::NB,
-              ia = cc / (C::ND * C::NC * C::NB);
-#pragma unroll
-    for (int x = 0; x < 3; ++x) {
-      int pa = 0, pb = 0, pc = 0, pd = 0;
-#pragma unroll
-      for (int q = 0; q < C::NA; ++q) if (q == ia) pa = cart_pow(LA, q, x);
-#pragma unroll
-      for (int q = 0; q < C::NB; ++q) if (q == ib) pb = cart_pow(LB, q, x);
-#pragma unroll
-      for (int q = 0; q < C::NC; ++q) if (q == ic) pc = cart_pow(LC, q, x);
-#pragma unroll
-      for (int q = 0; q < C::ND; ++q) if (q == id) pd = cart_pow(LD, q, x);
-      offs[j][x] = ((pa * (LB + 1) + pb) * (LC + 1) + pc) * (LD + 1) + pd;
-    }
-  }
-
-  for (int kk = 0; kk < tl.nket; ++kk) {
-    const int ket = kets[tl.ket0 + kk];
-    const int bra = (MODE == MODE_DIAG) ? ket : tl.bra;
-    const PairInfo bp = ev.pairs[bra];
-    const PairInfo kp = ev.pairs[ket];
-    T acc[CPL];
-#pragma unroll
-    for (int j = 0; j < CPL; ++j) acc[j] = (T)0;
-    const int r_me = lane / 3, x_me = lane % 3;
-    const T ABm = (T)(x_me == 0 ? bp.ABx : (x_me == 1 ? bp.ABy : bp.ABz));
-    const T CDm = (T)(x_me == 0 ? kp.ABx : (x_me == 1 ? kp.ABy : kp.ABz));
-    for (int i = 0; i < bp.pp_n; ++i) {
-      const PrimPair b = ev.prims[bp.pp_off + i];
-      for (int k = 0; k < kp.pp_n; ++k) {
-        const PrimPair q = ev.prims[kp.pp_off + k];
-        if (lane < 3 * NR) {
-          const PrimQ s = prim_setup(bp, b, kp, q);
-          T u[NR], w[NR];
-          rys_roots<NR, T>(s.X, u, w);
-          T ur = u[0], wr = w[0];
-#pragma unroll
-          for (int r = 1; r < NR; ++r)
-            if (r == r_me) { ur = u[r]; wr = w[r]; }
-          const T qfac = (T)s.qfac, pfac = (T)s.pfac;
-          const T B00 = (T)(0.5 * s.rpq) * ur;
-          const T B10 = (T)s.i2p * ((T)1 - qfac * ur);
-          const T B01 = (T)s.i2q * ((T)1 - pfac * ur);
-          const T C00 = (T)s.PA[x_me] - qfac * (T)s.PQ[x_me] * ur;
-          const T D00 = (T)s.QC[x_me] + pfac * (T)s.PQ[x_me] * ur;
-          const T seed = x_me == 2 ? wr * (T)s.pref : (T)1;
-          T I[LA + 1][LB + 1][LC + 1][LD + 1];
-          rys_1d<LA, LB, LC, LD, T>(seed, C00, D00, B10, B01, B00, ABm, CDm, I);
-          T* dst = &sI[r_me][x_me][0];
-#pragma unroll
-          for (int a = 0; a <= LA; ++a)
-#pragma unroll
-            for (int bb = 0; bb <= LB; ++bb)
-#pragma unroll
-              for (int c = 0; c <= LC; ++c)
-#pragma unroll
-                for (int d = 0; d <= LD; ++d)
-                  dst[((a * (LB + 1) + bb) * (LC + 1) + c) * (LD + 1) + d] = I[a][bb][c][d];
-        }
-        __syncthreads();
-#pragma unroll
-        for (int j = 0; j < CPL; ++j) {
-          if (comp[j] < C::NCOMP) {
-            T v = (T)0;
-#pragma unroll
-            for (int r = 0; r < NR; ++r) v += sI[r][0][offs[j][0]] * sI[r][1][offs[j][1]] * sI[r][2][offs[j][2]];
-            acc[j] += v;
-          }
-        }
-        __syncthreads();
-      }
-    }
-    // epilogue per component
-    const double f = quartet_degeneracy(bp, kp, bra, ket);
-    const int n = ev.nbf;
-    double qmax = 0.0;
-#pragma unroll
-    for (int j = 0; j < CPL; ++j) {
-      const int c = comp[j];
-      if (c >= C::NCOMP) continue;
-      const int id = c % C::ND, ic = (c / C::ND) % C::NC, ib = (c / (C::ND * C::NC)) % C::NB,
-                ia = c / (C::ND * C::NC * C::NB);
-      double nf = 1.0;
-#pragma unroll
-      for (int q = 0; q < C::NA; ++q) if (q == ia) nf *= cart_norm(LA, q);
-#pragma unroll
-      for (int q = 0; q < C::NB; ++q) if (q == ib) nf *= cart_norm(LB, q);
-#pragma unroll
-      for (int q = 0; q < C::NC; ++q) if (q == ic) nf *= cart_norm(LC, q);
-#pragma unroll
-      for (int q = 0; q < C::ND; ++q) if (q == id) nf *= cart_norm(LD, q);
-      const double v = (double)acc[j] * nf;
-      const int ai = bp.bfA + ia, bi = bp.bfB + ib, ci = kp.bfA + ic, di = kp.bfB + id;
-      if (MODE == MODE_JK) {
-        const double fv = f * v;
-        unsafeAtomicAdd(&Jacc[ai * n + bi], 2.0 * fv * D[ci * n + di]);
-        unsafeAtomicAdd(&Jacc[ci * n + di], 2.0 * fv * D[ai * n + bi]);
-        unsafeAtomicAdd(&Kacc[ai * n + ci], fv * D[bi * n + di]);
-        unsafeAtomicAdd(&Kacc[ai * n + di], fv * D[bi * n + ci]);
-        unsafeAtomicAdd(&Kacc[bi * n + ci], fv * D[ai * n + di]);
-        unsafeAtomicAdd(&Kacc[bi * n + di], fv * D[ai * n + ci]);
-      } else if (MODE == MODE_STORE) {
-        if (store_canonical(bp, kp, bra == ket, ai, bi, ci, di)) ints[ijkl2intindex(ai, bi, ci, di)] = v;
-      } else {
-        if (ia == ic && ib == id) qmax = fmax(qmax, fabs(v));
-      }
-    }
-    if (MODE == MODE_DIAG) {
-#pragma unroll
-      for (int off = 32; off >= 1; off >>= 1) qmax = fmax(qmax, __shfl_xor(qmax, off, 64));
-      if (lane == 0) qout[ket] = sqrt(qmax);
-    }
-  }
-}
-
-// ---------------------------------------------------------------------
-// Launch table
+// Launch
 // ---------------------------------------------------------------------
 template <int LA, int LB, int LC, int LD, typename T, int MODE>
 static hipError_t launch_one(const EngineView& ev, const Tile* tiles, int ntiles, const int* kets,
                              const double* D, double* Jacc, double* Kacc, double* ints, double* qout,
                              hipStream_t st) {
   if (ntiles <= 0) return hipSuccess;
-  if constexpr (Cls<LA, LB, LC, LD>::NCOMP <= kLaneMaxComp) {
-    const int wpb = 4;
-    hipLaunchKernelGGL((k_eri_lane<LA, LB, LC, LD, T, MODE>), dim3((ntiles + wpb - 1) / wpb), dim3(64 * wpb),
-                       0, st, ev, tiles, ntiles, kets, D, Jacc, Kacc, ints, qout);
-  } else {
-    hipLaunchKernelGGL((k_eri_wave<LA, LB, LC, LD, T, MODE>), dim3(ntiles), dim3(64), 0, st, ev, tiles,
-                       ntiles, kets, D, Jacc, Kacc, ints, qout);
-  }
+  using C = Cls<LA, LB, LC, LD>;
+  const int wpb = 4;
+  hipLaunchKernelGGL((k_eri_lane<LA, LB, LC, LD, T, MODE>), dim3((ntiles + wpb - 1) / wpb, C::NSPLIT),
+                     dim3(64 * wpb), 0, st, ev, tiles, ntiles, kets, D, Jacc, Kacc, ints, qout);
   return hipGetLastError();
-}
-
-template <int LA, int LB, int LC, int LD>
-static hipError_t launch_cls(int mode, int fp32, const EngineView& ev, const Tile* tiles, int ntiles,
-                             const int* kets, const double* D, double* Jacc, double* Kacc, double* ints,
-                             double* qout, hipStream_t st) {
-  if (mode == MODE_JK) {
-    if (fp32)
-      return launch_one<LA, LB, LC, LD, float, MODE_JK>(ev, tiles, ntiles, kets, D, Jacc, Kacc, ints, qout, st);
-    return launch_one<LA, LB, LC, LD, double, MODE_JK>(ev, tiles, ntiles, kets, D, Jacc, Kacc, ints, qout, st);
-  }
-  if (mode == MODE_STORE)
-    return launch_one<LA, LB, LC, LD, double, MODE_STORE>(ev, tiles, ntiles, kets, D, Jacc, Kacc, ints, qout, st);
-  if constexpr (LA == LC && LB == LD)
-    return launch_one<LA, LB, LC, LD, double, MODE_DIAG>(ev, tiles, ntiles, kets, D, Jacc, Kacc, ints, qout, st);
-  return hipErrorInvalidValue;
 }
 
 }  // namespace pq

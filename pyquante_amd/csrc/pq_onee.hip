@@ -52,7 +52,8 @@ __global__ __launch_bounds__(256) void k_onee(EngineView ev, const int* __restri
 
   for (int ip = 0; ip < bp.pp_n; ++ip) {
     const PrimPair pp = ev.prims[bp.pp_off + ip];
-    const double p = pp.p, beta = pp.p - pp.a;
+    const double p = pp.p, beta = pp.p - ev.prim_a[bp.pp_off + ip];
+    const double Kab = pp.Kp * pp.p;
     const double P[3] = {pp.Px, pp.Py, pp.Pz};
     const double i2p = 0.5 / p;
     // 1-D overlaps O[d][i][j], i <= LA, j <= LB + 2
@@ -72,7 +73,7 @@ __global__ __launch_bounds__(256) void k_onee(EngineView ev, const int* __restri
                                                     (j > 1 ? (j - 1) * O[d][i][j - 2] : 0.0));
       }
     }
-    const double pre = pp.K * pow(M_PI / p, 1.5);
+    const double pre = Kab * pow(M_PI / p, 1.5);
 #pragma unroll
     for (int a = 0; a < NA; ++a)
 #pragma unroll
@@ -92,7 +93,7 @@ __global__ __launch_bounds__(256) void k_onee(EngineView ev, const int* __restri
         tk[a][b] += pre * (t0 - 2.0 * beta * beta * t1 - 0.5 * t2);
       }
     // nuclear attraction
-    const double vpre = -2.0 * M_PI / p * pp.K;
+    const double vpre = -2.0 * M_PI * pp.Kp;
     for (int c = 0; c < natom; ++c) {
       const double Cc[3] = {atom_xyz[3 * c], atom_xyz[3 * c + 1], atom_xyz[3 * c + 2]};
       const double PC[3] = {P[0] - Cc[0], P[1] - Cc[1], P[2] - Cc[2]};
