@@ -65,7 +65,7 @@ struct Tile {
   int bra;   // pair id
   int ket0;  // first entry of the ket pair list covered by this tile
   int nket;  // number of kets
-  int pad;
+  int prim;  // bra primitive-pair range: first (low 16 bits), count (high 16 bits; 0 = all)
 };
 
 // Everything a kernel needs about the basis and the pair data.

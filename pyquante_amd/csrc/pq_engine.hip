@@ -115,6 +115,12 @@ struct pq_engine {
   double* d_Jacc = nullptr;
   double* d_Kacc = nullptr;
 
+  // side streams (default priority) for running class kernels concurrently
+  static constexpr int kSide = 4;
+  hipStream_t side[kSide] = {};
+  hipEvent_t fork_ev = nullptr, join_ev[kSide] = {};
+  bool streams_ok = false;
+
   EngineView view() const { return EngineView{d_pairs, d_prims, d_prim_a, nbf}; }
 
   void free_work(std::vector<ClassWork>& w) {
@@ -137,6 +143,13 @@ struct pq_engine {
     pairs_ok = work_ok = store_ok = false;
   }
   ~pq_engine() {
+    if (streams_ok) {
+      for (int i = 0; i < kSide; ++i) {
+        (void)hipStreamDestroy(side[i]);
+        (void)hipEventDestroy(join_ev[i]);
+      }
+      (void)hipEventDestroy(fork_ev);
+    }
     free_pairs();
     dev_free(d_atom_xyz);
     dev_free(d_atom_z);
@@ -252,7 +265,12 @@ int build_pair_data(pq_engine* e) {
   return PQ_OK;
 }
 
-int build_work(pq_engine* e, std::vector<ClassWork>& out, double cut, double fp32cut) {
+// Bra primitive pairs per tile are capped so that one lane evaluates at most
+// about kPrimBudget primitive quartets: heavily contracted quartets are
+// spread over several tiles (their partial sums digest independently).
+constexpr int kPrimBudget = 36;
+
+int build_work(pq_engine* e, std::vector<ClassWork>& out, double cut, double fp32cut, bool split_prims) {
   for (auto& c : out) {
     dev_free(c.d_t64);
     dev_free(c.d_t32);
@@ -307,8 +325,16 @@ int build_work(pq_engine* e, std::vector<ClassWork>& out, double cut, double fp3
           if (X == Y) end = std::min(end, (int)i + 1);
           int e64 = end;
           if (fp32cut > 0.0) e64 = qi <= 0.0 ? g0 : std::min(end, g0 + count_ge(g0, g1, fp32cut / qi));
-          for (int k = g0; k < e64; k += tile) cw.t64.push_back(Tile{bras[i], k, std::min(tile, e64 - k), 0});
-          for (int k = e64; k < end; k += tile) cw.t32.push_back(Tile{bras[i], k, std::min(tile, end - k), 0});
+          // split the bra primitive sum when the quartets are heavily contracted
+          const int kpn = e->pairs[kets[g0]].pp_n;
+          int chunk = bpn;
+          if (split_prims) chunk = std::max(1, std::min(bpn, kPrimBudget / std::max(1, kpn)));
+          for (int p0 = 0; p0 < bpn; p0 += chunk) {
+            const int np = std::min(chunk, bpn - p0);
+            const int prim = np == bpn ? 0 : (p0 | (np << 16));
+            for (int k = g0; k < e64; k += tile) cw.t64.push_back(Tile{bras[i], k, std::min(tile, e64 - k), prim});
+            for (int k = e64; k < end; k += tile) cw.t32.push_back(Tile{bras[i], k, std::min(tile, end - k), prim});
+          }
           cw.nq64 += e64 - g0;
           cw.nq32 += end - e64;
           cw.np64 += bpn * (kpre[e64] - kpre[g0]);
@@ -510,7 +536,7 @@ int pq_build_pairs(pq_engine* eng, double schwarz_cut, double fp32_cut) {
     int rc = build_pair_data(eng);
     if (rc) return rc;
   }
-  int rc = build_work(eng, eng->work, schwarz_cut, fp32_cut);
+  int rc = build_work(eng, eng->work, schwarz_cut, fp32_cut, true);
   if (rc) return rc;
   eng->cut = schwarz_cut;
   eng->fp32cut = fp32_cut;
@@ -546,7 +572,7 @@ int pq_get2ints(pq_engine* eng, double* out_dev, uint64_t len, void* stream) {
     if (rc) return rc;
   }
   if (!eng->store_ok) {
-    int rc = build_work(eng, eng->store_work, 0.0, 0.0);
+    int rc = build_work(eng, eng->store_work, 0.0, 0.0, false);
     if (rc) return rc;
     eng->store_ok = true;
   }
@@ -627,7 +653,41 @@ static int fock_jk_impl(pq_engine* eng, const double* D_dev, double* J_dev, doub
       rc = fail(PQ_E_HIP, hipGetErrorString(err));
       break;
     }
-    for (size_t ci = 0; ci < eng->work.size() && rc == PQ_OK; ++ci) {
+    if (!ms) {
+      // fork: class kernels spread over the side streams, joined before the
+      // symmetrisation (all of them only add into Jacc/Kacc)
+      if (!eng->streams_ok) {
+        for (int i = 0; i < pq_engine::kSide && err == hipSuccess; ++i) {
+          err = hipStreamCreateWithFlags(&eng->side[i], hipStreamNonBlocking);
+          if (err == hipSuccess) err = hipEventCreateWithFlags(&eng->join_ev[i], hipEventDisableTiming);
+        }
+        if (err == hipSuccess) err = hipEventCreateWithFlags(&eng->fork_ev, hipEventDisableTiming);
+        if (err != hipSuccess) { rc = fail(PQ_E_HIP, hipGetErrorString(err)); break; }
+        eng->streams_ok = true;
+      }
+      if ((err = hipEventRecord(eng->fork_ev, st)) != hipSuccess) { rc = fail(PQ_E_HIP, hipGetErrorString(err)); break; }
+      for (int i = 0; i < pq_engine::kSide && err == hipSuccess; ++i) err = hipStreamWaitEvent(eng->side[i], eng->fork_ev, 0);
+      if (err != hipSuccess) { rc = fail(PQ_E_HIP, hipGetErrorString(err)); break; }
+      int next = 0;
+      for (size_t ci = 0; ci < eng->work.size() && err == hipSuccess; ++ci) {
+        const auto& c = eng->work[ci];
+        const int la = kPairL[c.X][0], lb = kPairL[c.X][1], lc = kPairL[c.Y][0], ld = kPairL[c.Y][1];
+        for (int prec = 0; prec < 2 && err == hipSuccess; ++prec) {
+          const auto& tl = prec ? c.t32 : c.t64;
+          if (tl.empty()) continue;
+          err = launch_eri_class(la, lb, lc, ld, MODE_JK, prec, eng->view(), prec ? c.d_t32 : c.d_t64,
+                                 (int)tl.size(), eng->d_sorted[c.Y], D_dev, eng->d_Jacc, eng->d_Kacc, nullptr,
+                                 nullptr, eng->side[next]);
+          next = (next + 1) % pq_engine::kSide;
+        }
+      }
+      for (int i = 0; i < pq_engine::kSide && err == hipSuccess; ++i) {
+        err = hipEventRecord(eng->join_ev[i], eng->side[i]);
+        if (err == hipSuccess) err = hipStreamWaitEvent(st, eng->join_ev[i], 0);
+      }
+      if (err != hipSuccess) { rc = fail(PQ_E_HIP, hipGetErrorString(err)); break; }
+    }
+    for (size_t ci = 0; ms && ci < eng->work.size() && rc == PQ_OK; ++ci) {
       const auto& c = eng->work[ci];
       const int la = kPairL[c.X][0], lb = kPairL[c.X][1], lc = kPairL[c.Y][0], ld = kPairL[c.Y][1];
       for (int prec = 0; prec < 2; ++prec) {

@@ -72,18 +72,20 @@ __device__ __forceinline__ PrimQ prim_setup(const PairInfo& bp, const PrimPair& 
 // b in [B0, B0+NBS) and every ket component; PyQuante component order, without
 // the per-component normalisation factor.
 template <int LA, int LB, int LC, int LD, int A0, int NAS, int B0, int NBS, typename T>
-__device__ __forceinline__ void eri_block(const EngineView& ev, const PairInfo& bp, const PairInfo& kp,
-                                          T (&out)[NAS * NBS * ncart(LC) * ncart(LD)]) {
+__device__ __forceinline__ void eri_block(const EngineView& ev, const PairInfo& bp, const PairInfo& kp, int i0,
+                                          int i1, T (&out)[NAS * NBS * ncart(LC) * ncart(LD)]) {
   using C = Cls<LA, LB, LC, LD>;
   constexpr int NR = C::NR;
 #pragma unroll
   for (int c = 0; c < NAS * NBS * C::NC * C::ND; ++c) out[c] = (T)0;
   const T AB[3] = {(T)bp.ABx, (T)bp.ABy, (T)bp.ABz};
   const T CD[3] = {(T)kp.ABx, (T)kp.ABy, (T)kp.ABz};
-  for (int i = 0; i < bp.pp_n; ++i) {
-    const PrimPair b = ev.prims[bp.pp_off + i];
-    for (int k = 0; k < kp.pp_n; ++k) {
-      const PrimPair q = ev.prims[kp.pp_off + k];
+  // ket primitives (per lane) outside, bra primitives (shared by the
+  // wavefront, scalar loads) inside
+  for (int k = 0; k < kp.pp_n; ++k) {
+    const PrimPair q = ev.prims[kp.pp_off + k];
+    for (int i = i0; i < i1; ++i) {
+      const PrimPair b = ev.prims[bp.pp_off + i];
       const PrimQ s = prim_setup(bp, b, kp, q);
       T u[NR], w[NR];
       rys_roots<NR, T>(s.X, u, w);
@@ -247,7 +249,8 @@ __device__ __forceinline__ void lane_body(const EngineView& ev, const Tile& tl, 
   constexpr int NO = NAS * NBS * C::NC * C::ND;
   T out[NO];
   if (active) {
-    eri_block<LA, LB, LC, LD, A0, NAS, B0, NBS, T>(ev, bp, kp, out);
+    const int i0 = tl.prim & 0xffff, ni = tl.prim >> 16;
+    eri_block<LA, LB, LC, LD, A0, NAS, B0, NBS, T>(ev, bp, kp, i0, ni ? i0 + ni : bp.pp_n, out);
   } else {
 #pragma unroll
     for (int c = 0; c < NO; ++c) out[c] = (T)0;
@@ -305,7 +308,11 @@ __global__ __launch_bounds__(256) void k_eri_lane(EngineView ev, const Tile* __r
   const int lane = threadIdx.x & 63;
   const int t = __builtin_amdgcn_readfirstlane(blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6));
   if (t >= ntiles) return;  // uniform per wavefront
-  const Tile tl = tiles[t];
+  Tile tl = tiles[t];
+  tl.bra = __builtin_amdgcn_readfirstlane(tl.bra);
+  tl.ket0 = __builtin_amdgcn_readfirstlane(tl.ket0);
+  tl.nket = __builtin_amdgcn_readfirstlane(tl.nket);
+  tl.prim = __builtin_amdgcn_readfirstlane(tl.prim);
   const bool active = lane < tl.nket;
   const int ket = kets[tl.ket0 + (active ? lane : 0)];
   const int bra = (MODE == MODE_DIAG) ? ket : tl.bra;
