@@ -31,14 +31,19 @@ __host__ __device__ constexpr int dfact(int n) { return n <= 1 ? 1 : n * dfact(n
 // Per-component part of the primitive normalisation (primitive-gto.c:76-83):
 // N = Nrad(alpha, L) / sqrt((2l-1)!!(2m-1)!!(2n-1)!!).  Nrad is folded into
 // the primitive-pair prefactor, this factor is applied per component.
-__host__ __device__ constexpr double const_sqrt(double x) {
-  double r = x > 1.0 ? x : 1.0;
-  for (int it = 0; it < 64; ++it) r = 0.5 * (r + x / r);
-  return r;
+// 1/sqrt(n) for the double-factorial products that occur up to L = 4; kept
+// as literals so that the factor folds to a constant after loop unrolling.
+__host__ __device__ constexpr double inv_sqrt_dfprod(int n) {
+  return n == 1 ? 1.0
+       : n == 3 ? 0.57735026918962576451
+       : n == 9 ? 0.33333333333333333333
+       : n == 15 ? 0.25819888974716112568
+       : n == 105 ? 0.09759000729485331981
+       : 0.0;
 }
 __host__ __device__ constexpr double cart_norm(int L, int i) {
-  return 1.0 / const_sqrt((double)(dfact(2 * cart_pow(L, i, 0) - 1) * dfact(2 * cart_pow(L, i, 1) - 1) *
-                                   dfact(2 * cart_pow(L, i, 2) - 1)));
+  return inv_sqrt_dfprod(dfact(2 * cart_pow(L, i, 0) - 1) * dfact(2 * cart_pow(L, i, 1) - 1) *
+                         dfact(2 * cart_pow(L, i, 2) - 1));
 }
 
 // ---------------------------------------------------------------------
@@ -102,6 +107,54 @@ __device__ __forceinline__ void rys_roots(double X, T* u, T* w) {
     for (int j = RYS_DEG - 1; j >= 0; --j) v = v * s + (T)c[f * (RYS_DEG + 1) + j];
     if (f < NR) u[f] = v; else w[f - NR] = v;
   }
+}
+
+// One root r of the order-NR rule (large heavy-class bodies keep the root loop
+// rolled to bound code size).  `asym` and `c`/`sv` are set by rys_prepare.
+template <int NR>
+struct RysPoint {
+  bool asym;
+  double rx, rs;        // 1/X, 1/sqrt(X) on the large-X branch
+  const double* c;      // coefficient block of the interval
+  double sv;            // local variable on the interval
+};
+
+template <int NR>
+__device__ __forceinline__ RysPoint<NR> rys_prepare(double X) {
+  RysPoint<NR> p;
+  p.asym = X >= rys_xasym[NR];
+  if (p.asym) {
+    p.rx = 1.0 / X;
+    p.rs = rsqrt(X);
+    p.c = nullptr;
+    p.sv = 0.0;
+  } else {
+    const int k = (int)X;
+    p.rx = p.rs = 0.0;
+    p.sv = 2.0 * (X - (double)k) - 1.0;
+    p.c = rys_coef + rys_off[NR - 1] + k * (2 * NR * (RYS_DEG + 1));
+  }
+  return p;
+}
+
+template <int NR, typename T>
+__device__ __forceinline__ void rys_root(const RysPoint<NR>& p, int r, T& u, T& w) {
+  if (p.asym) {
+    u = (T)(rys_asym[NR - 1][r] * p.rx);
+    w = (T)(rys_asym[NR - 1][NR + r] * p.rs);
+    return;
+  }
+  const T s = (T)p.sv;
+  const double* cu = p.c + r * (RYS_DEG + 1);
+  const double* cw = p.c + (NR + r) * (RYS_DEG + 1);
+  T vu = (T)cu[RYS_DEG], vw = (T)cw[RYS_DEG];
+#pragma unroll
+  for (int j = RYS_DEG - 1; j >= 0; --j) {
+    vu = vu * s + (T)cu[j];
+    vw = vw * s + (T)cw[j];
+  }
+  u = vu;
+  w = vw;
 }
 
 // Runtime-order variant (scalar entry points and one-electron kernels).

@@ -82,18 +82,22 @@ __device__ __forceinline__ void eri_block(const EngineView& ev, const PairInfo& 
   const T CD[3] = {(T)kp.ABx, (T)kp.ABy, (T)kp.ABz};
   // ket primitives (per lane) outside, bra primitives (shared by the
   // wavefront, scalar loads) inside
+#pragma unroll 1
   for (int k = 0; k < kp.pp_n; ++k) {
     const PrimPair q = ev.prims[kp.pp_off + k];
+#pragma unroll 1
     for (int i = i0; i < i1; ++i) {
       const PrimPair b = ev.prims[bp.pp_off + i];
       const PrimQ s = prim_setup(bp, b, kp, q);
-      T u[NR], w[NR];
-      rys_roots<NR, T>(s.X, u, w);
+      const RysPoint<NR> rp = rys_prepare<NR>(s.X);
       const T pref = (T)s.pref, qfac = (T)s.qfac, pfac = (T)s.pfac;
       const T half_rpq = (T)(0.5 * s.rpq), i2p = (T)s.i2p, i2q = (T)s.i2q;
-#pragma unroll
+      // roots unrolled for the small orders; rolled above to bound code size
+      constexpr int UNR = NR <= 2 ? NR : 1;
+#pragma unroll UNR
       for (int r = 0; r < NR; ++r) {
-        const T ur = u[r];
+        T ur, wr;
+        rys_root<NR, T>(rp, r, ur, wr);
         const T B00 = half_rpq * ur;
         const T B10 = i2p * ((T)1 - qfac * ur);
         const T B01 = i2q * ((T)1 - pfac * ur);
@@ -103,7 +107,7 @@ __device__ __forceinline__ void eri_block(const EngineView& ev, const PairInfo& 
                                   (T)s.QC[0] + pfac * (T)s.PQ[0] * ur, B10, B01, B00, AB[0], CD[0], Ix);
         rys_1d<LA, LB, LC, LD, T>((T)1, (T)s.PA[1] - qfac * (T)s.PQ[1] * ur,
                                   (T)s.QC[1] + pfac * (T)s.PQ[1] * ur, B10, B01, B00, AB[1], CD[1], Iy);
-        rys_1d<LA, LB, LC, LD, T>(w[r] * pref, (T)s.PA[2] - qfac * (T)s.PQ[2] * ur,
+        rys_1d<LA, LB, LC, LD, T>(wr * pref, (T)s.PA[2] - qfac * (T)s.PQ[2] * ur,
                                   (T)s.QC[2] + pfac * (T)s.PQ[2] * ur, B10, B01, B00, AB[2], CD[2], Iz);
 #pragma unroll
         for (int a = 0; a < NAS; ++a)
@@ -132,11 +136,16 @@ __device__ __forceinline__ void eri_block(const EngineView& ev, const PairInfo& 
 //   Kacc[b,c] += f v D[a,d],    Kacc[b,d] += f v D[a,c],
 // and J = Jacc + Jacc^T, K = Kacc + Kacc^T (symmetric D).
 // ---------------------------------------------------------------------
+// krow, when non-null, is this wavefront's LDS copy of the bra rows of K
+// (rows a in [A0, A0+NAS) then b in [B0, B0+NBS), nbf columns each): the four
+// K blocks then accumulate there with LDS atomics and leave the wavefront as
+// coalesced global atomics (flush_krows), one cache line per 16 columns
+// instead of one per lane.
 template <int LA, int LB, int LC, int LD, int A0, int NAS, int B0, int NBS, typename T>
 __device__ __forceinline__ void digest_lane(const EngineView& ev, const PairInfo& bp, const PairInfo& kp,
                                             const T (&out)[NAS * NBS * ncart(LC) * ncart(LD)], double f,
                                             bool active, const double* __restrict__ D, double* Jacc,
-                                            double* Kacc) {
+                                            double* Kacc, double* krow) {
   using C = Cls<LA, LB, LC, LD>;
   const int n = ev.nbf;
   const int a0 = bp.bfA + A0, b0 = bp.bfB + B0, c0 = kp.bfA, d0 = kp.bfB;
@@ -194,6 +203,23 @@ __device__ __forceinline__ void digest_lane(const EngineView& ev, const PairInfo
   for (int c = 0; c < C::NC; ++c)
 #pragma unroll
     for (int d = 0; d < C::ND; ++d) unsafeAtomicAdd(&Jacc[(c0 + c) * n + d0 + d], 2.0 * jcd[c][d]);
+  if (krow) {
+#pragma unroll
+    for (int a = 0; a < NAS; ++a) {
+#pragma unroll
+      for (int c = 0; c < C::NC; ++c) atomicAdd(&krow[a * n + c0 + c], kac[a][c]);
+#pragma unroll
+      for (int d = 0; d < C::ND; ++d) atomicAdd(&krow[a * n + d0 + d], kad[a][d]);
+    }
+#pragma unroll
+    for (int b = 0; b < NBS; ++b) {
+#pragma unroll
+      for (int c = 0; c < C::NC; ++c) atomicAdd(&krow[(NAS + b) * n + c0 + c], kbc[b][c]);
+#pragma unroll
+      for (int d = 0; d < C::ND; ++d) atomicAdd(&krow[(NAS + b) * n + d0 + d], kbd[b][d]);
+    }
+    return;
+  }
 #pragma unroll
   for (int a = 0; a < NAS; ++a) {
 #pragma unroll
@@ -239,10 +265,27 @@ __device__ __forceinline__ void atomic_max_nonneg(double* p, double v) {
 }
 
 // One lane, one shell quartet, bra components [A0, A0+NAS) x [B0, B0+NBS).
+// Flush this wavefront's LDS K rows with coalesced global atomics (zeros skipped).
+template <int NAS, int NBS>
+__device__ __forceinline__ void flush_krows(double* krow, int n, int a0, int b0, double* Kacc) {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  const int lane = threadIdx.x & 63;
+#pragma unroll
+  for (int r = 0; r < NAS + NBS; ++r) {
+    const int grow = r < NAS ? a0 + r : b0 + (r - NAS);
+    for (int col = lane; col < n; col += 64) {
+      const double v = krow[r * n + col];
+      if (v != 0.0) unsafeAtomicAdd(&Kacc[grow * n + col], v);
+    }
+  }
+}
+
 template <int LA, int LB, int LC, int LD, typename T, int MODE, int A0, int NAS, int B0, int NBS>
 __device__ __forceinline__ void lane_body(const EngineView& ev, const Tile& tl, bool active, int bra, int ket,
                                           const double* __restrict__ D, double* Jacc, double* Kacc,
-                                          double* ints, double* qout) {
+                                          double* ints, double* qout, double* krow) {
   using C = Cls<LA, LB, LC, LD>;
   const PairInfo bp = ev.pairs[bra];
   const PairInfo kp = ev.pairs[ket];
@@ -257,7 +300,15 @@ __device__ __forceinline__ void lane_body(const EngineView& ev, const Tile& tl, 
   }
   if constexpr (MODE == MODE_JK) {
     const double f = active ? quartet_degeneracy(bp, kp, bra, ket) : 0.0;
-    digest_lane<LA, LB, LC, LD, A0, NAS, B0, NBS, T>(ev, bp, kp, out, f, active, D, Jacc, Kacc);
+    if (krow) {
+      const int lane = threadIdx.x & 63;
+      for (int i = lane; i < (NAS + NBS) * ev.nbf; i += 64) krow[i] = 0.0;
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    }
+    digest_lane<LA, LB, LC, LD, A0, NAS, B0, NBS, T>(ev, bp, kp, out, f, active, D, Jacc, Kacc, krow);
+    if (krow) flush_krows<NAS, NBS>(krow, ev.nbf, bp.bfA + A0, bp.bfB + B0, Kacc);
   } else if constexpr (MODE == MODE_STORE) {
     if (!active) return;
     const bool same = bra == ket;
@@ -303,10 +354,13 @@ template <int LA, int LB, int LC, int LD, typename T, int MODE>
 __global__ __launch_bounds__(256) void k_eri_lane(EngineView ev, const Tile* __restrict__ tiles, int ntiles,
                                                   const int* __restrict__ kets, const double* __restrict__ D,
                                                   double* Jacc, double* Kacc, double* ints,
-                                                  double* qout) {
+                                                  double* qout, int lds_rows) {
   using C = Cls<LA, LB, LC, LD>;
+  extern __shared__ double pq_lds[];
   const int lane = threadIdx.x & 63;
-  const int t = __builtin_amdgcn_readfirstlane(blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6));
+  const int wv = threadIdx.x >> 6;
+  double* krow = lds_rows > 0 ? pq_lds + wv * lds_rows * ev.nbf : nullptr;
+  const int t = __builtin_amdgcn_readfirstlane(blockIdx.x * (blockDim.x >> 6) + wv);
   if (t >= ntiles) return;  // uniform per wavefront
   Tile tl = tiles[t];
   tl.bra = __builtin_amdgcn_readfirstlane(tl.bra);
@@ -317,17 +371,18 @@ __global__ __launch_bounds__(256) void k_eri_lane(EngineView ev, const Tile* __r
   const int ket = kets[tl.ket0 + (active ? lane : 0)];
   const int bra = (MODE == MODE_DIAG) ? ket : tl.bra;
   if constexpr (C::SPLIT == 0) {
-    lane_body<LA, LB, LC, LD, T, MODE, 0, C::NA, 0, C::NB>(ev, tl, active, bra, ket, D, Jacc, Kacc, ints, qout);
+    lane_body<LA, LB, LC, LD, T, MODE, 0, C::NA, 0, C::NB>(ev, tl, active, bra, ket, D, Jacc, Kacc, ints, qout,
+                                                          krow);
   } else if constexpr (C::SPLIT == 1) {
     static_switch<0, C::NA>(blockIdx.y, [&](auto ia) {
       lane_body<LA, LB, LC, LD, T, MODE, decltype(ia)::value, 1, 0, C::NB>(ev, tl, active, bra, ket, D, Jacc,
-                                                                            Kacc, ints, qout);
+                                                                            Kacc, ints, qout, krow);
     });
   } else {
     static_switch<0, C::NA * C::NB>(blockIdx.y, [&](auto iab) {
       constexpr int v = decltype(iab)::value;
       lane_body<LA, LB, LC, LD, T, MODE, v / C::NB, 1, v % C::NB, 1>(ev, tl, active, bra, ket, D, Jacc, Kacc,
-                                                                      ints, qout);
+                                                                      ints, qout, krow);
     });
   }
 }
@@ -341,9 +396,17 @@ static hipError_t launch_one(const EngineView& ev, const Tile* tiles, int ntiles
                              hipStream_t st) {
   if (ntiles <= 0) return hipSuccess;
   using C = Cls<LA, LB, LC, LD>;
+  // K bra rows staged in LDS (one wavefront per block) when they fit in 64 KB
+  constexpr int rows = C::SPLIT == 0 ? C::NA + C::NB : (C::SPLIT == 1 ? 1 + C::NB : 2);
+  const size_t lds = (size_t)rows * ev.nbf * sizeof(double);
+  if (MODE == MODE_JK && lds <= 65536) {
+    hipLaunchKernelGGL((k_eri_lane<LA, LB, LC, LD, T, MODE>), dim3(ntiles, C::NSPLIT), dim3(64), lds, st, ev,
+                       tiles, ntiles, kets, D, Jacc, Kacc, ints, qout, rows);
+    return hipGetLastError();
+  }
   const int wpb = 4;
   hipLaunchKernelGGL((k_eri_lane<LA, LB, LC, LD, T, MODE>), dim3((ntiles + wpb - 1) / wpb, C::NSPLIT),
-                     dim3(64 * wpb), 0, st, ev, tiles, ntiles, kets, D, Jacc, Kacc, ints, qout);
+                     dim3(64 * wpb), 0, st, ev, tiles, ntiles, kets, D, Jacc, Kacc, ints, qout, 0);
   return hipGetLastError();
 }
 
