@@ -138,6 +138,14 @@ int pq_digest_stored(int nbf, const double *ints_dev, uint64_t len, const double
  * (h = T + V), nbf x nbf row-major FP64, on the device. */
 int pq_get1ints(pq_engine *eng, double *S_dev, double *T_dev, double *V_dev, void *stream);
 
+/* RHF nuclear gradient dE/dR (force.py:31-70 rhf_force; der2JmK
+ * force.py:318-357; AnalyticDerivatives.py:27-675), natom x 3 row-major into
+ * grad_dev (hartree/bohr): 2 tr(D dH) + sum (ij|kl)^x [2 D_ij D_kl - D_ik D_jl]
+ * - 2 tr(W dS) + dV_nn, with D = C_occ C_occ^T and the energy-weighted
+ * W = C_occ diag(e_occ) C_occ^T (Wavefunction.py:143-160).  Derivative
+ * integrals are contracted on the fly over the current (FP64) work lists. */
+int pq_forces(pq_engine *eng, const double *D_dev, const double *W_dev, double *grad_dev, void *stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -114,6 +114,8 @@ struct pq_engine {
   bool store_ok = false;
   double* d_Jacc = nullptr;
   double* d_Kacc = nullptr;
+  double* d_gacc = nullptr;  // private gradient copies
+  size_t gacc_n = 0;
 
   // side streams (default priority) for running class kernels concurrently
   static constexpr int kSide = 4;
@@ -155,6 +157,7 @@ struct pq_engine {
     dev_free(d_atom_z);
     dev_free(d_Jacc);
     dev_free(d_Kacc);
+    dev_free(d_gacc);
   }
 };
 
@@ -742,6 +745,36 @@ int pq_digest_stored(int nbf, const double* ints_dev, uint64_t len, const double
   dev_free(Jacc);
   dev_free(Kacc);
   PQ_HIP(err);
+  return PQ_OK;
+}
+
+int pq_forces(pq_engine* eng, const double* D_dev, const double* W_dev, double* grad_dev, void* stream) {
+  if (!eng || !D_dev || !W_dev || !grad_dev) return fail(PQ_E_ARG, "null argument");
+  if (!eng->work_ok) return fail(PQ_E_STATE, "pq_build_pairs must be called before pq_forces");
+  hipStream_t st = (hipStream_t)stream;
+  const int ncopy = kGradCopies;
+  const size_t gsize = (size_t)ncopy * 3 * eng->natom;
+  if (eng->gacc_n < gsize) {
+    dev_free(eng->d_gacc);
+    PQ_HIP(dev_alloc(&eng->d_gacc, gsize));
+    eng->gacc_n = gsize;
+  }
+  PQ_HIP(launch_fill(eng->d_gacc, (int64_t)gsize, 0.0, st));
+  for (const auto& c : eng->work) {
+    const int la = kPairL[c.X][0], lb = kPairL[c.X][1], lc = kPairL[c.Y][0], ld = kPairL[c.Y][1];
+    if (!c.t64.empty())
+      PQ_HIP(launch_grad_class(la, lb, lc, ld, eng->view(), c.d_t64, (int)c.t64.size(), eng->d_sorted[c.Y], D_dev,
+                               eng->d_gacc, eng->natom, st));
+    if (!c.t32.empty())
+      PQ_HIP(launch_grad_class(la, lb, lc, ld, eng->view(), c.d_t32, (int)c.t32.size(), eng->d_sorted[c.Y], D_dev,
+                               eng->d_gacc, eng->natom, st));
+  }
+  for (int c = 0; c < 6; ++c) {
+    if (eng->members[c].empty()) continue;
+    PQ_HIP(launch_onee_grad(kPairL[c][0], kPairL[c][1], eng->view(), eng->d_members[c], (int)eng->members[c].size(),
+                            eng->d_atom_xyz, eng->d_atom_z, eng->natom, D_dev, W_dev, eng->d_gacc, st));
+  }
+  PQ_HIP(launch_grad_finish(eng->d_gacc, ncopy, eng->natom, eng->d_atom_xyz, eng->d_atom_z, grad_dev, st));
   return PQ_OK;
 }
 

@@ -39,6 +39,32 @@ PQ_DECL1(2, 0, 2, 0, 3)
 PQ_DECL1(2, 1, 2, 1, 3)
 PQ_DECL1(2, 2, 2, 2, 3)
 #undef PQ_DECL
+
+#define PQ_GDECL(A, B, C_, D_)                                                                     \
+  hipError_t launch_cls_##A##B##C_##D_##_m4(const EngineView& ev, const Tile* tiles, int ntiles, const int* kets, \
+                                            const double* D, double* gacc, int natom, hipStream_t st);
+PQ_GDECL(0, 0, 0, 0)
+PQ_GDECL(1, 0, 0, 0)
+PQ_GDECL(1, 0, 1, 0)
+PQ_GDECL(1, 1, 0, 0)
+PQ_GDECL(1, 1, 1, 0)
+PQ_GDECL(1, 1, 1, 1)
+PQ_GDECL(2, 0, 0, 0)
+PQ_GDECL(2, 0, 1, 0)
+PQ_GDECL(2, 0, 1, 1)
+PQ_GDECL(2, 0, 2, 0)
+PQ_GDECL(2, 1, 0, 0)
+PQ_GDECL(2, 1, 1, 0)
+PQ_GDECL(2, 1, 1, 1)
+PQ_GDECL(2, 1, 2, 0)
+PQ_GDECL(2, 1, 2, 1)
+PQ_GDECL(2, 2, 0, 0)
+PQ_GDECL(2, 2, 1, 0)
+PQ_GDECL(2, 2, 1, 1)
+PQ_GDECL(2, 2, 2, 0)
+PQ_GDECL(2, 2, 2, 1)
+PQ_GDECL(2, 2, 2, 2)
+#undef PQ_GDECL
 #undef PQ_DECL1
 
 // ---------------------------------------------------------------------
@@ -219,6 +245,37 @@ hipError_t launch_eri_class(int la, int lb, int lc, int ld, int mode, int fp32, 
   PQ_CASE(2, 2, 2, 1)
   PQ_CASE(2, 2, 2, 2)
 #undef PQ_CASE
+  return hipErrorInvalidValue;
+}
+
+
+hipError_t launch_grad_class(int la, int lb, int lc, int ld, const EngineView& ev, const Tile* tiles, int ntiles,
+                             const int* kets, const double* D, double* gacc, int natom, hipStream_t st) {
+#define PQ_GCASE(A, B, C_, D_)                            \
+  if (la == A && lb == B && lc == C_ && ld == D_)         \
+    return launch_cls_##A##B##C_##D_##_m4(ev, tiles, ntiles, kets, D, gacc, natom, st);
+  PQ_GCASE(0, 0, 0, 0)
+  PQ_GCASE(1, 0, 0, 0)
+  PQ_GCASE(1, 0, 1, 0)
+  PQ_GCASE(1, 1, 0, 0)
+  PQ_GCASE(1, 1, 1, 0)
+  PQ_GCASE(1, 1, 1, 1)
+  PQ_GCASE(2, 0, 0, 0)
+  PQ_GCASE(2, 0, 1, 0)
+  PQ_GCASE(2, 0, 1, 1)
+  PQ_GCASE(2, 0, 2, 0)
+  PQ_GCASE(2, 1, 0, 0)
+  PQ_GCASE(2, 1, 1, 0)
+  PQ_GCASE(2, 1, 1, 1)
+  PQ_GCASE(2, 1, 2, 0)
+  PQ_GCASE(2, 1, 2, 1)
+  PQ_GCASE(2, 2, 0, 0)
+  PQ_GCASE(2, 2, 1, 0)
+  PQ_GCASE(2, 2, 1, 1)
+  PQ_GCASE(2, 2, 2, 0)
+  PQ_GCASE(2, 2, 2, 1)
+  PQ_GCASE(2, 2, 2, 2)
+#undef PQ_GCASE
   return hipErrorInvalidValue;
 }
 

@@ -1,13 +1,21 @@
 // pq_eri_cls.hip -- one angular-momentum class and one mode of the Rys ERI
 // kernels per translation unit (PQ_LA..PQ_LD and PQ_MODE given on the
 // compiler command line: 0 J/K FP64, 1 J/K FP32, 2 store, 3 Schwarz
-// diagonal), so the instantiations compile in parallel.
+// diagonal, 4 nuclear gradient), so the instantiations compile in parallel.
 #include "pq_eri_impl.h"
+#include "pq_grad_impl.h"
 
 #define PQ_CAT2(a, b, c, d, m) launch_cls_##a##b##c##d##_m##m
 #define PQ_CAT(a, b, c, d, m) PQ_CAT2(a, b, c, d, m)
 
 namespace pq {
+#if PQ_MODE == 4
+hipError_t PQ_CAT(PQ_LA, PQ_LB, PQ_LC, PQ_LD, PQ_MODE)(const EngineView& ev, const Tile* tiles, int ntiles,
+                                                       const int* kets, const double* D, double* gacc, int natom,
+                                                       hipStream_t st) {
+  return launch_grad_one<PQ_LA, PQ_LB, PQ_LC, PQ_LD>(ev, tiles, ntiles, kets, D, gacc, natom, st);
+}
+#else
 hipError_t PQ_CAT(PQ_LA, PQ_LB, PQ_LC, PQ_LD, PQ_MODE)(const EngineView& ev, const Tile* tiles, int ntiles,
                                                        const int* kets, const double* D, double* Jacc,
                                                        double* Kacc, double* ints, double* qout, hipStream_t st) {
@@ -25,4 +33,5 @@ hipError_t PQ_CAT(PQ_LA, PQ_LB, PQ_LC, PQ_LD, PQ_MODE)(const EngineView& ev, con
                                                                    qout, st);
 #endif
 }
+#endif
 }  // namespace pq

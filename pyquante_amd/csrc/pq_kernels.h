@@ -11,9 +11,27 @@ enum { MODE_JK = 0, MODE_STORE = 1, MODE_DIAG = 2 };
 // classes with at most this many Cartesian integrals run one lane per quartet
 constexpr int kLaneMaxComp = 81;
 
+// private copies of the natom x 3 gradient that the gradient kernels spread
+// their atomics over (summed by launch_grad_finish)
+constexpr int kGradCopies = 256;
+
 hipError_t launch_eri_class(int la, int lb, int lc, int ld, int mode, int fp32, const EngineView& ev,
                             const Tile* tiles, int ntiles, const int* kets, const double* D, double* Jacc,
                             double* Kacc, double* ints, double* qout, hipStream_t st);
+
+// two-electron nuclear gradient into kGradCopies private copies of the
+// natom x 3 gradient (gacc), summed by launch_grad_finish
+hipError_t launch_grad_class(int la, int lb, int lc, int ld, const EngineView& ev, const Tile* tiles, int ntiles,
+                             const int* kets, const double* D, double* gacc, int natom, hipStream_t st);
+
+// one-electron gradient (2 tr(D dh) - 2 tr(W dS)) of the pairs of one class
+hipError_t launch_onee_grad(int la, int lb, const EngineView& ev, const int* pair_ids, int npairs,
+                            const double* atom_xyz, const double* atom_z, int natom, const double* D,
+                            const double* W, double* gacc, hipStream_t st);
+
+// grad = sum of the copies + nuclear-repulsion gradient
+hipError_t launch_grad_finish(const double* gacc, int ncopies, int natom, const double* atom_xyz,
+                              const double* atom_z, double* grad, hipStream_t st);
 
 hipError_t launch_symmetrize(int n, const double* Jacc, const double* Kacc, double* J, double* K,
                              double jscale, double kscale, hipStream_t st);

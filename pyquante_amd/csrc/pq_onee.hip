@@ -172,6 +172,217 @@ hipError_t launch_onee(int la, int lb, const EngineView& ev, const int* pair_ids
 }
 
 // ---------------------------------------------------------------------
+// One-electron part of the RHF nuclear gradient (force.py:48-70: 2 tr(D dH)
+// - 2 tr(Q dS), der_Hcore_element / der_overlap_element of
+// AnalyticDerivatives.py:27-455): centre derivatives of S, T and V by the
+// shift rule d/dA_x g(l) = 2 alpha g(l+1) - l g(l-1) on the primitives; the
+// nucleus-C derivative of V from translational invariance of the
+// three-centre integral; d/dB of S and T = -d/dA.  One lane per shell pair,
+// contracted at once with D and W; results go to kGradCopies private copies.
+// ---------------------------------------------------------------------
+template <int LA, int LB>
+__global__ __launch_bounds__(128) void k_onee_grad(EngineView ev, const int* __restrict__ pair_ids, int npairs,
+                                                   const double* __restrict__ atom_xyz,
+                                                   const double* __restrict__ atom_z, int natom,
+                                                   const double* __restrict__ D, const double* __restrict__ W,
+                                                   double* gacc) {
+  constexpr int NA = ncart(LA), NB = ncart(LB);
+  constexpr int NRV = (LA + LB + 1) / 2 + 1;
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= npairs) return;
+  const PairInfo bp = ev.pairs[pair_ids[t]];
+  const int n = ev.nbf;
+  double* gcopy = gacc + (size_t)(t % kGradCopies) * 3 * natom;
+  const double A[3] = {bp.Ax, bp.Ay, bp.Az};
+  const double B[3] = {bp.Ax - bp.ABx, bp.Ay - bp.ABy, bp.Az - bp.ABz};
+  const double sym = bp.shA == bp.shB ? 1.0 : 2.0;
+  double dw[NA][NB], ww[NA][NB];
+#pragma unroll
+  for (int a = 0; a < NA; ++a)
+#pragma unroll
+    for (int b = 0; b < NB; ++b) {
+      const double nf = cart_norm(LA, a) * cart_norm(LB, b);
+      dw[a][b] = 2.0 * sym * nf * D[(bp.bfA + a) * n + bp.bfB + b];
+      ww[a][b] = 2.0 * sym * nf * W[(bp.bfA + a) * n + bp.bfB + b];
+    }
+  double gA[3] = {0, 0, 0}, gB[3] = {0, 0, 0};
+  for (int ip = 0; ip < bp.pp_n; ++ip) {
+    const PrimPair pp = ev.prims[bp.pp_off + ip];
+    const double p = pp.p, alp = ev.prim_a[bp.pp_off + ip], beta = p - alp;
+    const double P[3] = {pp.Px, pp.Py, pp.Pz};
+    const double i2p = 0.5 / p;
+    // 1-D overlaps, i <= LA + 1, j <= LB + 2
+    double O[3][LA + 2][LB + 3];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      const double PA = P[d] - A[d], PB = P[d] - B[d];
+      O[d][0][0] = 1.0;
+#pragma unroll
+      for (int i = 0; i <= LA + 1; ++i) {
+        if (i > 0) O[d][i][0] = PA * O[d][i - 1][0] + (i > 1 ? (i - 1) * i2p * O[d][i - 2][0] : 0.0);
+#pragma unroll
+        for (int j = 1; j <= LB + 2; ++j)
+          O[d][i][j] = PB * O[d][i][j - 1] + i2p * ((i > 0 ? i * O[d][i - 1][j - 1] : 0.0) +
+                                                    (j > 1 ? (j - 1) * O[d][i][j - 2] : 0.0));
+      }
+    }
+    const double pre = pp.Kp * p * pow(M_PI / p, 1.5);
+    // kinetic integral with bra powers (ax, ay, az) and ket powers (bx, by, bz)
+    auto tk = [&](int ax, int ay, int az, int bx, int by, int bz) -> double {
+      const double s0 = O[0][ax][bx] * O[1][ay][by] * O[2][az][bz];
+      double t1 = O[0][ax][bx + 2] * O[1][ay][by] * O[2][az][bz] + O[0][ax][bx] * O[1][ay][by + 2] * O[2][az][bz] +
+                  O[0][ax][bx] * O[1][ay][by] * O[2][az][bz + 2];
+      double t2 = 0.0;
+      if (bx >= 2) t2 += bx * (bx - 1) * O[0][ax][bx - 2] * O[1][ay][by] * O[2][az][bz];
+      if (by >= 2) t2 += by * (by - 1) * O[0][ax][bx] * O[1][ay][by - 2] * O[2][az][bz];
+      if (bz >= 2) t2 += bz * (bz - 1) * O[0][ax][bx] * O[1][ay][by] * O[2][az][bz - 2];
+      return beta * (2 * (bx + by + bz) + 3) * s0 - 2.0 * beta * beta * t1 - 0.5 * t2;
+    };
+#pragma unroll
+    for (int a = 0; a < NA; ++a)
+#pragma unroll
+      for (int b = 0; b < NB; ++b) {
+        const int ap[3] = {cart_pow(LA, a, 0), cart_pow(LA, a, 1), cart_pow(LA, a, 2)};
+        const int bq[3] = {cart_pow(LB, b, 0), cart_pow(LB, b, 1), cart_pow(LB, b, 2)};
+#pragma unroll
+        for (int x = 0; x < 3; ++x) {
+          int up[3] = {ap[0], ap[1], ap[2]}, dn[3] = {ap[0], ap[1], ap[2]};
+          up[x] += 1;
+          dn[x] -= 1;
+          const double sup = O[0][up[0]][bq[0]] * O[1][up[1]][bq[1]] * O[2][up[2]][bq[2]];
+          const double sdn = ap[x] ? O[0][dn[0] < 0 ? 0 : dn[0]][bq[0]] * O[1][dn[1] < 0 ? 0 : dn[1]][bq[1]] *
+                                         O[2][dn[2] < 0 ? 0 : dn[2]][bq[2]]
+                                   : 0.0;
+          const double dS = pre * (2.0 * alp * sup - ap[x] * sdn);
+          const double tup = tk(up[0], up[1], up[2], bq[0], bq[1], bq[2]);
+          const double tdn = ap[x] ? tk(dn[0] < 0 ? 0 : dn[0], dn[1] < 0 ? 0 : dn[1], dn[2] < 0 ? 0 : dn[2], bq[0],
+                                        bq[1], bq[2])
+                                   : 0.0;
+          const double dT = pre * (2.0 * alp * tup - ap[x] * tdn);
+          const double g = dw[a][b] * dT - ww[a][b] * dS;
+          gA[x] += g;
+          gB[x] -= g;
+        }
+      }
+    // nuclear attraction: derivatives on A, B and the nucleus
+    const double vpre = -2.0 * M_PI * pp.Kp;
+    for (int c = 0; c < natom; ++c) {
+      const double Cc[3] = {atom_xyz[3 * c], atom_xyz[3 * c + 1], atom_xyz[3 * c + 2]};
+      const double PC[3] = {P[0] - Cc[0], P[1] - Cc[1], P[2] - Cc[2]};
+      const double X = p * (PC[0] * PC[0] + PC[1] * PC[1] + PC[2] * PC[2]);
+      double u[NRV], w[NRV];
+      rys_roots<NRV, double>(X, u, w);
+      const double zc = atom_z[c] * vpre;
+      double vA[3] = {0, 0, 0}, vB[3] = {0, 0, 0};
+#pragma unroll
+      for (int r = 0; r < NRV; ++r) {
+        const double B10 = i2p * (1.0 - u[r]);
+        double I[3][LA + 2][LB + 2];
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+          const double C00 = (P[d] - A[d]) - u[r] * PC[d];
+          const double AB = A[d] - B[d];
+          constexpr int NN = LA + LB + 2;
+          double G[NN + 1];
+          G[0] = 1.0;
+          G[1] = C00;
+#pragma unroll
+          for (int m = 2; m <= NN; ++m) G[m] = C00 * G[m - 1] + (m - 1) * B10 * G[m - 2];
+          double H[LB + 2][NN + 1];
+#pragma unroll
+          for (int m = 0; m <= NN; ++m) H[0][m] = G[m];
+#pragma unroll
+          for (int j = 1; j <= LB + 1; ++j)
+#pragma unroll
+            for (int m = 0; m <= NN - j; ++m) H[j][m] = H[j - 1][m + 1] + AB * H[j - 1][m];
+#pragma unroll
+          for (int i = 0; i <= LA + 1; ++i)
+#pragma unroll
+            for (int j = 0; j <= LB + 1; ++j) I[d][i][j] = H[j][i];
+        }
+        const double wz = w[r] * zc;
+#pragma unroll
+        for (int a = 0; a < NA; ++a)
+#pragma unroll
+          for (int b = 0; b < NB; ++b) {
+            const int ap[3] = {cart_pow(LA, a, 0), cart_pow(LA, a, 1), cart_pow(LA, a, 2)};
+            const int bq[3] = {cart_pow(LB, b, 0), cart_pow(LB, b, 1), cart_pow(LB, b, 2)};
+            const double i0[3] = {I[0][ap[0]][bq[0]], I[1][ap[1]][bq[1]], I[2][ap[2]][bq[2]]};
+            const double g = dw[a][b] * wz;
+#pragma unroll
+            for (int x = 0; x < 3; ++x) {
+              const int y = (x + 1) % 3, z = (x + 2) % 3;
+              const double oth = i0[y] * i0[z];
+              const double da = 2.0 * alp * I[x][ap[x] + 1][bq[x]] - (ap[x] ? ap[x] * I[x][ap[x] ? ap[x] - 1 : 0][bq[x]] : 0.0);
+              const double db = 2.0 * beta * I[x][ap[x]][bq[x] + 1] - (bq[x] ? bq[x] * I[x][ap[x]][bq[x] ? bq[x] - 1 : 0] : 0.0);
+              vA[x] += g * da * oth;
+              vB[x] += g * db * oth;
+            }
+          }
+      }
+#pragma unroll
+      for (int x = 0; x < 3; ++x) {
+        gA[x] += vA[x];
+        gB[x] += vB[x];
+        unsafeAtomicAdd(&gcopy[3 * c + x], -(vA[x] + vB[x]));
+      }
+    }
+  }
+#pragma unroll
+  for (int x = 0; x < 3; ++x) {
+    unsafeAtomicAdd(&gcopy[3 * bp.atA + x], gA[x]);
+    unsafeAtomicAdd(&gcopy[3 * bp.atB + x], gB[x]);
+  }
+}
+
+hipError_t launch_onee_grad(int la, int lb, const EngineView& ev, const int* pair_ids, int npairs,
+                            const double* atom_xyz, const double* atom_z, int natom, const double* D,
+                            const double* W, double* gacc, hipStream_t st) {
+  if (npairs <= 0) return hipSuccess;
+  dim3 g((npairs + 127) / 128), b(128);
+#define PQ_OGCASE(A, B_)                                                                                    \
+  if (la == A && lb == B_) {                                                                                \
+    hipLaunchKernelGGL((k_onee_grad<A, B_>), g, b, 0, st, ev, pair_ids, npairs, atom_xyz, atom_z, natom, D, W, \
+                       gacc);                                                                               \
+    return hipGetLastError();                                                                               \
+  }
+  PQ_OGCASE(0, 0)
+  PQ_OGCASE(1, 0)
+  PQ_OGCASE(1, 1)
+  PQ_OGCASE(2, 0)
+  PQ_OGCASE(2, 1)
+  PQ_OGCASE(2, 2)
+#undef PQ_OGCASE
+  return hipErrorInvalidValue;
+}
+
+// grad[i] = sum over the private copies + d/dR of sum_{A<B} Z_A Z_B / R_AB
+__global__ void k_grad_finish(const double* __restrict__ gacc, int ncopies, int natom,
+                              const double* __restrict__ xyz, const double* __restrict__ z, double* grad) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= 3 * natom) return;
+  double s = 0.0;
+  for (int c = 0; c < ncopies; ++c) s += gacc[(size_t)c * 3 * natom + i];
+  const int a = i / 3, x = i % 3;
+  for (int b = 0; b < natom; ++b) {
+    if (b == a) continue;
+    const double dx = xyz[3 * a] - xyz[3 * b], dy = xyz[3 * a + 1] - xyz[3 * b + 1], dz = xyz[3 * a + 2] - xyz[3 * b + 2];
+    const double r2 = dx * dx + dy * dy + dz * dz;
+    const double r = sqrt(r2);
+    const double dd = x == 0 ? dx : (x == 1 ? dy : dz);
+    s -= z[a] * z[b] * dd / (r2 * r);
+  }
+  grad[i] = s;
+}
+
+hipError_t launch_grad_finish(const double* gacc, int ncopies, int natom, const double* atom_xyz,
+                              const double* atom_z, double* grad, hipStream_t st) {
+  hipLaunchKernelGGL(k_grad_finish, dim3((3 * natom + 127) / 128), dim3(128), 0, st, gacc, ncopies, natom, atom_xyz,
+                     atom_z, grad);
+  return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------
 // Boys function F_m(x) = 1/2 x^(-m-1/2) gamma(m+1/2, x) by the incomplete
 // gamma series (x < m + 3/2) or Lentz continued fraction, as the
 // reference's Fgamma does (cints.c:340-411); eps is the relative stopping

@@ -163,6 +163,18 @@ class Engine(object):
                              _stream()))
         return J, K
 
+    def gradient(self, D, W):
+        """dE/dR (natom x 3, hartree/bohr) of the RHF energy at density D and
+        energy-weighted density W (pq_forces)."""
+        if self.cut is None:
+            self.build()
+        D = self._dtensor(D)
+        W = self._dtensor(W)
+        g = torch.empty((self.natom, 3), dtype=torch.float64, device=self.device)
+        check(lib.pq_forces(self.handle, ctypes.c_void_p(D.data_ptr()), ctypes.c_void_p(W.data_ptr()),
+                            ctypes.c_void_p(g.data_ptr()), _stream()))
+        return g
+
     def get2ints(self):
         """The reference's flat unique-ERI array as a device tensor (pq_get2ints)."""
         n = self.nbf

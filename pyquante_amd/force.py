@@ -1,0 +1,119 @@
+"""RHF nuclear gradients and forces on the GPU engine.
+
+Mirrors PyQuante/force.py:31-70 (hf_force / rhf_force: forces -dE/dR_A
+stored on each atom), with every term -- 2 tr(D dH), the two-electron
+derivative contracted with D on the fly, -2 tr(Q dS) and dV_nn -- computed
+by pq_forces.  D = C_occ C_occ^T and Q (W) = C_occ diag(e_occ) C_occ^T
+(Wavefunction.py:143-160).
+
+RHFEnergy is a torch.autograd.Function: forward runs the SCF at the given
+coordinates and returns E; backward returns grad_output * dE/dR from the
+gradient kernels, so E(R) composes with torch autograd.
+"""
+import numpy as np
+import torch
+
+from .basis import BasisSet
+from .engine import Engine
+from .hartree_fock import chol_orth, geigh, get_energy, mkdens
+
+
+class Wavefunction(object):
+    """Closed-shell wavefunction: orbitals (columns), energies, nclosed."""
+
+    restricted = True
+    unrestricted = False
+    fixedocc = False
+
+    def __init__(self, orbs, orbe, nclosed):
+        self.orbs = orbs
+        self.orbe = orbe
+        self.nclosed = nclosed
+
+    def mkdens(self):
+        c = self.orbs[:, :self.nclosed]
+        return c @ c.T
+
+    def mkQmatrix(self):
+        c = self.orbs[:, :self.nclosed]
+        return (c * self.orbe[:self.nclosed]) @ c.T
+
+
+def scf(engine, mol, ConvCriteria=1e-10, MaxIter=100, DoAveraging=True, precision=0):
+    """RHF on an engine (hartree_fock.rhf semantics, DIIS on by default here
+    for tight convergence).  Returns (E, orbe, orbs, D, W) as device tensors."""
+    from .hartree_fock import DIIS
+    S, T, V = engine.get1ints()
+    h = T + V
+    X = chol_orth(S)
+    orbe, C = geigh(h, S, X)
+    nocc = mol.get_closedopen()[0]
+    enuke = mol.get_enuke()
+    avg = DIIS(S) if DoAveraging else None
+    eold = 0.0
+    energy = 0.0
+    for _ in range(MaxIter):
+        D = mkdens(C, 0, nocc)
+        J, K = engine.fock_jk(D, precision=precision)
+        F = h + 2.0 * J - K
+        if avg is not None:
+            F = avg.getF(F, D)
+        orbe, C = geigh(F, S, X)
+        energy = get_energy(h, F, D, enuke)
+        if abs(energy - eold) < ConvCriteria:
+            break
+        eold = energy
+    D = mkdens(C, 0, nocc)
+    W = (C[:, :nocc] * orbe[:nocc]) @ C[:, :nocc].T
+    return energy, orbe, C, D, W
+
+
+def rhf_gradient(mol, basis_data="6-31g**", ConvCriteria=1e-10, schwarz_cut=1e-12, engine=None):
+    """(E, dE/dR) for a closed-shell molecule."""
+    eng = engine if engine is not None else Engine(BasisSet(mol, basis_data), mol)
+    eng.build(schwarz_cut, 0.0)
+    E, orbe, C, D, W = scf(eng, mol, ConvCriteria=ConvCriteria)
+    return E, eng.gradient(D, W)
+
+
+def rhf_force(mol, wf, bset, engine=None):
+    """force.py:48-70: sets atom.forces = -dE/dR_A for every atom."""
+    eng = engine if engine is not None else Engine(bset, mol)
+    if eng.cut is None:
+        eng.build()
+    D = torch.as_tensor(np.asarray(wf.mkdens(), dtype=np.float64), device=eng.device)
+    W = torch.as_tensor(np.asarray(wf.mkQmatrix(), dtype=np.float64), device=eng.device)
+    g = eng.gradient(D, W).cpu().numpy()
+    for atom, gi in zip(mol.atoms, g):
+        atom.forces = -gi
+    return -g
+
+
+def hf_force(mol, wf, bname):
+    """force.py:31-46 (restricted wavefunctions)."""
+    if not wf.restricted:
+        raise NotImplementedError("only restricted wavefunctions have GPU gradients here")
+    return rhf_force(mol, wf, BasisSet(mol, bname))
+
+
+class RHFEnergy(torch.autograd.Function):
+    """E(R): forward = converged RHF energy at coordinates R (natom x 3, bohr),
+    backward = dE/dR from the gradient kernels."""
+
+    @staticmethod
+    def forward(ctx, coords, mol, basis_data, conv):
+        R = coords.detach().cpu().numpy()
+        m = mol.copy_with_coords(R)
+        eng = Engine(BasisSet(m, basis_data), m)
+        eng.build(1e-12, 0.0)
+        E, orbe, C, D, W = scf(eng, m, ConvCriteria=conv)
+        ctx.grad = eng.gradient(D, W).to(coords.device, coords.dtype)
+        return torch.tensor(E, dtype=coords.dtype, device=coords.device)
+
+    @staticmethod
+    def backward(ctx, grad_out):
+        return grad_out * ctx.grad, None, None, None
+
+
+def rhf_energy(coords, mol, basis_data="6-31g**", conv=1e-10):
+    return RHFEnergy.apply(coords, mol, basis_data, conv)

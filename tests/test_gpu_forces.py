@@ -1,0 +1,71 @@
+"""GPU nuclear gradients against central finite differences of energies:
+the oracle's CPU SCF (independent code) for H2O/STO-3G and the GPU's own
+E(R) for 6-31G** water clusters; plus translational invariance."""
+import numpy as np
+import pytest
+import torch
+
+from oracle import oracle as O
+from oracle import scf as oscf
+
+pytestmark = pytest.mark.gpu
+
+
+def _h2o_bent():
+    from pyquante_amd.molecule import Molecule
+    return Molecule("h2o", [(8, (0.0, 0.1, -0.05)), (1, (1.43, 1.05, 0.07)), (1, (-1.39, 1.12, -0.11))])
+
+
+def test_gradient_vs_oracle_finite_difference_sto3g():
+    from pyquante_amd.basis import BasisSet
+    from pyquante_amd.force import rhf_gradient
+    mol = _h2o_bent()
+    E, g = rhf_gradient(mol, "sto-3g", ConvCriteria=1e-11, schwarz_cut=0.0)
+    g = g.cpu().numpy()
+    e_ref = oscf.rhf(BasisSet(mol, "sto-3g").bfs, mol, ConvCriteria=1e-11, MaxIter=200)[0]
+    assert abs(E - e_ref) < 1e-9
+    h = 1e-4
+    X = mol.coords()
+    for a in range(3):
+        for x in range(3):
+            Xp = X.copy(); Xp[a, x] += h
+            Xm = X.copy(); Xm[a, x] -= h
+            mp = mol.copy_with_coords(Xp)
+            mm = mol.copy_with_coords(Xm)
+            ep = oscf.rhf(BasisSet(mp, "sto-3g").bfs, mp, ConvCriteria=1e-12, MaxIter=200)[0]
+            em = oscf.rhf(BasisSet(mm, "sto-3g").bfs, mm, ConvCriteria=1e-12, MaxIter=200)[0]
+            fd = (ep - em) / (2 * h)
+            assert abs(fd - g[a, x]) < 1e-6, (a, x, fd, g[a, x])
+    assert np.max(np.abs(g.sum(0))) < 1e-8  # translational invariance
+
+
+def test_gradient_vs_own_finite_difference_water_dimer_631ss():
+    from pyquante_amd.force import rhf_gradient
+    from pyquante_amd.molecule import water_cluster
+    mol = water_cluster((2, 1, 1), seed=4)
+    E, g = rhf_gradient(mol, "6-31g**", ConvCriteria=1e-11)
+    g = g.cpu().numpy()
+    assert np.max(np.abs(g.sum(0))) < 1e-7
+    rng = np.random.default_rng(8)
+    X = mol.coords()
+    h = 1e-4
+    for _ in range(4):
+        a, x = int(rng.integers(len(mol))), int(rng.integers(3))
+        Xp = X.copy(); Xp[a, x] += h
+        Xm = X.copy(); Xm[a, x] -= h
+        ep = rhf_gradient(mol.copy_with_coords(Xp), "6-31g**", ConvCriteria=1e-11)[0]
+        em = rhf_gradient(mol.copy_with_coords(Xm), "6-31g**", ConvCriteria=1e-11)[0]
+        fd = (ep - em) / (2 * h)
+        assert abs(fd - g[a, x]) < 1e-6, (a, x, fd, g[a, x])
+
+
+def test_autograd_function_and_hf_force():
+    from pyquante_amd.basis import BasisSet
+    from pyquante_amd.force import Wavefunction, hf_force, rhf_energy, rhf_gradient
+    mol = _h2o_bent()
+    R = torch.tensor(mol.coords(), dtype=torch.float64, requires_grad=True)
+    E = rhf_energy(R, mol, "sto-3g")
+    E.backward()
+    E2, g = rhf_gradient(mol, "sto-3g")
+    assert abs(float(E) - E2) < 1e-9
+    assert torch.max(torch.abs(R.grad - g.cpu())).item() < 1e-9
