@@ -64,6 +64,61 @@ def cpu_baseline(bfs, seconds=10.0, seed=0):
                       "crys.c contr_coulomb compiled -O2, %.1f s, 1 thread" % (done, t_used)}
 
 
+def _timed(fn, steps):
+    import torch
+    fn()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        fn()
+    torch.cuda.synchronize()
+    return (time.perf_counter() - t0) / steps
+
+
+def extra_workloads(args):
+    """Mixed precision on (H2O)_16 and forces on (H2O)_8 (outside the
+    headline timed region; reported alongside it)."""
+    import torch
+
+    from pyquante_amd import _abi
+    from pyquante_amd.basis import BasisSet
+    from pyquante_amd.engine import Engine
+    from pyquante_amd.force import scf
+    from pyquante_amd.molecule import water_cluster
+
+    out = {}
+    # (H2O)_16: FP64 vs mixed (quartets with Schwarz bound < 1e-4 in FP32)
+    mol = water_cluster((2, 2, 4), seed=args.seed)
+    eng = Engine(BasisSet(mol, "6-31g**"), mol)
+    eng.build(1e-12, 0.0)
+    e64, _, _, D64, _ = scf(eng, mol, ConvCriteria=1e-9)
+    t64 = _timed(lambda: eng.fock_jk(D64), args.steps)
+    st64 = eng.stats()
+    eng.build(1e-12, 1e-4)
+    st32 = eng.stats()
+    emx, _, _, _, _ = scf(eng, mol, ConvCriteria=1e-9, precision=_abi.PQ_PREC_MIXED)
+    tmx = _timed(lambda: eng.fock_jk(D64, precision=_abi.PQ_PREC_MIXED), args.steps)
+    out.update({"h2o16_nbf": eng.nbf, "h2o16_fp64_ms_per_fock_build": t64 * 1e3,
+                "h2o16_mixed_ms_per_fock_build": tmx * 1e3, "h2o16_mixed_speedup": t64 / tmx,
+                "h2o16_fp64_quartets_per_s": st64["fquartets_fp64"] / t64,
+                "h2o16_fp32_quartet_fraction": st32["fquartets_fp32"] / max(1, st32["fquartets_fp32"] +
+                                                                               st32["fquartets_fp64"]),
+                "h2o16_energy_fp64_eh": e64, "h2o16_energy_mixed_eh": emx,
+                "h2o16_mixed_energy_error_eh": abs(emx - e64)})
+    print("bench: (H2O)_16 nbf=%d FP64 %.2f ms, mixed %.2f ms (x%.2f), |dE| = %.2e Eh"
+          % (eng.nbf, t64 * 1e3, tmx * 1e3, t64 / tmx, abs(emx - e64)), file=sys.stderr, flush=True)
+    del eng
+    # forces on (H2O)_8 from the converged density
+    mol = water_cluster((2, 2, 2), seed=args.seed)
+    eng = Engine(BasisSet(mol, "6-31g**"), mol)
+    eng.build(1e-12, 0.0)
+    _, _, _, D, W = scf(eng, mol, ConvCriteria=1e-9)
+    tg = _timed(lambda: eng.gradient(D, W), args.steps)
+    out.update({"h2o8_ms_per_gradient": tg * 1e3, "h2o8_forces_per_s": 1.0 / tg})
+    print("bench: (H2O)_8 gradient %.2f ms (%.1f forces/s)" % (tg * 1e3, 1.0 / tg), file=sys.stderr, flush=True)
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -72,6 +127,10 @@ def main():
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-extra", action="store_true", help="skip the (H2O)_16 and forces workloads")
+    ap.add_argument("--streams", type=int, default=4, help="engine streams for the class kernels (0-4)")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="also run one 256 MiB device copy (reference for rocprofv3 FETCH/WRITE_SIZE)")
     ap.add_argument("--classes", action="store_true", help="print per-class kernel times")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -93,6 +152,7 @@ def main():
     mol = water_cluster((2, 2, 2), seed=args.seed)
     bs = BasisSet(mol, "6-31g**")
     eng = Engine(bs, mol)
+    eng.set_streams(args.streams)
     eng.build(1e-12, 0.0)
     st = eng.stats()
     npair = eng.nbf * (eng.nbf + 1) // 2
@@ -144,6 +204,15 @@ def main():
     print("bench: %.2f ms/build; dominant kernel %s %.3f ms, %.2f TFLOP/s; all kernels %.2f TFLOP/s"
           % (dt * 1e3, lname, ktime * 1e3, achieved, total_flops / (acc[42] * 1e-3) / 1e12),
           file=sys.stderr, flush=True)
+    if args.copy_calibration:
+        src = torch.ones(32 * 1024 * 1024, dtype=torch.float64, device="cuda")
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
+    extra = {}
+    if not args.no_extra:
+        extra = extra_workloads(args)
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(bs.bfs, seconds=args.cpu_seconds)
@@ -164,7 +233,7 @@ def main():
         "config": {"workload": "(H2O)_8 6-31G** RHF Fock build (J+K), 200 Cartesian functions, "
                                "Schwarz 1e-12, FP64", "nbf": eng.nbf, "shell_quartets": st["quartets_fp64"],
                    "function_quartets": fq, "unscreened_function_quartets": unscreened,
-                   "s_per_fock_build": dt},
+                   "s_per_fock_build": dt, **extra},
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved / FP64_PEAK_TFLOPS, "traffic": None,
                      "kernel": "k_eri_%s %s FP64 J/K" % ("lane" if c["class"] != (2, 2, 2, 2) else "wave", lname),

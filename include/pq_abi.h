@@ -93,6 +93,12 @@ int pq_engine_set_geometry(pq_engine *eng, const double *atom_xyz_host);
  * builds, quartets with bound below fp32_cut run in FP32. */
 int pq_build_pairs(pq_engine *eng, double schwarz_cut, double fp32_cut);
 
+/* Number of engine-owned streams (0..4, default 4) the class kernels of one
+ * Fock build are spread over; 0 launches everything on the caller's stream
+ * (one kernel at a time, e.g. for per-kernel profiling).  No reference
+ * counterpart (a launch knob). */
+int pq_engine_set_streams(pq_engine *eng, int nstreams);
+
 /* Statistics of the current work lists: counts[0..5] = FP64 shell quartets,
  * FP32 shell quartets, FP64 primitive quartets, FP32 primitive quartets,
  * FP64 and FP32 unique function quartets (i>=j, k>=l, ij>=kl: the unit the

@@ -122,6 +122,7 @@ struct pq_engine {
   hipStream_t side[kSide] = {};
   hipEvent_t fork_ev = nullptr, join_ev[kSide] = {};
   bool streams_ok = false;
+  int nside = kSide;  // streams in use (0 = caller's stream only)
 
   EngineView view() const { return EngineView{d_pairs, d_prims, d_prim_a, nbf}; }
 
@@ -547,6 +548,13 @@ int pq_build_pairs(pq_engine* eng, double schwarz_cut, double fp32_cut) {
   return PQ_OK;
 }
 
+int pq_engine_set_streams(pq_engine* eng, int nstreams) {
+  if (!eng) return fail(PQ_E_ARG, "null engine");
+  if (nstreams < 0 || nstreams > pq_engine::kSide) return fail(PQ_E_ARG, "nstreams must be 0..4");
+  eng->nside = nstreams;
+  return PQ_OK;
+}
+
 int pq_engine_stats(const pq_engine* eng, uint64_t counts[6], double flops[2]) {
   if (!eng || !counts || !flops) return fail(PQ_E_ARG, "null argument");
   if (!eng->work_ok) return fail(PQ_E_STATE, "pq_build_pairs not called");
@@ -656,7 +664,20 @@ static int fock_jk_impl(pq_engine* eng, const double* D_dev, double* J_dev, doub
       rc = fail(PQ_E_HIP, hipGetErrorString(err));
       break;
     }
-    if (!ms) {
+    if (!ms && eng->nside == 0) {
+      for (size_t ci = 0; ci < eng->work.size() && err == hipSuccess; ++ci) {
+        const auto& c = eng->work[ci];
+        const int la = kPairL[c.X][0], lb = kPairL[c.X][1], lc = kPairL[c.Y][0], ld = kPairL[c.Y][1];
+        for (int prec = 0; prec < 2 && err == hipSuccess; ++prec) {
+          const auto& tl = prec ? c.t32 : c.t64;
+          if (tl.empty()) continue;
+          err = launch_eri_class(la, lb, lc, ld, MODE_JK, prec, eng->view(), prec ? c.d_t32 : c.d_t64,
+                                 (int)tl.size(), eng->d_sorted[c.Y], D_dev, eng->d_Jacc, eng->d_Kacc, nullptr,
+                                 nullptr, st);
+        }
+      }
+      if (err != hipSuccess) { rc = fail(PQ_E_HIP, hipGetErrorString(err)); break; }
+    } else if (!ms) {
       // fork: class kernels spread over the side streams, joined before the
       // symmetrisation (all of them only add into Jacc/Kacc)
       if (!eng->streams_ok) {
@@ -669,7 +690,7 @@ static int fock_jk_impl(pq_engine* eng, const double* D_dev, double* J_dev, doub
         eng->streams_ok = true;
       }
       if ((err = hipEventRecord(eng->fork_ev, st)) != hipSuccess) { rc = fail(PQ_E_HIP, hipGetErrorString(err)); break; }
-      for (int i = 0; i < pq_engine::kSide && err == hipSuccess; ++i) err = hipStreamWaitEvent(eng->side[i], eng->fork_ev, 0);
+      for (int i = 0; i < eng->nside && err == hipSuccess; ++i) err = hipStreamWaitEvent(eng->side[i], eng->fork_ev, 0);
       if (err != hipSuccess) { rc = fail(PQ_E_HIP, hipGetErrorString(err)); break; }
       int next = 0;
       for (size_t ci = 0; ci < eng->work.size() && err == hipSuccess; ++ci) {
@@ -681,10 +702,10 @@ static int fock_jk_impl(pq_engine* eng, const double* D_dev, double* J_dev, doub
           err = launch_eri_class(la, lb, lc, ld, MODE_JK, prec, eng->view(), prec ? c.d_t32 : c.d_t64,
                                  (int)tl.size(), eng->d_sorted[c.Y], D_dev, eng->d_Jacc, eng->d_Kacc, nullptr,
                                  nullptr, eng->side[next]);
-          next = (next + 1) % pq_engine::kSide;
+          next = (next + 1) % eng->nside;
         }
       }
-      for (int i = 0; i < pq_engine::kSide && err == hipSuccess; ++i) {
+      for (int i = 0; i < eng->nside && err == hipSuccess; ++i) {
         err = hipEventRecord(eng->join_ev[i], eng->side[i]);
         if (err == hipSuccess) err = hipStreamWaitEvent(st, eng->join_ev[i], 0);
       }

@@ -136,16 +136,18 @@ __device__ __forceinline__ void eri_block(const EngineView& ev, const PairInfo& 
 //   Kacc[b,c] += f v D[a,d],    Kacc[b,d] += f v D[a,c],
 // and J = Jacc + Jacc^T, K = Kacc + Kacc^T (symmetric D).
 // ---------------------------------------------------------------------
-// krow, when non-null, is this wavefront's LDS copy of the bra rows of K
-// (rows a in [A0, A0+NAS) then b in [B0, B0+NBS), nbf columns each): the four
-// K blocks then accumulate there with LDS atomics and leave the wavefront as
-// coalesced global atomics (flush_krows), one cache line per 16 columns
-// instead of one per lane.
+// krow, when non-null, is the block's LDS copy of the bra rows of K (rows
+// 0..NA-1 for the functions of shell A, NA..NA+NB-1 for shell B, nbf columns
+// each): the four K blocks accumulate there with LDS atomics and leave the
+// block as coalesced global atomics (flush_block), one cache line per 16
+// columns instead of one per lane.  jsum, when non-null, is the block's LDS
+// accumulator of the ket-side J block per lane (shared by the component
+// splits of one tile), flushed once per block.
 template <int LA, int LB, int LC, int LD, int A0, int NAS, int B0, int NBS, typename T>
 __device__ __forceinline__ void digest_lane(const EngineView& ev, const PairInfo& bp, const PairInfo& kp,
                                             const T (&out)[NAS * NBS * ncart(LC) * ncart(LD)], double f,
                                             bool active, const double* __restrict__ D, double* Jacc,
-                                            double* Kacc, double* krow) {
+                                            double* Kacc, double* krow, double* jsum) {
   using C = Cls<LA, LB, LC, LD>;
   const int n = ev.nbf;
   const int a0 = bp.bfA + A0, b0 = bp.bfB + B0, c0 = kp.bfA, d0 = kp.bfB;
@@ -199,24 +201,31 @@ __device__ __forceinline__ void digest_lane(const EngineView& ev, const PairInfo
       if (lane == 0) unsafeAtomicAdd(&Jacc[(a0 + a) * n + b0 + b], 2.0 * s);
     }
   if (!active) return;
+  if (jsum) {
 #pragma unroll
-  for (int c = 0; c < C::NC; ++c)
+    for (int c = 0; c < C::NC; ++c)
 #pragma unroll
-    for (int d = 0; d < C::ND; ++d) unsafeAtomicAdd(&Jacc[(c0 + c) * n + d0 + d], 2.0 * jcd[c][d]);
+      for (int d = 0; d < C::ND; ++d) atomicAdd(&jsum[lane * C::NC * C::ND + c * C::ND + d], 2.0 * jcd[c][d]);
+  } else {
+#pragma unroll
+    for (int c = 0; c < C::NC; ++c)
+#pragma unroll
+      for (int d = 0; d < C::ND; ++d) unsafeAtomicAdd(&Jacc[(c0 + c) * n + d0 + d], 2.0 * jcd[c][d]);
+  }
   if (krow) {
 #pragma unroll
     for (int a = 0; a < NAS; ++a) {
 #pragma unroll
-      for (int c = 0; c < C::NC; ++c) atomicAdd(&krow[a * n + c0 + c], kac[a][c]);
+      for (int c = 0; c < C::NC; ++c) atomicAdd(&krow[(A0 + a) * n + c0 + c], kac[a][c]);
 #pragma unroll
-      for (int d = 0; d < C::ND; ++d) atomicAdd(&krow[a * n + d0 + d], kad[a][d]);
+      for (int d = 0; d < C::ND; ++d) atomicAdd(&krow[(A0 + a) * n + d0 + d], kad[a][d]);
     }
 #pragma unroll
     for (int b = 0; b < NBS; ++b) {
 #pragma unroll
-      for (int c = 0; c < C::NC; ++c) atomicAdd(&krow[(NAS + b) * n + c0 + c], kbc[b][c]);
+      for (int c = 0; c < C::NC; ++c) atomicAdd(&krow[(C::NA + B0 + b) * n + c0 + c], kbc[b][c]);
 #pragma unroll
-      for (int d = 0; d < C::ND; ++d) atomicAdd(&krow[(NAS + b) * n + d0 + d], kbd[b][d]);
+      for (int d = 0; d < C::ND; ++d) atomicAdd(&krow[(C::NA + B0 + b) * n + d0 + d], kbd[b][d]);
     }
     return;
   }
@@ -265,27 +274,10 @@ __device__ __forceinline__ void atomic_max_nonneg(double* p, double v) {
 }
 
 // One lane, one shell quartet, bra components [A0, A0+NAS) x [B0, B0+NBS).
-// Flush this wavefront's LDS K rows with coalesced global atomics (zeros skipped).
-template <int NAS, int NBS>
-__device__ __forceinline__ void flush_krows(double* krow, int n, int a0, int b0, double* Kacc) {
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-  __builtin_amdgcn_wave_barrier();
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-  const int lane = threadIdx.x & 63;
-#pragma unroll
-  for (int r = 0; r < NAS + NBS; ++r) {
-    const int grow = r < NAS ? a0 + r : b0 + (r - NAS);
-    for (int col = lane; col < n; col += 64) {
-      const double v = krow[r * n + col];
-      if (v != 0.0) unsafeAtomicAdd(&Kacc[grow * n + col], v);
-    }
-  }
-}
-
 template <int LA, int LB, int LC, int LD, typename T, int MODE, int A0, int NAS, int B0, int NBS>
 __device__ __forceinline__ void lane_body(const EngineView& ev, const Tile& tl, bool active, int bra, int ket,
                                           const double* __restrict__ D, double* Jacc, double* Kacc,
-                                          double* ints, double* qout, double* krow) {
+                                          double* ints, double* qout, double* krow, double* jsum) {
   using C = Cls<LA, LB, LC, LD>;
   const PairInfo bp = ev.pairs[bra];
   const PairInfo kp = ev.pairs[ket];
@@ -300,15 +292,7 @@ __device__ __forceinline__ void lane_body(const EngineView& ev, const Tile& tl, 
   }
   if constexpr (MODE == MODE_JK) {
     const double f = active ? quartet_degeneracy(bp, kp, bra, ket) : 0.0;
-    if (krow) {
-      const int lane = threadIdx.x & 63;
-      for (int i = lane; i < (NAS + NBS) * ev.nbf; i += 64) krow[i] = 0.0;
-      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-      __builtin_amdgcn_wave_barrier();
-      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-    }
-    digest_lane<LA, LB, LC, LD, A0, NAS, B0, NBS, T>(ev, bp, kp, out, f, active, D, Jacc, Kacc, krow);
-    if (krow) flush_krows<NAS, NBS>(krow, ev.nbf, bp.bfA + A0, bp.bfB + B0, Kacc);
+    digest_lane<LA, LB, LC, LD, A0, NAS, B0, NBS, T>(ev, bp, kp, out, f, active, D, Jacc, Kacc, krow, jsum);
   } else if constexpr (MODE == MODE_STORE) {
     if (!active) return;
     const bool same = bra == ket;
@@ -350,18 +334,36 @@ __device__ __forceinline__ void static_switch(int i, F&& f) {
   }
 }
 
+// Block layouts:
+//   lds_mode 0: 4 wavefronts, 4 independent tiles, global atomics only;
+//   lds_mode 1: one tile per block; unsplit classes run 1 wavefront, split
+//     classes kWaveSplit wavefronts that take consecutive component splits of
+//     the same tile (blockIdx.y picks the chunk), sharing the LDS K rows and
+//     the ket-side J accumulator.
+constexpr int kWaveSplit = 4;
+
 template <int LA, int LB, int LC, int LD, typename T, int MODE>
 __global__ __launch_bounds__(256) void k_eri_lane(EngineView ev, const Tile* __restrict__ tiles, int ntiles,
                                                   const int* __restrict__ kets, const double* __restrict__ D,
                                                   double* Jacc, double* Kacc, double* ints,
-                                                  double* qout, int lds_rows) {
+                                                  double* qout, int lds_mode) {
   using C = Cls<LA, LB, LC, LD>;
   extern __shared__ double pq_lds[];
   const int lane = threadIdx.x & 63;
   const int wv = threadIdx.x >> 6;
-  double* krow = lds_rows > 0 ? pq_lds + wv * lds_rows * ev.nbf : nullptr;
-  const int t = __builtin_amdgcn_readfirstlane(blockIdx.x * (blockDim.x >> 6) + wv);
-  if (t >= ntiles) return;  // uniform per wavefront
+  const int n = ev.nbf;
+  const int t = __builtin_amdgcn_readfirstlane(lds_mode ? (int)blockIdx.x : (int)(blockIdx.x * 4 + wv));
+  double* krow = nullptr;
+  double* jsum = nullptr;
+  if (lds_mode) {
+    krow = pq_lds;
+    if (C::SPLIT) jsum = pq_lds + (C::NA + C::NB) * n;
+    const int tot = (C::NA + C::NB) * n + (C::SPLIT ? 64 * C::NC * C::ND : 0);
+    for (int i = threadIdx.x; i < tot; i += blockDim.x) pq_lds[i] = 0.0;
+    __syncthreads();
+  } else if (t >= ntiles) {
+    return;  // uniform per wavefront, no block barrier in this layout
+  }
   Tile tl = tiles[t];
   tl.bra = __builtin_amdgcn_readfirstlane(tl.bra);
   tl.ket0 = __builtin_amdgcn_readfirstlane(tl.ket0);
@@ -372,18 +374,45 @@ __global__ __launch_bounds__(256) void k_eri_lane(EngineView ev, const Tile* __r
   const int bra = (MODE == MODE_DIAG) ? ket : tl.bra;
   if constexpr (C::SPLIT == 0) {
     lane_body<LA, LB, LC, LD, T, MODE, 0, C::NA, 0, C::NB>(ev, tl, active, bra, ket, D, Jacc, Kacc, ints, qout,
-                                                          krow);
-  } else if constexpr (C::SPLIT == 1) {
-    static_switch<0, C::NA>(blockIdx.y, [&](auto ia) {
-      lane_body<LA, LB, LC, LD, T, MODE, decltype(ia)::value, 1, 0, C::NB>(ev, tl, active, bra, ket, D, Jacc,
-                                                                            Kacc, ints, qout, krow);
-    });
+                                                          krow, jsum);
   } else {
-    static_switch<0, C::NA * C::NB>(blockIdx.y, [&](auto iab) {
-      constexpr int v = decltype(iab)::value;
-      lane_body<LA, LB, LC, LD, T, MODE, v / C::NB, 1, v % C::NB, 1>(ev, tl, active, bra, ket, D, Jacc, Kacc,
-                                                                      ints, qout, krow);
-    });
+    const int split = lds_mode ? (int)blockIdx.y * kWaveSplit + wv : (int)blockIdx.y;
+    if constexpr (C::SPLIT == 1) {
+      static_switch<0, C::NA>(split, [&](auto ia) {
+        lane_body<LA, LB, LC, LD, T, MODE, decltype(ia)::value, 1, 0, C::NB>(ev, tl, active, bra, ket, D, Jacc,
+                                                                              Kacc, ints, qout, krow, jsum);
+      });
+    } else {
+      static_switch<0, C::NA * C::NB>(split, [&](auto iab) {
+        constexpr int v = decltype(iab)::value;
+        lane_body<LA, LB, LC, LD, T, MODE, v / C::NB, 1, v % C::NB, 1>(ev, tl, active, bra, ket, D, Jacc, Kacc,
+                                                                        ints, qout, krow, jsum);
+      });
+    }
+  }
+  if (!lds_mode) return;
+  __syncthreads();
+  // flush the K rows (coalesced) and the ket-side J accumulator
+  const PairInfo bp = ev.pairs[tl.bra];
+  for (int i = threadIdx.x; i < (C::NA + C::NB) * n; i += blockDim.x) {
+    const double v = krow[i];
+    if (v != 0.0) {
+      const int r = i / n, col = i - r * n;
+      const int grow = r < C::NA ? bp.bfA + r : bp.bfB + (r - C::NA);
+      unsafeAtomicAdd(&Kacc[grow * n + col], v);
+    }
+  }
+  if constexpr (C::SPLIT != 0) {
+    constexpr int NCD = C::NC * C::ND;
+    for (int i = threadIdx.x; i < 64 * NCD; i += blockDim.x) {
+      const int l = i / NCD, cd = i - l * NCD;
+      if (l >= tl.nket) break;
+      const double v = jsum[i];
+      if (v != 0.0) {
+        const PairInfo kq = ev.pairs[kets[tl.ket0 + l]];
+        unsafeAtomicAdd(&Jacc[(kq.bfA + cd / C::ND) * n + kq.bfB + cd % C::ND], v);
+      }
+    }
   }
 }
 
@@ -396,17 +425,16 @@ static hipError_t launch_one(const EngineView& ev, const Tile* tiles, int ntiles
                              hipStream_t st) {
   if (ntiles <= 0) return hipSuccess;
   using C = Cls<LA, LB, LC, LD>;
-  // K bra rows staged in LDS (one wavefront per block) when they fit in 64 KB
-  constexpr int rows = C::SPLIT == 0 ? C::NA + C::NB : (C::SPLIT == 1 ? 1 + C::NB : 2);
-  const size_t lds = (size_t)rows * ev.nbf * sizeof(double);
+  const size_t lds = ((size_t)(C::NA + C::NB) * ev.nbf + (C::SPLIT ? 64 * C::NC * C::ND : 0)) * sizeof(double);
   if (MODE == MODE_JK && lds <= 65536) {
-    hipLaunchKernelGGL((k_eri_lane<LA, LB, LC, LD, T, MODE>), dim3(ntiles, C::NSPLIT), dim3(64), lds, st, ev,
-                       tiles, ntiles, kets, D, Jacc, Kacc, ints, qout, rows);
+    const int waves = C::SPLIT ? kWaveSplit : 1;
+    const int chunks = (C::NSPLIT + waves - 1) / waves;
+    hipLaunchKernelGGL((k_eri_lane<LA, LB, LC, LD, T, MODE>), dim3(ntiles, chunks), dim3(64 * waves), lds, st, ev,
+                       tiles, ntiles, kets, D, Jacc, Kacc, ints, qout, 1);
     return hipGetLastError();
   }
-  const int wpb = 4;
-  hipLaunchKernelGGL((k_eri_lane<LA, LB, LC, LD, T, MODE>), dim3((ntiles + wpb - 1) / wpb, C::NSPLIT),
-                     dim3(64 * wpb), 0, st, ev, tiles, ntiles, kets, D, Jacc, Kacc, ints, qout, 0);
+  hipLaunchKernelGGL((k_eri_lane<LA, LB, LC, LD, T, MODE>), dim3((ntiles + 3) / 4, C::NSPLIT), dim3(256), 0, st,
+                     ev, tiles, ntiles, kets, D, Jacc, Kacc, ints, qout, 0);
   return hipGetLastError();
 }
 

@@ -109,6 +109,10 @@ class Engine(object):
         self.fp32cut = fp32_cut
         return self
 
+    def set_streams(self, n):
+        """Spread class kernels over n engine streams (0 = caller's stream only)."""
+        check(lib.pq_engine_set_streams(self.handle, int(n)))
+
     def stats(self):
         counts = (ctypes.c_uint64 * 6)()
         flops = (ctypes.c_double * 2)()
