@@ -3,7 +3,9 @@
 cluster (200 Cartesian functions, Schwarz cutoff 1e-12, FP64).
 
 A step is one direct Fock build (J and K from one density, pq_fock_jk) over
-the full screened quartet list.  Prints one JSON line.  The roofline block is
+the full screened quartet list, replayed from a HIP graph capture of that call
+(--eager times plain calls); the per-class bra-run lengths are autotuned on
+the same density first (--no-tune keeps the defaults).  Prints one JSON line.  The roofline block is
 for the dominant class kernel: algorithmic Rys flops of that kernel (the
 F_prim/F_dig operation count) over its HIP-event time on the launch stream,
 against the MI355X FP64 peak.  The cpu_baseline block times the compiled
@@ -64,6 +66,16 @@ def cpu_baseline(bfs, seconds=10.0, seed=0):
                       "crys.c contr_coulomb compiled -O2, %.1f s, 1 thread" % (done, t_used)}
 
 
+def kernel_name(cls, cap, nbf):
+    """Kernel template a class's FP64 J/K tiles run in: bra runs (k_eri_group)
+    for classes of at most 81 components when the run cap is nonzero."""
+    ncart = [1, 3, 6]
+    ncomp = ncart[cls[0]] * ncart[cls[1]] * ncart[cls[2]] * ncart[cls[3]]
+    if cap is None:
+        cap = 16 if nbf >= 300 else 0
+    return "k_eri_group" if ncomp <= 81 and cap > 0 else "k_eri_lane"
+
+
 def _timed(fn, steps):
     import torch
     fn()
@@ -92,12 +104,16 @@ def extra_workloads(args):
     eng = Engine(BasisSet(mol, "6-31g**"), mol)
     eng.build(1e-12, 0.0)
     e64, _, _, D64, _ = scf(eng, mol, ConvCriteria=1e-9)
-    t64 = _timed(lambda: eng.fock_jk(D64), args.steps)
+    if not args.no_tune:
+        eng.autotune(D64)
+    t64 = _timed(eng.capture_fock_jk(D64), args.steps)
     st64 = eng.stats()
     eng.build(1e-12, 1e-4)
     st32 = eng.stats()
     emx, _, _, _, _ = scf(eng, mol, ConvCriteria=1e-9, precision=_abi.PQ_PREC_MIXED)
-    tmx = _timed(lambda: eng.fock_jk(D64, precision=_abi.PQ_PREC_MIXED), args.steps)
+    if not args.no_tune:
+        eng.autotune(D64)
+    tmx = _timed(eng.capture_fock_jk(D64, precision=_abi.PQ_PREC_MIXED), args.steps)
     out.update({"h2o16_nbf": eng.nbf, "h2o16_fp64_ms_per_fock_build": t64 * 1e3,
                 "h2o16_mixed_ms_per_fock_build": tmx * 1e3, "h2o16_mixed_speedup": t64 / tmx,
                 "h2o16_fp64_quartets_per_s": st64["fquartets_fp64"] / t64,
@@ -122,7 +138,7 @@ def extra_workloads(args):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
@@ -132,6 +148,8 @@ def main():
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run one 256 MiB device copy (reference for rocprofv3 FETCH/WRITE_SIZE)")
     ap.add_argument("--classes", action="store_true", help="print per-class kernel times")
+    ap.add_argument("--no-tune", action="store_true", help="skip the per-class autotuner (default bra runs)")
+    ap.add_argument("--eager", action="store_true", help="time eager pq_fock_jk calls instead of graph replays")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py runs on one GPU only: this build has no multi-GPU path "
@@ -170,14 +188,28 @@ def main():
     D = mkdens(C, 0, nocc).contiguous()
     print("bench: (H2O)_8 6-31G** nbf=%d, %d shell quartets, %d function quartets (unscreened %d)"
           % (eng.nbf, st["quartets_fp64"], st["fquartets_fp64"], unscreened), file=sys.stderr, flush=True)
-    for _ in range(args.warmup):
-        eng.fock_jk(D)
+    caps = None
+    if not args.no_tune:
+        caps = eng.autotune(D)
+        print("bench: autotuned bra-run caps " + " ".join(
+            "(%s%s|%s%s)=%d" % (*("spd"[x] for x in cc["class"]), caps[ci][0])
+            for ci, cc in enumerate(eng.class_stats()) if cc["quartets_fp64"]), file=sys.stderr, flush=True)
+    J0, K0 = eng.fock_jk(D)
+    step = (lambda: eng.fock_jk(D)) if args.eager else eng.capture_fock_jk(D)
+    for _ in range(max(1, args.warmup)):
+        Jg, Kg = step()
     torch.cuda.synchronize()
+    graph_err = max(float((Jg - J0).abs().max()), float((Kg - K0).abs().max()))
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        step()
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / args.steps
     t0 = time.perf_counter()
     for _ in range(args.steps):
         eng.fock_jk(D)
     torch.cuda.synchronize()
-    dt = (time.perf_counter() - t0) / args.steps
+    dt_eager = (time.perf_counter() - t0) / args.steps
     # per-class kernel times (HIP events on the launch stream)
     cls = eng.class_stats()
     nt = 3
@@ -191,9 +223,10 @@ def main():
             if cc["quartets_fp64"] == 0:
                 continue
             t = acc[2 * ci] * 1e-3
-            print("  %s quartets %9d prim %11d  %.3f ms  %7.3f TFLOP/s  %6.1f Gquartet/s"
+            print("  %s quartets %9d prim %11d tiles %7d  %.3f ms  %7.3f TFLOP/s  %6.1f Gquartet/s"
                   % ("(%s%s|%s%s)" % tuple("spd"[x] for x in cc["class"]), cc["quartets_fp64"],
-                     cc["prim_quartets_fp64"], t * 1e3, cc["flops_fp64"] / t / 1e12 if t > 0 else 0,
+                     cc["prim_quartets_fp64"], cc["tiles_fp64"], t * 1e3,
+                     cc["flops_fp64"] / t / 1e12 if t > 0 else 0,
                      cc["fquartets_fp64"] / t / 1e9 if t > 0 else 0), file=sys.stderr)
     k = int(np.argmax(acc[:42:2]))
     c = cls[k]
@@ -201,9 +234,10 @@ def main():
     achieved = c["flops_fp64"] / ktime / 1e12
     lname = "(%s%s|%s%s)" % tuple("spd"[x] for x in c["class"])
     total_flops = st["flops_fp64"]
-    print("bench: %.2f ms/build; dominant kernel %s %.3f ms, %.2f TFLOP/s; all kernels %.2f TFLOP/s"
-          % (dt * 1e3, lname, ktime * 1e3, achieved, total_flops / (acc[42] * 1e-3) / 1e12),
-          file=sys.stderr, flush=True)
+    print("bench: %.3f ms/build (%s; eager %.3f ms, replay vs eager max|d| %.1e); dominant kernel %s %.3f ms, "
+          "%.2f TFLOP/s; all kernels %.2f TFLOP/s"
+          % (dt * 1e3, "eager" if args.eager else "graph replay", dt_eager * 1e3, graph_err, lname, ktime * 1e3,
+             achieved, total_flops / (acc[42] * 1e-3) / 1e12), file=sys.stderr, flush=True)
     if args.copy_calibration:
         src = torch.ones(32 * 1024 * 1024, dtype=torch.float64, device="cuda")
         dst = torch.empty_like(src)
@@ -233,10 +267,12 @@ def main():
         "config": {"workload": "(H2O)_8 6-31G** RHF Fock build (J+K), 200 Cartesian functions, "
                                "Schwarz 1e-12, FP64", "nbf": eng.nbf, "shell_quartets": st["quartets_fp64"],
                    "function_quartets": fq, "unscreened_function_quartets": unscreened,
-                   "s_per_fock_build": dt, **extra},
+                   "s_per_fock_build": dt, "timing": "eager" if args.eager else "HIP graph replay",
+                   "s_per_fock_build_eager": dt_eager, "autotuned": caps is not None, **extra},
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved / FP64_PEAK_TFLOPS, "traffic": None,
-                     "kernel": "k_eri_%s %s FP64 J/K" % ("lane" if c["class"] != (2, 2, 2, 2) else "wave", lname),
+                     "kernel": "%s %s FP64 J/K" % (kernel_name(c["class"], caps[k][0] if caps else None, eng.nbf),
+                                                    lname),
                      "all_kernels_tflops": total_flops / (acc[42] * 1e-3) / 1e12},
         "cpu_baseline": cpu,
     }

@@ -99,6 +99,23 @@ int pq_build_pairs(pq_engine *eng, double schwarz_cut, double fp32_cut);
  * counterpart (a launch knob). */
 int pq_engine_set_streams(pq_engine *eng, int nstreams);
 
+/* Bra runs: for the J/K kernels of classes with at most 81 Cartesian
+ * components, one 256-thread block walks up to `cap` consecutive tiles that
+ * share a bra shell pair and flushes its LDS copy of the bra K rows once.
+ * cap = 0 disables runs, -1 restores the default (16 tiles when nbf >= 300,
+ * else none).  Applies to every class and overrides tuned values.  No
+ * reference counterpart (a launch knob). */
+int pq_engine_set_group_cap(pq_engine *eng, int cap);
+
+/* Autotuner: for each class and precision of the current work lists, times
+ * the J/K kernel on D_dev (`reps` timed launches after one warm-up) for run
+ * caps {0, 4, 8, 16, 32} on `stream` and keeps the fastest.  The choice
+ * persists across pq_build_pairs calls of this engine.  caps_out (optional,
+ * 42 ints) receives the chosen cap per (class, precision); classes with more
+ * than 81 components report their unused cap.  Overwrites the engine's J/K
+ * scratch only.  No reference counterpart (a launch knob). */
+int pq_engine_autotune(pq_engine *eng, const double *D_dev, int reps, void *stream, int *caps_out);
+
 /* Statistics of the current work lists: counts[0..5] = FP64 shell quartets,
  * FP32 shell quartets, FP64 primitive quartets, FP32 primitive quartets,
  * FP64 and FP32 unique function quartets (i>=j, k>=l, ij>=kl: the unit the
@@ -108,9 +125,10 @@ int pq_engine_stats(const pq_engine *eng, uint64_t counts[6], double flops[2]);
 
 /* Per-class statistics for class index c in 0..20 (pair classes X >= Y in
  * the order ss, ps, pp, ds, dp, dd; c enumerates (X,Y) row by row):
- * l[0..3] = (la, lb, lc, ld), counts/flops as in pq_engine_stats.
+ * l[0..3] = (la, lb, lc, ld), counts[0..5] as in pq_engine_stats, counts[6]
+ * and counts[7] the FP64 and FP32 tile counts; flops as in pq_engine_stats.
  * No reference counterpart (diagnostics for the roofline report). */
-int pq_engine_class_stats(const pq_engine *eng, int c, int l[4], uint64_t counts[6], double flops[2]);
+int pq_engine_class_stats(const pq_engine *eng, int c, int l[4], uint64_t counts[8], double flops[2]);
 
 /* ---------------- integrals and Fock builds ---------------- */
 

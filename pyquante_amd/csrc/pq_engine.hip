@@ -77,6 +77,10 @@ struct ClassWork {
   std::vector<Tile> t64, t32;
   Tile* d_t64 = nullptr;
   Tile* d_t32 = nullptr;
+  std::vector<int2> g64, g32;  // runs of tiles with one bra pair
+  int2* d_g64 = nullptr;
+  int2* d_g32 = nullptr;
+  int cap[2] = {0, 0};         // tiles per run (0 = no runs) for FP64 / FP32 tiles
   uint64_t nq64 = 0, nq32 = 0;
   double np64 = 0, np32 = 0;
   double nf64 = 0, nf32 = 0;  // unique function quartets
@@ -124,12 +128,22 @@ struct pq_engine {
   bool streams_ok = false;
   int nside = kSide;  // streams in use (0 = caller's stream only)
 
+  // tiles per bra run for each (bra class, ket class, precision): -1 until
+  // set by pq_engine_autotune or pq_engine_set_group_cap
+  int group_cap[6][6][2];
+  pq_engine() {
+    for (auto& a : group_cap)
+      for (auto& b : a) b[0] = b[1] = -1;
+  }
+
   EngineView view() const { return EngineView{d_pairs, d_prims, d_prim_a, nbf}; }
 
   void free_work(std::vector<ClassWork>& w) {
     for (auto& c : w) {
       dev_free(c.d_t64);
       dev_free(c.d_t32);
+      dev_free(c.d_g64);
+      dev_free(c.d_g32);
     }
     w.clear();
   }
@@ -274,10 +288,35 @@ int build_pair_data(pq_engine* e) {
 // spread over several tiles (their partial sums digest independently).
 constexpr int kPrimBudget = 36;
 
+// Default tiles per bra run when no tuned value is set: a run saves
+// (tiles - 1) flushes of the bra K rows, which pays off when the rows are long.
+int default_group_cap(int nbf) { return nbf >= 300 ? 16 : 0; }
+
+// Rebuilds the bra runs of one precision of a class for a new cap (0 = none)
+// and uploads them.
+int set_groups(ClassWork& cw, int prec, int cap) {
+  const std::vector<Tile>& t = prec ? cw.t32 : cw.t64;
+  std::vector<int2>& g = prec ? cw.g32 : cw.g64;
+  int2*& d = prec ? cw.d_g32 : cw.d_g64;
+  g.clear();
+  dev_free(d);
+  cw.cap[prec] = cap;
+  if (cap <= 0) return PQ_OK;
+  for (int i = 0; i < (int)t.size(); ++i) {
+    if (g.empty() || t[i].bra != t[g.back().x].bra || g.back().y >= cap) g.push_back(make_int2(i, 0));
+    g.back().y += 1;
+  }
+  PQ_HIP(dev_alloc(&d, g.size()));
+  if (!g.empty()) PQ_HIP(hipMemcpy(d, g.data(), g.size() * sizeof(int2), hipMemcpyHostToDevice));
+  return PQ_OK;
+}
+
 int build_work(pq_engine* e, std::vector<ClassWork>& out, double cut, double fp32cut, bool split_prims) {
   for (auto& c : out) {
     dev_free(c.d_t64);
     dev_free(c.d_t32);
+    dev_free(c.d_g64);
+    dev_free(c.d_g32);
   }
   out.clear();
   for (int X = 0; X < 6; ++X)
@@ -353,6 +392,11 @@ int build_work(pq_engine* e, std::vector<ClassWork>& out, double cut, double fp3
           cw.nf64 += f64;
           cw.nf32 += f32;
         }
+      }
+      for (int prec = 0; prec < 2 && split_prims; ++prec) {
+        const int tuned = e->group_cap[X][Y][prec];
+        const int rc = set_groups(cw, prec, tuned >= 0 ? tuned : default_group_cap(e->nbf));
+        if (rc) return rc;
       }
       PQ_HIP(dev_alloc(&cw.d_t64, cw.t64.size()));
       PQ_HIP(dev_alloc(&cw.d_t32, cw.t32.size()));
@@ -555,6 +599,67 @@ int pq_engine_set_streams(pq_engine* eng, int nstreams) {
   return PQ_OK;
 }
 
+int pq_engine_set_group_cap(pq_engine* eng, int cap) {
+  if (!eng) return fail(PQ_E_ARG, "null engine");
+  if (cap < -1 || cap > 1024) return fail(PQ_E_ARG, "cap must be -1..1024");
+  for (auto& a : eng->group_cap)
+    for (auto& b : a) b[0] = b[1] = cap;
+  if (eng->work_ok)
+    for (auto& c : eng->work)
+      for (int prec = 0; prec < 2; ++prec) {
+        int rc = set_groups(c, prec, cap >= 0 ? cap : default_group_cap(eng->nbf));
+        if (rc) return rc;
+      }
+  return PQ_OK;
+}
+
+int pq_engine_autotune(pq_engine* eng, const double* D_dev, int reps, void* stream, int* caps_out) {
+  if (!eng || !D_dev) return fail(PQ_E_ARG, "null argument");
+  if (!eng->work_ok) return fail(PQ_E_STATE, "pq_build_pairs not called");
+  if (reps < 1 || reps > 100) return fail(PQ_E_ARG, "reps must be 1..100");
+  static const int kCand[] = {0, 4, 8, 16, 32};
+  hipStream_t st = (hipStream_t)stream;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  PQ_HIP(hipEventCreate(&e0));
+  PQ_HIP(hipEventCreate(&e1));
+  int rc = PQ_OK;
+  for (size_t ci = 0; ci < eng->work.size() && rc == PQ_OK; ++ci) {
+    ClassWork& c = eng->work[ci];
+    const int la = kPairL[c.X][0], lb = kPairL[c.X][1], lc = kPairL[c.Y][0], ld = kPairL[c.Y][1];
+    const bool unsplit = ncart(la) * ncart(lb) * ncart(lc) * ncart(ld) <= kLaneMaxComp;
+    for (int prec = 0; prec < 2 && rc == PQ_OK; ++prec) {
+      const auto& tl = prec ? c.t32 : c.t64;
+      if (caps_out) caps_out[2 * ci + prec] = c.cap[prec];
+      if (tl.empty() || !unsplit) continue;
+      float best = 1e30f;
+      int best_cap = c.cap[prec];
+      for (int cap : kCand) {
+        if ((rc = set_groups(c, prec, cap)) != PQ_OK) break;
+        hipError_t err = hipSuccess;
+        for (int r = 0; r <= reps && err == hipSuccess; ++r) {
+          if (r == 1) err = hipEventRecord(e0, st);
+          if (err == hipSuccess)
+            err = launch_eri_class(la, lb, lc, ld, MODE_JK, prec, eng->view(), prec ? c.d_t32 : c.d_t64,
+                                   (int)tl.size(), eng->d_sorted[c.Y], D_dev, eng->d_Jacc, eng->d_Kacc, nullptr,
+                                   nullptr, st, prec ? c.d_g32 : c.d_g64, (int)(prec ? c.g32 : c.g64).size());
+        }
+        if (err == hipSuccess) err = hipEventRecord(e1, st);
+        if (err == hipSuccess) err = hipEventSynchronize(e1);
+        float t = 0.f;
+        if (err == hipSuccess) err = hipEventElapsedTime(&t, e0, e1);
+        if (err != hipSuccess) { rc = fail(PQ_E_HIP, hipGetErrorString(err)); break; }
+        if (t < best) { best = t; best_cap = cap; }
+      }
+      if (rc == PQ_OK) rc = set_groups(c, prec, best_cap);
+      eng->group_cap[c.X][c.Y][prec] = best_cap;
+      if (caps_out) caps_out[2 * ci + prec] = best_cap;
+    }
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  return rc;
+}
+
 int pq_engine_stats(const pq_engine* eng, uint64_t counts[6], double flops[2]) {
   if (!eng || !counts || !flops) return fail(PQ_E_ARG, "null argument");
   if (!eng->work_ok) return fail(PQ_E_STATE, "pq_build_pairs not called");
@@ -598,7 +703,7 @@ int pq_get2ints(pq_engine* eng, double* out_dev, uint64_t len, void* stream) {
   return PQ_OK;
 }
 
-int pq_engine_class_stats(const pq_engine* eng, int c, int l[4], uint64_t counts[6], double flops[2]) {
+int pq_engine_class_stats(const pq_engine* eng, int c, int l[4], uint64_t counts[8], double flops[2]) {
   if (!eng || !l || !counts || !flops) return fail(PQ_E_ARG, "null argument");
   if (!eng->work_ok) return fail(PQ_E_STATE, "pq_build_pairs not called");
   if (c < 0 || c >= (int)eng->work.size()) return fail(PQ_E_ARG, "class index out of range");
@@ -613,6 +718,8 @@ int pq_engine_class_stats(const pq_engine* eng, int c, int l[4], uint64_t counts
   counts[3] = (uint64_t)w.np32;
   counts[4] = (uint64_t)w.nf64;
   counts[5] = (uint64_t)w.nf32;
+  counts[6] = w.t64.size();
+  counts[7] = w.t32.size();
   flops[0] = w.np64 * rys_flops_prim(l[0], l[1], l[2], l[3]) + w.nq64 * rys_flops_dig(l[0], l[1], l[2], l[3]);
   flops[1] = w.np32 * rys_flops_prim(l[0], l[1], l[2], l[3]) + w.nq32 * rys_flops_dig(l[0], l[1], l[2], l[3]);
   return PQ_OK;
@@ -673,7 +780,7 @@ static int fock_jk_impl(pq_engine* eng, const double* D_dev, double* J_dev, doub
           if (tl.empty()) continue;
           err = launch_eri_class(la, lb, lc, ld, MODE_JK, prec, eng->view(), prec ? c.d_t32 : c.d_t64,
                                  (int)tl.size(), eng->d_sorted[c.Y], D_dev, eng->d_Jacc, eng->d_Kacc, nullptr,
-                                 nullptr, st);
+                                 nullptr, st, prec ? c.d_g32 : c.d_g64, (int)(prec ? c.g32 : c.g64).size());
         }
       }
       if (err != hipSuccess) { rc = fail(PQ_E_HIP, hipGetErrorString(err)); break; }
@@ -701,7 +808,8 @@ static int fock_jk_impl(pq_engine* eng, const double* D_dev, double* J_dev, doub
           if (tl.empty()) continue;
           err = launch_eri_class(la, lb, lc, ld, MODE_JK, prec, eng->view(), prec ? c.d_t32 : c.d_t64,
                                  (int)tl.size(), eng->d_sorted[c.Y], D_dev, eng->d_Jacc, eng->d_Kacc, nullptr,
-                                 nullptr, eng->side[next]);
+                                 nullptr, eng->side[next], prec ? c.d_g32 : c.d_g64,
+                                 (int)(prec ? c.g32 : c.g64).size());
           next = (next + 1) % eng->nside;
         }
       }
@@ -720,7 +828,7 @@ static int fock_jk_impl(pq_engine* eng, const double* D_dev, double* J_dev, doub
         if ((err = mark()) != hipSuccess) { rc = fail(PQ_E_HIP, hipGetErrorString(err)); break; }
         err = launch_eri_class(la, lb, lc, ld, MODE_JK, prec, eng->view(), prec ? c.d_t32 : c.d_t64,
                                (int)tl.size(), eng->d_sorted[c.Y], D_dev, eng->d_Jacc, eng->d_Kacc, nullptr,
-                               nullptr, st);
+                               nullptr, st, prec ? c.d_g32 : c.d_g64, (int)(prec ? c.g32 : c.g64).size());
         if (err != hipSuccess) { rc = fail(PQ_E_HIP, hipGetErrorString(err)); break; }
         if ((err = mark()) != hipSuccess) { rc = fail(PQ_E_HIP, hipGetErrorString(err)); break; }
         slot.push_back((int)(2 * ci + prec));

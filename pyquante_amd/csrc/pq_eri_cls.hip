@@ -18,19 +18,20 @@ hipError_t PQ_CAT(PQ_LA, PQ_LB, PQ_LC, PQ_LD, PQ_MODE)(const EngineView& ev, con
 #else
 hipError_t PQ_CAT(PQ_LA, PQ_LB, PQ_LC, PQ_LD, PQ_MODE)(const EngineView& ev, const Tile* tiles, int ntiles,
                                                        const int* kets, const double* D, double* Jacc,
-                                                       double* Kacc, double* ints, double* qout, hipStream_t st) {
+                                                       double* Kacc, double* ints, double* qout, hipStream_t st,
+                                                       const int2* groups, int ngroups) {
 #if PQ_MODE == 0
   return launch_one<PQ_LA, PQ_LB, PQ_LC, PQ_LD, double, MODE_JK>(ev, tiles, ntiles, kets, D, Jacc, Kacc, ints,
-                                                                 qout, st);
+                                                                 qout, st, groups, ngroups);
 #elif PQ_MODE == 1
   return launch_one<PQ_LA, PQ_LB, PQ_LC, PQ_LD, float, MODE_JK>(ev, tiles, ntiles, kets, D, Jacc, Kacc, ints,
-                                                                qout, st);
+                                                                qout, st, groups, ngroups);
 #elif PQ_MODE == 2
   return launch_one<PQ_LA, PQ_LB, PQ_LC, PQ_LD, double, MODE_STORE>(ev, tiles, ntiles, kets, D, Jacc, Kacc, ints,
-                                                                    qout, st);
+                                                                    qout, st, nullptr, 0);
 #else
   return launch_one<PQ_LA, PQ_LB, PQ_LC, PQ_LD, double, MODE_DIAG>(ev, tiles, ntiles, kets, D, Jacc, Kacc, ints,
-                                                                   qout, st);
+                                                                   qout, st, nullptr, 0);
 #endif
 }
 #endif

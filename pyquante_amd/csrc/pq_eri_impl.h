@@ -342,6 +342,47 @@ __device__ __forceinline__ void static_switch(int i, F&& f) {
 //     the ket-side J accumulator.
 constexpr int kWaveSplit = 4;
 
+// Unsplit classes with bra groups: one block of 4 wavefronts walks a run of
+// tiles that share the bra pair (wavefront w takes tiles w, w+4, ...), so the
+// LDS K rows are zeroed and flushed once per run.
+template <int LA, int LB, int LC, int LD, typename T, int MODE>
+__global__ __launch_bounds__(256) void k_eri_group(EngineView ev, const Tile* __restrict__ tiles,
+                                                   const int2* __restrict__ groups, int ngroups,
+                                                   const int* __restrict__ kets, const double* __restrict__ D,
+                                                   double* Jacc, double* Kacc) {
+  using C = Cls<LA, LB, LC, LD>;
+  static_assert(C::SPLIT == 0, "bra groups are for unsplit classes");
+  extern __shared__ double pq_lds[];
+  const int lane = threadIdx.x & 63;
+  const int wv = threadIdx.x >> 6;
+  const int n = ev.nbf;
+  const int2 g = groups[blockIdx.x];
+  double* krow = pq_lds;
+  for (int i = threadIdx.x; i < (C::NA + C::NB) * n; i += blockDim.x) pq_lds[i] = 0.0;
+  __syncthreads();
+  for (int tt = wv; tt < g.y; tt += 4) {
+    Tile tl = tiles[g.x + tt];
+    tl.bra = __builtin_amdgcn_readfirstlane(tl.bra);
+    tl.ket0 = __builtin_amdgcn_readfirstlane(tl.ket0);
+    tl.nket = __builtin_amdgcn_readfirstlane(tl.nket);
+    tl.prim = __builtin_amdgcn_readfirstlane(tl.prim);
+    const bool active = lane < tl.nket;
+    const int ket = kets[tl.ket0 + (active ? lane : 0)];
+    lane_body<LA, LB, LC, LD, T, MODE_JK, 0, C::NA, 0, C::NB>(ev, tl, active, tl.bra, ket, D, Jacc, Kacc, nullptr,
+                                                             nullptr, krow, nullptr);
+  }
+  __syncthreads();
+  const PairInfo bp = ev.pairs[tiles[g.x].bra];
+  for (int i = threadIdx.x; i < (C::NA + C::NB) * n; i += blockDim.x) {
+    const double v = krow[i];
+    if (v != 0.0) {
+      const int r = i / n, col = i - r * n;
+      const int grow = r < C::NA ? bp.bfA + r : bp.bfB + (r - C::NA);
+      unsafeAtomicAdd(&Kacc[grow * n + col], v);
+    }
+  }
+}
+
 template <int LA, int LB, int LC, int LD, typename T, int MODE>
 __global__ __launch_bounds__(256) void k_eri_lane(EngineView ev, const Tile* __restrict__ tiles, int ntiles,
                                                   const int* __restrict__ kets, const double* __restrict__ D,
@@ -422,10 +463,17 @@ __global__ __launch_bounds__(256) void k_eri_lane(EngineView ev, const Tile* __r
 template <int LA, int LB, int LC, int LD, typename T, int MODE>
 static hipError_t launch_one(const EngineView& ev, const Tile* tiles, int ntiles, const int* kets,
                              const double* D, double* Jacc, double* Kacc, double* ints, double* qout,
-                             hipStream_t st) {
+                             hipStream_t st, const int2* groups, int ngroups) {
   if (ntiles <= 0) return hipSuccess;
   using C = Cls<LA, LB, LC, LD>;
   const size_t lds = ((size_t)(C::NA + C::NB) * ev.nbf + (C::SPLIT ? 64 * C::NC * C::ND : 0)) * sizeof(double);
+  if constexpr (C::SPLIT == 0 && MODE == MODE_JK) {
+    if (groups && ngroups > 0 && lds <= 65536) {
+      hipLaunchKernelGGL((k_eri_group<LA, LB, LC, LD, T, MODE>), dim3(ngroups), dim3(256), lds, st, ev, tiles,
+                         groups, ngroups, kets, D, Jacc, Kacc);
+      return hipGetLastError();
+    }
+  }
   if (MODE == MODE_JK && lds <= 65536) {
     const int waves = C::SPLIT ? kWaveSplit : 1;
     const int chunks = (C::NSPLIT + waves - 1) / waves;

@@ -113,6 +113,22 @@ class Engine(object):
         """Spread class kernels over n engine streams (0 = caller's stream only)."""
         check(lib.pq_engine_set_streams(self.handle, int(n)))
 
+    def set_group_cap(self, cap):
+        """Tiles per bra run for every class (0 = none, -1 = default)."""
+        check(lib.pq_engine_set_group_cap(self.handle, int(cap)))
+
+    def autotune(self, D, reps=3):
+        """Times each class's J/K kernel on density D for the candidate bra
+        run lengths and keeps the fastest (pq_engine_autotune); returns the
+        chosen cap per (class, precision) as a 21 x 2 list."""
+        if self.cut is None:
+            self.build()
+        D = self._dtensor(D)
+        caps = (ctypes.c_int * 42)()
+        check(lib.pq_engine_autotune(self.handle, ctypes.c_void_p(D.data_ptr()), int(reps), _stream(), caps))
+        torch.cuda.synchronize(self.device)
+        return [[caps[2 * c], caps[2 * c + 1]] for c in range(21)]
+
     def stats(self):
         counts = (ctypes.c_uint64 * 6)()
         flops = (ctypes.c_double * 2)()
@@ -126,12 +142,13 @@ class Engine(object):
         out = []
         for c in range(21):
             l = (ctypes.c_int * 4)()
-            counts = (ctypes.c_uint64 * 6)()
+            counts = (ctypes.c_uint64 * 8)()
             flops = (ctypes.c_double * 2)()
             check(lib.pq_engine_class_stats(self.handle, c, l, counts, flops))
             out.append({"class": tuple(l), "quartets_fp64": counts[0], "quartets_fp32": counts[1],
                         "prim_quartets_fp64": counts[2], "prim_quartets_fp32": counts[3],
                         "fquartets_fp64": counts[4], "fquartets_fp32": counts[5],
+                        "tiles_fp64": counts[6], "tiles_fp32": counts[7],
                         "flops_fp64": flops[0], "flops_fp32": flops[1]})
         return out
 
@@ -166,6 +183,10 @@ class Engine(object):
                              ctypes.c_void_p(K.data_ptr() if K is not None else 0), int(precision),
                              _stream()))
         return J, K
+
+    def capture_fock_jk(self, D, precision=_abi.PQ_PREC_FP64):
+        """Captures one direct Fock build into a HIP graph (FockGraph)."""
+        return FockGraph(self, D, precision)
 
     def gradient(self, D, W):
         """dE/dR (natom x 3, hartree/bohr) of the RHF energy at density D and
@@ -209,3 +230,37 @@ def digest_stored(ints, D, want_j=True, want_k=True):
                                ctypes.c_void_p(J.data_ptr() if J is not None else 0),
                                ctypes.c_void_p(K.data_ptr() if K is not None else 0), _stream()))
     return J, K
+
+
+class FockGraph:
+    """One pq_fock_jk call captured into a HIP graph on fixed D/J/K buffers.
+
+    Calling it copies a new density into the captured D buffer (when given)
+    and replays the graph: the fill, class and symmetrisation kernels of the
+    build, with the engine's stream fork/join, are launched as one graph.
+    The returned J and K are the graph's own buffers, overwritten by the
+    next replay.  The work lists must not be rebuilt (pq_build_pairs,
+    autotune) while the graph is in use: the graph holds their addresses.
+    """
+
+    def __init__(self, engine, D, precision=_abi.PQ_PREC_FP64):
+        self.engine = engine
+        self.precision = precision
+        self.D = engine._dtensor(D).clone()
+        side = torch.cuda.Stream(device=engine.device)
+        side.wait_stream(torch.cuda.current_stream(engine.device))
+        with torch.cuda.stream(side):
+            # first call outside capture: creates the engine's streams/events
+            engine.fock_jk(self.D, precision=precision)
+        torch.cuda.current_stream(engine.device).wait_stream(side)
+        torch.cuda.synchronize(engine.device)
+        self.graph = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(self.graph):
+            self.J, self.K = engine.fock_jk(self.D, precision=precision)
+        torch.cuda.synchronize(engine.device)
+
+    def __call__(self, D=None):
+        if D is not None:
+            self.D.copy_(self.engine._dtensor(D))
+        self.graph.replay()
+        return self.J, self.K
