@@ -5,7 +5,7 @@ cluster (200 Cartesian functions, Schwarz cutoff 1e-12, FP64).
 A step is one direct Fock build (J and K from one density, pq_fock_jk) over
 the full screened quartet list, replayed from a HIP graph capture of that call
 (--eager times plain calls); the per-class bra-run lengths are autotuned on
-the same density first (--no-tune keeps the defaults).  Prints one JSON line.  The roofline block is
+the same density (stored table pyquante_amd/tuning.json, --tune).  Prints one JSON line.  The roofline block is
 for the dominant class kernel: algorithmic Rys flops of that kernel (the
 F_prim/F_dig operation count) over its HIP-event time on the launch stream,
 against the MI355X FP64 peak.  The cpu_baseline block times the compiled
@@ -76,6 +76,22 @@ def kernel_name(cls, cap, nbf):
     return "k_eri_group" if ncomp <= 81 and cap > 0 else "k_eri_lane"
 
 
+def apply_tuning(eng, D, name, args, precision=0):
+    """Sets the engine's bra-run caps for workload `name` (see --tune);
+    returns the 21 x 2 cap table in use, or None for the defaults."""
+    from pyquante_amd import tuning
+    if args.tune == "none":
+        return None
+    caps = tuning.load(name) if args.tune == "table" else None
+    if caps is None:
+        caps = eng.autotune(D)
+        if args.write_tuning:
+            tuning.store(name, caps, nbf=eng.nbf, precision="mixed" if precision else "fp64")
+    else:
+        eng.set_group_caps(caps)
+    return caps
+
+
 def _timed(fn, steps):
     import torch
     fn()
@@ -104,15 +120,13 @@ def extra_workloads(args):
     eng = Engine(BasisSet(mol, "6-31g**"), mol)
     eng.build(1e-12, 0.0)
     e64, _, _, D64, _ = scf(eng, mol, ConvCriteria=1e-9)
-    if not args.no_tune:
-        eng.autotune(D64)
+    apply_tuning(eng, D64, "h2o16_631ss_fp64", args)
     t64 = _timed(eng.capture_fock_jk(D64), args.steps)
     st64 = eng.stats()
     eng.build(1e-12, 1e-4)
     st32 = eng.stats()
     emx, _, _, _, _ = scf(eng, mol, ConvCriteria=1e-9, precision=_abi.PQ_PREC_MIXED)
-    if not args.no_tune:
-        eng.autotune(D64)
+    apply_tuning(eng, D64, "h2o16_631ss_mixed", args, precision=1)
     tmx = _timed(eng.capture_fock_jk(D64, precision=_abi.PQ_PREC_MIXED), args.steps)
     out.update({"h2o16_nbf": eng.nbf, "h2o16_fp64_ms_per_fock_build": t64 * 1e3,
                 "h2o16_mixed_ms_per_fock_build": tmx * 1e3, "h2o16_mixed_speedup": t64 / tmx,
@@ -148,7 +162,10 @@ def main():
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run one 256 MiB device copy (reference for rocprofv3 FETCH/WRITE_SIZE)")
     ap.add_argument("--classes", action="store_true", help="print per-class kernel times")
-    ap.add_argument("--no-tune", action="store_true", help="skip the per-class autotuner (default bra runs)")
+    ap.add_argument("--tune", choices=["table", "live", "none"], default="table",
+                    help="bra-run caps: the stored autotuner table for the workload (falling back to a live "
+                         "autotune when none is stored), a live autotune, or the defaults")
+    ap.add_argument("--write-tuning", action="store_true", help="store live autotune results in the tuning table")
     ap.add_argument("--eager", action="store_true", help="time eager pq_fock_jk calls instead of graph replays")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -188,10 +205,9 @@ def main():
     D = mkdens(C, 0, nocc).contiguous()
     print("bench: (H2O)_8 6-31G** nbf=%d, %d shell quartets, %d function quartets (unscreened %d)"
           % (eng.nbf, st["quartets_fp64"], st["fquartets_fp64"], unscreened), file=sys.stderr, flush=True)
-    caps = None
-    if not args.no_tune:
-        caps = eng.autotune(D)
-        print("bench: autotuned bra-run caps " + " ".join(
+    caps = apply_tuning(eng, D, "h2o8_631ss_fp64", args)
+    if caps is not None:
+        print("bench: bra-run caps " + " ".join(
             "(%s%s|%s%s)=%d" % (*("spd"[x] for x in cc["class"]), caps[ci][0])
             for ci, cc in enumerate(eng.class_stats()) if cc["quartets_fp64"]), file=sys.stderr, flush=True)
     J0, K0 = eng.fock_jk(D)
@@ -268,7 +284,7 @@ def main():
                                "Schwarz 1e-12, FP64", "nbf": eng.nbf, "shell_quartets": st["quartets_fp64"],
                    "function_quartets": fq, "unscreened_function_quartets": unscreened,
                    "s_per_fock_build": dt, "timing": "eager" if args.eager else "HIP graph replay",
-                   "s_per_fock_build_eager": dt_eager, "autotuned": caps is not None, **extra},
+                   "s_per_fock_build_eager": dt_eager, "bra_run_caps": args.tune, **extra},
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved / FP64_PEAK_TFLOPS, "traffic": None,
                      "kernel": "%s %s FP64 J/K" % (kernel_name(c["class"], caps[k][0] if caps else None, eng.nbf),

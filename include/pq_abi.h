@@ -107,6 +107,12 @@ int pq_engine_set_streams(pq_engine *eng, int nstreams);
  * reference counterpart (a launch knob). */
 int pq_engine_set_group_cap(pq_engine *eng, int cap);
 
+/* Per-class bra-run caps: caps[2*c + prec] for class index c (the order of
+ * pq_engine_class_stats) and prec 0 = FP64 tiles, 1 = FP32 tiles; -1 keeps
+ * the default.  For replaying a table written by pq_engine_autotune.  No
+ * reference counterpart (a launch knob). */
+int pq_engine_set_group_caps(pq_engine *eng, const int *caps);
+
 /* Autotuner: for each class and precision of the current work lists, times
  * the J/K kernel on D_dev (`reps` timed launches after one warm-up) for run
  * caps {0, 4, 8, 16, 32} on `stream` and keeps the fastest.  The choice

@@ -16,7 +16,15 @@ the device).  For X >= XA_n the large-X limit is used:
 with (s_i, 2 l_i) the Gauss rule of s^(-1/2) e^(-s) on [0, inf); XA_n is
 chosen so that this limit is accurate to < 1e-15 relative.
 
-Run: python gen_rys_table.py  (writes rys_table.h next to this file).
+FP32 table (rys_table32.h, `--fp32`): the same construction with degree
+DEG32 = 6 on unit intervals (max relative error ~2e-8 over n <= 5, below
+float rounding), stored as float with each function padded to 8 entries so
+that a function's coefficients are two aligned 16-byte loads; the large-X
+limit (evaluated in FP64 on the device) takes over where it is accurate to
+< 5e-8 relative.
+
+Run: python gen_rys_table.py [--fp32]  (writes rys_table.h or rys_table32.h
+next to this file).
 """
 import multiprocessing as mpr
 import os
@@ -89,8 +97,17 @@ def asym(n, X):
     return [si / X for si in s], [li / (2 * mp.sqrt(X)) for li in lam]
 
 
+DEG32 = 6
+STRIDE32 = 8
+
+
 def fit_interval(args):
-    n, k = args
+    n, k = args[:2]
+    deg = args[2] if len(args) > 2 else DEG
+    return _fit(n, k, deg)
+
+
+def _fit(n, k, DEG):
     x0 = k * H
     nodes = [mp.cos(mp.pi * (j + mp.mpf(1) / 2) / (DEG + 1)) for j in range(DEG + 1)]
     vals = []
@@ -114,17 +131,53 @@ def fit_interval(args):
     return n, k, coefs
 
 
-def threshold(n):
-    """Smallest multiple of H past which the large-X limit is < 1e-15 relative."""
-    X = 10.0
+def threshold(n, tol=1e-15, start=10.0):
+    """Smallest multiple of H (from start) past which the large-X limit is
+    < tol relative."""
+    X = start
     while True:
         u, w = rys(n, X)
         ua, wa = asym(n, X)
         err = max(max(abs((a - b) / b) for a, b in zip(ua, u)),
                   max(abs((a - b) / b) for a, b in zip(wa, w)))
-        if err < 1e-15:
+        if err < tol:
             return X
         X += H
+
+
+def main32():
+    xa = {n: threshold(n, 5e-8, 4.0) for n in range(1, NMAX + 1)}
+    jobs = [(n, k, DEG32) for n in range(1, NMAX + 1) for k in range(int(round(xa[n] / H)))]
+    with mpr.Pool(min(8, os.cpu_count() or 1)) as pool:
+        res = pool.map(fit_interval, jobs, chunksize=2)
+    table = {(n, k): coefs for n, k, coefs in res}
+    nint = {n: int(round(xa[n] / H)) for n in range(1, NMAX + 1)}
+    offs = [0]
+    for n in range(1, NMAX + 1):
+        offs.append(offs[-1] + nint[n] * 2 * n * STRIDE32)
+    lines = ["// Generated by gen_rys_table.py --fp32 -- do not edit.",
+             "// FP32 piecewise Rys roots u_i = t_i^2 and weights w_i, n = 1..%d." % NMAX,
+             "#pragma once", "",
+             "#ifndef RYS_DEV",
+             "#define RYS_DEV __device__",
+             "#endif", "",
+             "#define RYS_DEG32 %d" % DEG32,
+             "#define RYS_STRIDE32 %d" % STRIDE32, ""]
+    lines.append("RYS_DEV const int rys_off32[%d] = {%s};" % (NMAX, ", ".join(str(o) for o in offs[:NMAX])))
+    lines.append("RYS_DEV const double rys_xasym32[%d] = {0, %s};" %
+                 (NMAX + 1, ", ".join("%.1f" % xa[n] for n in range(1, NMAX + 1))))
+    lines.append("// coefficients: [n][interval][function: u_1..u_n, w_1..w_n][power 0..DEG32, zero pad]")
+    lines.append("RYS_DEV const float __attribute__((aligned(16))) rys_coef32[%d] = {" % offs[-1])
+    body = []
+    for n in range(1, NMAX + 1):
+        for k in range(nint[n]):
+            for f in table[(n, k)]:
+                v = list(f) + [0.0] * (STRIDE32 - len(f))
+                body.append("  " + ", ".join("%.9ef" % x for x in v))
+    lines.append(",\n".join(body) + "};")
+    with open(os.path.join(HERE, "rys_table32.h"), "w") as f:
+        f.write("\n".join(lines) + "\n")
+    print("fp32 thresholds", xa, "floats", offs[-1])
 
 
 def main():
@@ -175,4 +228,4 @@ def main():
 
 
 if __name__ == "__main__":
-    sys.exit(main())
+    sys.exit(main32() if "--fp32" in sys.argv[1:] else main())

@@ -6,6 +6,7 @@
 #include <stdint.h>
 
 #include "rys_table.h"
+#include "rys_table32.h"
 
 namespace pq {
 
@@ -157,6 +158,75 @@ __device__ __forceinline__ void rys_root(const RysPoint<NR>& p, int r, T& u, T& 
   w = vw;
 }
 
+// FP32 rule (FP32 quartets of mixed-precision builds): degree-6 float table
+// below rys_xasym32[NR], each function's coefficients two aligned float4
+// loads; above it the large-X limit, evaluated in FP64 and rounded.
+template <int NR>
+struct RysPoint32 {
+  bool asym;
+  double rx, rs;
+  const float* c;
+  float sv;
+};
+
+template <int NR>
+__device__ __forceinline__ RysPoint32<NR> rys_prepare32(double X) {
+  RysPoint32<NR> p;
+  p.asym = X >= rys_xasym32[NR];
+  if (p.asym) {
+    p.rx = 1.0 / X;
+    p.rs = rsqrt(X);
+    p.c = nullptr;
+    p.sv = 0.f;
+  } else {
+    const int k = (int)X;
+    p.rx = p.rs = 0.0;
+    p.sv = (float)(2.0 * (X - (double)k) - 1.0);
+    p.c = rys_coef32 + rys_off32[NR - 1] + k * (2 * NR * RYS_STRIDE32);
+  }
+  return p;
+}
+
+__device__ __forceinline__ float horner6(const float4 lo, const float4 hi, float s) {
+  static_assert(RYS_DEG32 == 6 && RYS_STRIDE32 == 8, "table layout");
+  float v = hi.z;
+  v = v * s + hi.y;
+  v = v * s + hi.x;
+  v = v * s + lo.w;
+  v = v * s + lo.z;
+  v = v * s + lo.y;
+  return v * s + lo.x;
+}
+
+template <int NR>
+__device__ __forceinline__ void rys_root32(const RysPoint32<NR>& p, int r, float& u, float& w) {
+  if (p.asym) {
+    u = (float)(rys_asym[NR - 1][r] * p.rx);
+    w = (float)(rys_asym[NR - 1][NR + r] * p.rs);
+    return;
+  }
+  const float4* cu = reinterpret_cast<const float4*>(p.c + r * RYS_STRIDE32);
+  const float4* cw = reinterpret_cast<const float4*>(p.c + (NR + r) * RYS_STRIDE32);
+  u = horner6(cu[0], cu[1], p.sv);
+  w = horner6(cw[0], cw[1], p.sv);
+}
+
+// Rule selection by arithmetic type: FP64 table for double, FP32 table for float.
+template <int NR, typename T>
+struct RysRule {
+  using Point = RysPoint<NR>;
+  static __device__ __forceinline__ Point prepare(double X) { return rys_prepare<NR>(X); }
+  static __device__ __forceinline__ void root(const Point& p, int r, T& u, T& w) { rys_root<NR, T>(p, r, u, w); }
+};
+template <int NR>
+struct RysRule<NR, float> {
+  using Point = RysPoint32<NR>;
+  static __device__ __forceinline__ Point prepare(double X) { return rys_prepare32<NR>(X); }
+  static __device__ __forceinline__ void root(const Point& p, int r, float& u, float& w) {
+    rys_root32<NR>(p, r, u, w);
+  }
+};
+
 // Runtime-order variant (scalar entry points and one-electron kernels).
 template <typename T>
 __device__ inline void rys_roots_rt(int nr, double X, T* u, T* w) {
@@ -239,6 +309,20 @@ __host__ __device__ inline uint64_t ijkl2intindex(int64_t i, int64_t j, int64_t 
 }
 
 // Wavefront-wide sum (64 lanes) by butterfly exchange.
+// Load of read-only data at a wavefront-uniform address through the constant
+// address space, so that the compiler can use a scalar load (no vector memory
+// instruction per lane).  Only for memory no kernel writes while it runs.
+template <typename T>
+__device__ __forceinline__ T ld_uniform(const T* p) {
+  static_assert(sizeof(T) % 4 == 0, "word-sized records only");
+  T r;
+  const __attribute__((address_space(4))) uint32_t* src = (const __attribute__((address_space(4))) uint32_t*)p;
+  uint32_t* dst = reinterpret_cast<uint32_t*>(&r);
+#pragma unroll
+  for (int i = 0; i < (int)(sizeof(T) / 4); ++i) dst[i] = src[i];
+  return r;
+}
+
 __device__ __forceinline__ double wave_sum(double v) {
 #pragma unroll
   for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off, 64);

@@ -613,6 +613,24 @@ int pq_engine_set_group_cap(pq_engine* eng, int cap) {
   return PQ_OK;
 }
 
+int pq_engine_set_group_caps(pq_engine* eng, const int* caps) {
+  if (!eng || !caps) return fail(PQ_E_ARG, "null argument");
+  for (int i = 0; i < 42; ++i)
+    if (caps[i] < -1 || caps[i] > 1024) return fail(PQ_E_ARG, "caps must be -1..1024");
+  int ci = 0;
+  for (int X = 0; X < 6; ++X)
+    for (int Y = 0; Y <= X; ++Y, ++ci)
+      for (int prec = 0; prec < 2; ++prec) eng->group_cap[X][Y][prec] = caps[2 * ci + prec];
+  if (eng->work_ok)
+    for (auto& c : eng->work)
+      for (int prec = 0; prec < 2; ++prec) {
+        const int cap = eng->group_cap[c.X][c.Y][prec];
+        int rc = set_groups(c, prec, cap >= 0 ? cap : default_group_cap(eng->nbf));
+        if (rc) return rc;
+      }
+  return PQ_OK;
+}
+
 int pq_engine_autotune(pq_engine* eng, const double* D_dev, int reps, void* stream, int* caps_out) {
   if (!eng || !D_dev) return fail(PQ_E_ARG, "null argument");
   if (!eng->work_ok) return fail(PQ_E_STATE, "pq_build_pairs not called");

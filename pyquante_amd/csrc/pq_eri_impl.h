@@ -71,7 +71,9 @@ __device__ __forceinline__ PrimQ prim_setup(const PairInfo& bp, const PrimPair& 
 // Contracted block of one shell quartet for bra components a in [A0, A0+NAS),
 // b in [B0, B0+NBS) and every ket component; PyQuante component order, without
 // the per-component normalisation factor.
-template <int LA, int LB, int LC, int LD, int A0, int NAS, int B0, int NBS, typename T>
+// UB: the bra pair is the same for the whole wavefront (scalar loads of its
+// primitive pairs).
+template <int LA, int LB, int LC, int LD, int A0, int NAS, int B0, int NBS, typename T, bool UB = true>
 __device__ __forceinline__ void eri_block(const EngineView& ev, const PairInfo& bp, const PairInfo& kp, int i0,
                                           int i1, T (&out)[NAS * NBS * ncart(LC) * ncart(LD)]) {
   using C = Cls<LA, LB, LC, LD>;
@@ -80,50 +82,54 @@ __device__ __forceinline__ void eri_block(const EngineView& ev, const PairInfo& 
   for (int c = 0; c < NAS * NBS * C::NC * C::ND; ++c) out[c] = (T)0;
   const T AB[3] = {(T)bp.ABx, (T)bp.ABy, (T)bp.ABz};
   const T CD[3] = {(T)kp.ABx, (T)kp.ABy, (T)kp.ABz};
+  // one primitive quartet of the contraction, accumulated into out
+  auto quartet = [&](const PrimPair& b, const PrimPair& q) __attribute__((always_inline)) {
+    const PrimQ s = prim_setup(bp, b, kp, q);
+    const typename RysRule<NR, T>::Point rp = RysRule<NR, T>::prepare(s.X);
+    const T pref = (T)s.pref, qfac = (T)s.qfac, pfac = (T)s.pfac;
+    const T half_rpq = (T)(0.5 * s.rpq), i2p = (T)s.i2p, i2q = (T)s.i2q;
+    // roots unrolled for the small orders; rolled above to bound code size
+    constexpr int UNR = NR <= 2 ? NR : 1;
+#pragma unroll UNR
+    for (int r = 0; r < NR; ++r) {
+      T ur, wr;
+      RysRule<NR, T>::root(rp, r, ur, wr);
+      const T B00 = half_rpq * ur;
+      const T B10 = i2p * ((T)1 - qfac * ur);
+      const T B01 = i2q * ((T)1 - pfac * ur);
+      T Ix[LA + 1][LB + 1][LC + 1][LD + 1], Iy[LA + 1][LB + 1][LC + 1][LD + 1],
+          Iz[LA + 1][LB + 1][LC + 1][LD + 1];
+      rys_1d<LA, LB, LC, LD, T>((T)1, (T)s.PA[0] - qfac * (T)s.PQ[0] * ur,
+                                (T)s.QC[0] + pfac * (T)s.PQ[0] * ur, B10, B01, B00, AB[0], CD[0], Ix);
+      rys_1d<LA, LB, LC, LD, T>((T)1, (T)s.PA[1] - qfac * (T)s.PQ[1] * ur,
+                                (T)s.QC[1] + pfac * (T)s.PQ[1] * ur, B10, B01, B00, AB[1], CD[1], Iy);
+      rys_1d<LA, LB, LC, LD, T>(wr * pref, (T)s.PA[2] - qfac * (T)s.PQ[2] * ur,
+                                (T)s.QC[2] + pfac * (T)s.PQ[2] * ur, B10, B01, B00, AB[2], CD[2], Iz);
+#pragma unroll
+      for (int a = 0; a < NAS; ++a)
+#pragma unroll
+        for (int bb = 0; bb < NBS; ++bb)
+#pragma unroll
+          for (int c = 0; c < C::NC; ++c)
+#pragma unroll
+            for (int d = 0; d < C::ND; ++d) {
+              const int o = ((a * NBS + bb) * C::NC + c) * C::ND + d;
+              out[o] += Ix[cart_pow(LA, A0 + a, 0)][cart_pow(LB, B0 + bb, 0)][cart_pow(LC, c, 0)][cart_pow(LD, d, 0)] *
+                        Iy[cart_pow(LA, A0 + a, 1)][cart_pow(LB, B0 + bb, 1)][cart_pow(LC, c, 1)][cart_pow(LD, d, 1)] *
+                        Iz[cart_pow(LA, A0 + a, 2)][cart_pow(LB, B0 + bb, 2)][cart_pow(LC, c, 2)][cart_pow(LD, d, 2)];
+            }
+    }
+  };
+  auto bra_prim = [&](int i) __attribute__((always_inline)) {
+    return UB ? ld_uniform(ev.prims + bp.pp_off + i) : ev.prims[bp.pp_off + i];
+  };
   // ket primitives (per lane) outside, bra primitives (shared by the
   // wavefront, scalar loads) inside
 #pragma unroll 1
   for (int k = 0; k < kp.pp_n; ++k) {
     const PrimPair q = ev.prims[kp.pp_off + k];
 #pragma unroll 1
-    for (int i = i0; i < i1; ++i) {
-      const PrimPair b = ev.prims[bp.pp_off + i];
-      const PrimQ s = prim_setup(bp, b, kp, q);
-      const RysPoint<NR> rp = rys_prepare<NR>(s.X);
-      const T pref = (T)s.pref, qfac = (T)s.qfac, pfac = (T)s.pfac;
-      const T half_rpq = (T)(0.5 * s.rpq), i2p = (T)s.i2p, i2q = (T)s.i2q;
-      // roots unrolled for the small orders; rolled above to bound code size
-      constexpr int UNR = NR <= 2 ? NR : 1;
-#pragma unroll UNR
-      for (int r = 0; r < NR; ++r) {
-        T ur, wr;
-        rys_root<NR, T>(rp, r, ur, wr);
-        const T B00 = half_rpq * ur;
-        const T B10 = i2p * ((T)1 - qfac * ur);
-        const T B01 = i2q * ((T)1 - pfac * ur);
-        T Ix[LA + 1][LB + 1][LC + 1][LD + 1], Iy[LA + 1][LB + 1][LC + 1][LD + 1],
-            Iz[LA + 1][LB + 1][LC + 1][LD + 1];
-        rys_1d<LA, LB, LC, LD, T>((T)1, (T)s.PA[0] - qfac * (T)s.PQ[0] * ur,
-                                  (T)s.QC[0] + pfac * (T)s.PQ[0] * ur, B10, B01, B00, AB[0], CD[0], Ix);
-        rys_1d<LA, LB, LC, LD, T>((T)1, (T)s.PA[1] - qfac * (T)s.PQ[1] * ur,
-                                  (T)s.QC[1] + pfac * (T)s.PQ[1] * ur, B10, B01, B00, AB[1], CD[1], Iy);
-        rys_1d<LA, LB, LC, LD, T>(wr * pref, (T)s.PA[2] - qfac * (T)s.PQ[2] * ur,
-                                  (T)s.QC[2] + pfac * (T)s.PQ[2] * ur, B10, B01, B00, AB[2], CD[2], Iz);
-#pragma unroll
-        for (int a = 0; a < NAS; ++a)
-#pragma unroll
-          for (int bb = 0; bb < NBS; ++bb)
-#pragma unroll
-            for (int c = 0; c < C::NC; ++c)
-#pragma unroll
-              for (int d = 0; d < C::ND; ++d) {
-                const int o = ((a * NBS + bb) * C::NC + c) * C::ND + d;
-                out[o] += Ix[cart_pow(LA, A0 + a, 0)][cart_pow(LB, B0 + bb, 0)][cart_pow(LC, c, 0)][cart_pow(LD, d, 0)] *
-                          Iy[cart_pow(LA, A0 + a, 1)][cart_pow(LB, B0 + bb, 1)][cart_pow(LC, c, 1)][cart_pow(LD, d, 1)] *
-                          Iz[cart_pow(LA, A0 + a, 2)][cart_pow(LB, B0 + bb, 2)][cart_pow(LC, c, 2)][cart_pow(LD, d, 2)];
-              }
-      }
-    }
+    for (int i = i0; i < i1; ++i) quartet(bra_prim(i), q);
   }
 }
 
@@ -279,13 +285,14 @@ __device__ __forceinline__ void lane_body(const EngineView& ev, const Tile& tl, 
                                           const double* __restrict__ D, double* Jacc, double* Kacc,
                                           double* ints, double* qout, double* krow, double* jsum) {
   using C = Cls<LA, LB, LC, LD>;
-  const PairInfo bp = ev.pairs[bra];
+  constexpr bool UB = MODE != MODE_DIAG;  // Schwarz diagonal: bra = ket per lane
+  const PairInfo bp = UB ? ld_uniform(ev.pairs + bra) : ev.pairs[bra];
   const PairInfo kp = ev.pairs[ket];
   constexpr int NO = NAS * NBS * C::NC * C::ND;
   T out[NO];
   if (active) {
     const int i0 = tl.prim & 0xffff, ni = tl.prim >> 16;
-    eri_block<LA, LB, LC, LD, A0, NAS, B0, NBS, T>(ev, bp, kp, i0, ni ? i0 + ni : bp.pp_n, out);
+    eri_block<LA, LB, LC, LD, A0, NAS, B0, NBS, T, UB>(ev, bp, kp, i0, ni ? i0 + ni : bp.pp_n, out);
   } else {
 #pragma unroll
     for (int c = 0; c < NO; ++c) out[c] = (T)0;
@@ -361,7 +368,7 @@ __global__ __launch_bounds__(256) void k_eri_group(EngineView ev, const Tile* __
   for (int i = threadIdx.x; i < (C::NA + C::NB) * n; i += blockDim.x) pq_lds[i] = 0.0;
   __syncthreads();
   for (int tt = wv; tt < g.y; tt += 4) {
-    Tile tl = tiles[g.x + tt];
+    Tile tl = ld_uniform(tiles + g.x + tt);
     tl.bra = __builtin_amdgcn_readfirstlane(tl.bra);
     tl.ket0 = __builtin_amdgcn_readfirstlane(tl.ket0);
     tl.nket = __builtin_amdgcn_readfirstlane(tl.nket);
@@ -405,7 +412,7 @@ __global__ __launch_bounds__(256) void k_eri_lane(EngineView ev, const Tile* __r
   } else if (t >= ntiles) {
     return;  // uniform per wavefront, no block barrier in this layout
   }
-  Tile tl = tiles[t];
+  Tile tl = ld_uniform(tiles + t);
   tl.bra = __builtin_amdgcn_readfirstlane(tl.bra);
   tl.ket0 = __builtin_amdgcn_readfirstlane(tl.ket0);
   tl.nket = __builtin_amdgcn_readfirstlane(tl.nket);

@@ -31,7 +31,7 @@ __device__ __forceinline__ void grad_lane(const EngineView& ev, const Tile& tl, 
   using C = Cls<LA, LB, LC, LD>;
   constexpr int NR = GCls<LA, LB, LC, LD>::NR;
   constexpr int NO = NAS * NBS * C::NC * C::ND;
-  const PairInfo bp = ev.pairs[bra];
+  const PairInfo bp = ld_uniform(ev.pairs + bra);
   const PairInfo kp = ev.pairs[ket];
   const int n = ev.nbf;
   const int a0 = bp.bfA + A0, b0 = bp.bfB + B0, c0 = kp.bfA, d0 = kp.bfB;
@@ -68,8 +68,8 @@ __device__ __forceinline__ void grad_lane(const EngineView& ev, const Tile& tl, 
       const double gam = ev.prim_a[kp.pp_off + k];
 #pragma unroll 1
       for (int i = i0; i < i1; ++i) {
-        const PrimPair b = ev.prims[bp.pp_off + i];
-        const double alp = ev.prim_a[bp.pp_off + i], bet = b.p - alp;
+        const PrimPair b = ld_uniform(ev.prims + bp.pp_off + i);
+        const double alp = ld_uniform(ev.prim_a + bp.pp_off + i), bet = b.p - alp;
         const PrimQ s = prim_setup(bp, b, kp, q);
         const RysPoint<NR> rp = rys_prepare<NR>(s.X);
         constexpr int UNR = NR <= 2 ? NR : 1;
@@ -156,7 +156,7 @@ __global__ __launch_bounds__(64) void k_eri_grad(EngineView ev, const Tile* __re
   double* gl = pq_lds;
   for (int i = lane; i < 3 * natom; i += 64) gl[i] = 0.0;
   __syncthreads();
-  Tile tl = tiles[t];
+  Tile tl = ld_uniform(tiles + t);
   tl.bra = __builtin_amdgcn_readfirstlane(tl.bra);
   tl.ket0 = __builtin_amdgcn_readfirstlane(tl.ket0);
   tl.nket = __builtin_amdgcn_readfirstlane(tl.nket);

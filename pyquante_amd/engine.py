@@ -117,6 +117,12 @@ class Engine(object):
         """Tiles per bra run for every class (0 = none, -1 = default)."""
         check(lib.pq_engine_set_group_cap(self.handle, int(cap)))
 
+    def set_group_caps(self, caps):
+        """Per-class bra-run caps, a 21 x 2 list (class, FP64/FP32 tiles) as
+        returned by autotune; -1 keeps the default."""
+        flat = np.ascontiguousarray(np.asarray(caps, dtype=np.int32).reshape(42))
+        check(lib.pq_engine_set_group_caps(self.handle, pi(flat)))
+
     def autotune(self, D, reps=3):
         """Times each class's J/K kernel on density D for the candidate bra
         run lengths and keeps the fastest (pq_engine_autotune); returns the

@@ -64,3 +64,44 @@ def test_gpu_paths_fail_loudly_without_device():
     from pyquante_amd import crys
     with pytest.raises(_abi.PQError):
         crys.Fgamma(0, 1.0)
+
+
+def _parse_table32():
+    path = os.path.join(os.path.dirname(__file__), "..", "pyquante_amd", "csrc", "rys_table32.h")
+    src = open(path).read()
+    off = [int(v) for v in re.search(r"rys_off32\[\d+\] = \{([^}]*)\}", src).group(1).split(",")]
+    xa = [float(v) for v in re.search(r"rys_xasym32\[\d+\] = \{([^}]*)\}", src).group(1).split(",")]
+    body = re.search(r"rys_coef32\[\d+\] = \{(.*?)\};", src, re.S).group(1)
+    coef = np.array([float(v.strip().rstrip("f")) for v in body.split(",")], dtype=np.float32)
+    stride = int(re.search(r"#define RYS_STRIDE32 (\d+)", src).group(1))
+    deg = int(re.search(r"#define RYS_DEG32 (\d+)", src).group(1))
+    return off, xa, coef, stride, deg
+
+
+def test_fp32_rys_table_moments():
+    """The FP32 root table, evaluated in float32 Horner as the device does,
+    reproduces the Boys moments sum_i w_i u_i^k = F_k(X), k < 2n, to float
+    accuracy; the FP64 large-X limit takes over at rys_xasym32."""
+    off, xa, coef, stride, deg = _parse_table32()
+    rng = np.random.default_rng(3)
+    for n in range(1, 6):
+        X = np.concatenate([[0.0, 0.5, xa[n] - 1e-9], rng.uniform(0, xa[n], 60), [xa[n], xa[n] + 3.7]])
+        for x in X:
+            if x < xa[n]:
+                k = int(x)
+                s = np.float32(2.0 * (x - k) - 1.0)
+                base = off[n - 1] + k * 2 * n * stride
+                vals = []
+                for f in range(2 * n):
+                    c = coef[base + f * stride: base + f * stride + deg + 1]
+                    v = c[deg]
+                    for j in range(deg - 1, -1, -1):
+                        v = np.float32(v * s + c[j])
+                    vals.append(float(v))
+                u, w = np.array(vals[:n]), np.array(vals[n:])
+            else:
+                u, w = O.rys_roots(n, x)  # the large-X branch is FP64 on the device too
+            for m in range(2 * n):
+                ref = O.boys(m, x)
+                got = float(np.sum(w * u ** m))
+                assert abs(got - ref) <= 1e-6 * ref + 1e-30, (n, x, m, got, ref)
