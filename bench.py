@@ -4,13 +4,20 @@ cluster (200 Cartesian functions, Schwarz cutoff 1e-12, FP64).
 
 A step is one direct Fock build (J and K from one density, pq_fock_jk) over
 the full screened quartet list, replayed from a HIP graph capture of that call
-(--eager times plain calls); the per-class bra-run lengths are autotuned on
-the same density (stored table pyquante_amd/tuning.json, --tune).  Prints one JSON line.  The roofline block is
-for the dominant class kernel: algorithmic Rys flops of that kernel (the
-F_prim/F_dig operation count) over its HIP-event time on the launch stream,
-against the MI355X FP64 peak.  The cpu_baseline block times the compiled
-reference crys.c (oracle/_ref) on a seeded sample of the same workload's
-unique function quartets, single-threaded.
+(--eager times plain calls).  The class kernels run concurrently on the
+engine's streams; their bra-run lengths come from the stored autotuner table
+(pyquante_amd/tuning.json, --tune).  Prints one JSON line.
+
+The roofline block is for the dominant class kernel: its algorithmic Rys
+flops per launch (the F_prim/F_dig operation count over the class's
+primitive and contracted quartets) over its average launch duration, timed
+with HIP events on the stream the kernel is launched on, in the same
+concurrent configuration as the timed builds, against the MI355X FP64 vector
+peak.  `traffic` is the HBM bytes per launch of that kernel from rocprofv3
+FETCH_SIZE/WRITE_SIZE runs (profiles/traffic.json), when recorded.  The
+cpu_baseline block times the compiled reference crys.c (oracle/_ref) on a
+seeded sample of the same workload's unique function quartets,
+single-threaded.
 """
 import argparse
 import json
@@ -66,6 +73,17 @@ def cpu_baseline(bfs, seconds=10.0, seed=0):
                       "crys.c contr_coulomb compiled -O2, %.1f s, 1 thread" % (done, t_used)}
 
 
+def recorded_traffic(key):
+    """HBM bytes per launch recorded for `key` (kernel and workload) in
+    profiles/traffic.json from rocprofv3 counter runs, or None."""
+    path = os.path.join(ROOT, "profiles", "traffic.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        entry = json.load(f).get(key)
+    return None if entry is None else entry["bytes_per_launch"]
+
+
 def kernel_name(cls, cap, nbf):
     """Kernel template a class's FP64 J/K tiles run in: bra runs (k_eri_group)
     for classes of at most 81 components when the run cap is nonzero."""
@@ -73,7 +91,8 @@ def kernel_name(cls, cap, nbf):
     ncomp = ncart[cls[0]] * ncart[cls[1]] * ncart[cls[2]] * ncart[cls[3]]
     if cap is None:
         cap = 16 if nbf >= 300 else 0
-    return "k_eri_group" if ncomp <= 81 and cap > 0 else "k_eri_lane"
+    tmpl = "k_eri_group" if ncomp <= 81 and cap > 0 else "k_eri_lane"
+    return "%s<%d, %d, %d, %d, double, 0>" % (tmpl, *cls)
 
 
 def apply_tuning(eng, D, name, args, precision=0):
@@ -226,9 +245,10 @@ def main():
         eng.fock_jk(D)
     torch.cuda.synchronize()
     dt_eager = (time.perf_counter() - t0) / args.steps
-    # per-class kernel times (HIP events on the launch stream)
+    # per-class kernel times: HIP events on each kernel's launch stream, same
+    # concurrent configuration as the timed builds
     cls = eng.class_stats()
-    nt = 3
+    nt = 10
     acc = np.zeros(43)
     for _ in range(nt):
         _, _, ms = eng.fock_jk_timed(D)
@@ -245,6 +265,8 @@ def main():
                      cc["flops_fp64"] / t / 1e12 if t > 0 else 0,
                      cc["fquartets_fp64"] / t / 1e9 if t > 0 else 0), file=sys.stderr)
     k = int(np.argmax(acc[:42:2]))
+    kname = kernel_name(cls[k]["class"], caps[k][0] if caps else None, eng.nbf)
+    traffic = recorded_traffic("%s h2o8_631ss_fp64" % kname)
     c = cls[k]
     ktime = acc[2 * k] * 1e-3
     achieved = c["flops_fp64"] / ktime / 1e12
@@ -285,10 +307,11 @@ def main():
                    "function_quartets": fq, "unscreened_function_quartets": unscreened,
                    "s_per_fock_build": dt, "timing": "eager" if args.eager else "HIP graph replay",
                    "s_per_fock_build_eager": dt_eager, "bra_run_caps": args.tune, **extra},
-        "roofline": {"bound": "mfma", "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
-                     "frac": achieved / FP64_PEAK_TFLOPS, "traffic": None,
-                     "kernel": "%s %s FP64 J/K" % (kernel_name(c["class"], caps[k][0] if caps else None, eng.nbf),
-                                                    lname),
+        "roofline": {"bound": "mfma", "compute": "FP64 vector ALU (Rys recurrences are not matrix products; "
+                                                 "78.6 TFLOP/s is also the MI355X FP64 vector peak)",
+                     "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+                     "frac": achieved / FP64_PEAK_TFLOPS, "traffic": traffic,
+                     "kernel": "%s, %s FP64 J/K" % (kname, lname), "launch_ms": ktime * 1e3,
                      "all_kernels_tflops": total_flops / (acc[42] * 1e-3) / 1e12},
         "cpu_baseline": cpu,
     }

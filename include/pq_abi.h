@@ -152,10 +152,12 @@ int pq_get2ints(pq_engine *eng, double *out_dev, uint64_t len, void *stream);
 int pq_fock_jk(pq_engine *eng, const double *D_dev, double *J_dev, double *K_dev, int precision,
                void *stream);
 
-/* pq_fock_jk with HIP events around every class launch on the given stream;
- * ms[2c] / ms[2c+1] receive the FP64 / FP32 kernel time of class c (21
- * classes, 42 floats), ms[42] the whole build.  Synchronises the stream.
- * No reference counterpart (measurement). */
+/* pq_fock_jk with HIP events around every class launch, recorded on the
+ * stream the kernel is launched on (the engine streams of
+ * pq_engine_set_streams, so concurrent kernels are timed as they run; 0
+ * streams times each kernel alone); ms[2c] / ms[2c+1] receive the FP64 / FP32
+ * kernel time of class c (21 classes, 42 floats), ms[42] the whole build.
+ * Synchronises the stream.  No reference counterpart (measurement). */
 int pq_fock_jk_timed(pq_engine *eng, const double *D_dev, double *J_dev, double *K_dev, int precision,
                      void *stream, float *ms);
 

@@ -767,42 +767,30 @@ static int fock_jk_impl(pq_engine* eng, const double* D_dev, double* J_dev, doub
     return fail(PQ_E_STATE, "work lists were built for mixed precision; rebuild with fp32_cut = 0");
   hipStream_t st = (hipStream_t)stream;
   const int64_t nn = (int64_t)eng->nbf * eng->nbf;
+  // timing: one event pair per class launch, recorded on the stream the
+  // kernel runs on (so concurrent launches are timed as they run)
   std::vector<hipEvent_t> ev;
-  auto mark = [&]() -> hipError_t {
-    if (!ms) return hipSuccess;
+  std::vector<int> slot;  // ms[] entry of each event pair after the first (whole build)
+  auto mark = [&](hipStream_t s) -> hipError_t {
     hipEvent_t e;
     hipError_t r = hipEventCreate(&e);
     if (r != hipSuccess) return r;
     ev.push_back(e);
-    return hipEventRecord(e, st);
-  };
-  auto cleanup = [&]() {
-    for (auto e : ev) (void)hipEventDestroy(e);
+    return hipEventRecord(e, s);
   };
   int rc = PQ_OK;
-  std::vector<int> slot;  // which ms[] entry each consecutive event pair measures
   do {
-    hipError_t err;
-    if ((err = mark()) != hipSuccess) { rc = fail(PQ_E_HIP, hipGetErrorString(err)); break; }
-    if ((err = launch_fill(eng->d_Jacc, nn, 0.0, st)) != hipSuccess ||
-        (err = launch_fill(eng->d_Kacc, nn, 0.0, st)) != hipSuccess) {
-      rc = fail(PQ_E_HIP, hipGetErrorString(err));
-      break;
-    }
-    if (!ms && eng->nside == 0) {
-      for (size_t ci = 0; ci < eng->work.size() && err == hipSuccess; ++ci) {
-        const auto& c = eng->work[ci];
-        const int la = kPairL[c.X][0], lb = kPairL[c.X][1], lc = kPairL[c.Y][0], ld = kPairL[c.Y][1];
-        for (int prec = 0; prec < 2 && err == hipSuccess; ++prec) {
-          const auto& tl = prec ? c.t32 : c.t64;
-          if (tl.empty()) continue;
-          err = launch_eri_class(la, lb, lc, ld, MODE_JK, prec, eng->view(), prec ? c.d_t32 : c.d_t64,
-                                 (int)tl.size(), eng->d_sorted[c.Y], D_dev, eng->d_Jacc, eng->d_Kacc, nullptr,
-                                 nullptr, st, prec ? c.d_g32 : c.d_g64, (int)(prec ? c.g32 : c.g64).size());
-        }
-      }
-      if (err != hipSuccess) { rc = fail(PQ_E_HIP, hipGetErrorString(err)); break; }
-    } else if (!ms) {
+    hipError_t err = hipSuccess;
+#define PQ_TRY(x)                                  \
+  if ((err = (x)) != hipSuccess) {                 \
+    rc = fail(PQ_E_HIP, hipGetErrorString(err));   \
+    break;                                         \
+  }
+    if (ms) PQ_TRY(mark(st));
+    PQ_TRY(launch_fill(eng->d_Jacc, nn, 0.0, st));
+    PQ_TRY(launch_fill(eng->d_Kacc, nn, 0.0, st));
+    const int nside = eng->nside;
+    if (nside > 0) {
       // fork: class kernels spread over the side streams, joined before the
       // symmetrisation (all of them only add into Jacc/Kacc)
       if (!eng->streams_ok) {
@@ -811,55 +799,42 @@ static int fock_jk_impl(pq_engine* eng, const double* D_dev, double* J_dev, doub
           if (err == hipSuccess) err = hipEventCreateWithFlags(&eng->join_ev[i], hipEventDisableTiming);
         }
         if (err == hipSuccess) err = hipEventCreateWithFlags(&eng->fork_ev, hipEventDisableTiming);
-        if (err != hipSuccess) { rc = fail(PQ_E_HIP, hipGetErrorString(err)); break; }
+        PQ_TRY(err);
         eng->streams_ok = true;
       }
-      if ((err = hipEventRecord(eng->fork_ev, st)) != hipSuccess) { rc = fail(PQ_E_HIP, hipGetErrorString(err)); break; }
-      for (int i = 0; i < eng->nside && err == hipSuccess; ++i) err = hipStreamWaitEvent(eng->side[i], eng->fork_ev, 0);
-      if (err != hipSuccess) { rc = fail(PQ_E_HIP, hipGetErrorString(err)); break; }
-      int next = 0;
-      for (size_t ci = 0; ci < eng->work.size() && err == hipSuccess; ++ci) {
-        const auto& c = eng->work[ci];
-        const int la = kPairL[c.X][0], lb = kPairL[c.X][1], lc = kPairL[c.Y][0], ld = kPairL[c.Y][1];
-        for (int prec = 0; prec < 2 && err == hipSuccess; ++prec) {
-          const auto& tl = prec ? c.t32 : c.t64;
-          if (tl.empty()) continue;
-          err = launch_eri_class(la, lb, lc, ld, MODE_JK, prec, eng->view(), prec ? c.d_t32 : c.d_t64,
-                                 (int)tl.size(), eng->d_sorted[c.Y], D_dev, eng->d_Jacc, eng->d_Kacc, nullptr,
-                                 nullptr, eng->side[next], prec ? c.d_g32 : c.d_g64,
-                                 (int)(prec ? c.g32 : c.g64).size());
-          next = (next + 1) % eng->nside;
-        }
-      }
-      for (int i = 0; i < eng->nside && err == hipSuccess; ++i) {
-        err = hipEventRecord(eng->join_ev[i], eng->side[i]);
-        if (err == hipSuccess) err = hipStreamWaitEvent(st, eng->join_ev[i], 0);
-      }
-      if (err != hipSuccess) { rc = fail(PQ_E_HIP, hipGetErrorString(err)); break; }
+      PQ_TRY(hipEventRecord(eng->fork_ev, st));
+      for (int i = 0; i < nside && err == hipSuccess; ++i) err = hipStreamWaitEvent(eng->side[i], eng->fork_ev, 0);
+      PQ_TRY(err);
     }
-    for (size_t ci = 0; ms && ci < eng->work.size() && rc == PQ_OK; ++ci) {
+    int next = 0;
+    for (size_t ci = 0; ci < eng->work.size() && rc == PQ_OK; ++ci) {
       const auto& c = eng->work[ci];
       const int la = kPairL[c.X][0], lb = kPairL[c.X][1], lc = kPairL[c.Y][0], ld = kPairL[c.Y][1];
       for (int prec = 0; prec < 2; ++prec) {
         const auto& tl = prec ? c.t32 : c.t64;
         if (tl.empty()) continue;
-        if ((err = mark()) != hipSuccess) { rc = fail(PQ_E_HIP, hipGetErrorString(err)); break; }
-        err = launch_eri_class(la, lb, lc, ld, MODE_JK, prec, eng->view(), prec ? c.d_t32 : c.d_t64,
-                               (int)tl.size(), eng->d_sorted[c.Y], D_dev, eng->d_Jacc, eng->d_Kacc, nullptr,
-                               nullptr, st, prec ? c.d_g32 : c.d_g64, (int)(prec ? c.g32 : c.g64).size());
-        if (err != hipSuccess) { rc = fail(PQ_E_HIP, hipGetErrorString(err)); break; }
-        if ((err = mark()) != hipSuccess) { rc = fail(PQ_E_HIP, hipGetErrorString(err)); break; }
-        slot.push_back((int)(2 * ci + prec));
+        hipStream_t ks = nside > 0 ? eng->side[next] : st;
+        if (nside > 0) next = (next + 1) % nside;
+        if (ms) PQ_TRY(mark(ks));
+        PQ_TRY(launch_eri_class(la, lb, lc, ld, MODE_JK, prec, eng->view(), prec ? c.d_t32 : c.d_t64,
+                                (int)tl.size(), eng->d_sorted[c.Y], D_dev, eng->d_Jacc, eng->d_Kacc, nullptr,
+                                nullptr, ks, prec ? c.d_g32 : c.d_g64, (int)(prec ? c.g32 : c.g64).size()));
+        if (ms) {
+          PQ_TRY(mark(ks));
+          slot.push_back((int)(2 * ci + prec));
+        }
       }
     }
     if (rc != PQ_OK) break;
-    if ((err = launch_symmetrize(eng->nbf, eng->d_Jacc, eng->d_Kacc, J_dev, K_dev, 1.0, 1.0, st)) != hipSuccess) {
-      rc = fail(PQ_E_HIP, hipGetErrorString(err));
-      break;
+    for (int i = 0; i < nside && err == hipSuccess; ++i) {
+      err = hipEventRecord(eng->join_ev[i], eng->side[i]);
+      if (err == hipSuccess) err = hipStreamWaitEvent(st, eng->join_ev[i], 0);
     }
-    if ((err = mark()) != hipSuccess) { rc = fail(PQ_E_HIP, hipGetErrorString(err)); break; }
+    PQ_TRY(err);
+    PQ_TRY(launch_symmetrize(eng->nbf, eng->d_Jacc, eng->d_Kacc, J_dev, K_dev, 1.0, 1.0, st));
     if (ms) {
-      if ((err = hipStreamSynchronize(st)) != hipSuccess) { rc = fail(PQ_E_HIP, hipGetErrorString(err)); break; }
+      PQ_TRY(mark(st));
+      PQ_TRY(hipStreamSynchronize(st));
       for (int i = 0; i < 43; ++i) ms[i] = 0.f;
       for (size_t k = 0; k < slot.size(); ++k) {
         float t = 0.f;
@@ -870,8 +845,9 @@ static int fock_jk_impl(pq_engine* eng, const double* D_dev, double* J_dev, doub
       (void)hipEventElapsedTime(&tot, ev.front(), ev.back());
       ms[42] = tot;
     }
+#undef PQ_TRY
   } while (0);
-  cleanup();
+  for (auto e : ev) (void)hipEventDestroy(e);
   return rc;
 }
 
