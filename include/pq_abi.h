@@ -24,7 +24,7 @@ extern "C" {
 enum {
   PQ_OK = 0,
   PQ_E_ARG = -1,      /* bad argument (null pointer, size mismatch) */
-  PQ_E_AM = -2,       /* shell angular momentum above d (l > 2) */
+  PQ_E_AM = -2,       /* angular momentum beyond what the entry point supports (shells above d, HGP leaves) */
   PQ_E_LTOTAL = -3,   /* L_total > 9: no Rys rule (the reference's Root6 stub, crys.c:1440-1443) */
   PQ_E_ALLOC = -4,    /* device allocation failed */
   PQ_E_HIP = -5,      /* HIP runtime error */
@@ -68,6 +68,19 @@ int pq_rys_roots_batch(int n, int nr, const double *X_host, double *u_host, doub
 int pq_contr_coulomb_batch(int n, const int *nprim_host, const int *poff_host, int nprim_total,
                            const double *exps_host, const double *coefs_host, const double *norms_host,
                            const double *origins_host, const int *powers_host, double *out_host);
+
+/* Head-Gordon-Pople contracted ERIs, replacing chgp.contr_coulomb /
+ * coulomb_repulsion (Src/chgp.c:78-169 contr_hrr/contr_vrr, 172-242 hrr,
+ * 244-419 vrr) and, with mext, chgp.vrr (chgp.c:857-890).  Same batch layout
+ * as pq_contr_coulomb_batch.  mext (NULL or n ints): the auxiliary order m of
+ * a primitive leaf [a0|c0]^(m) (powers of b and d must be zero).
+ * compat_nr = 1 uses the reference's Boys function (EPS = 3e-7, started at
+ * each leaf's own total order) and reproduces chgp's values; 0 uses the
+ * full-precision Boys function.  Limits (PQ_E_AM beyond): per side
+ * (a+b, c+d) prod_i (p_i + 1) <= 12 and total order L + mext <= 8. */
+int pq_hgp_contr_coulomb_batch(int n, const int *nprim, const int *poff, int nprim_total, const double *exps,
+                               const double *coefs, const double *norms, const double *origins, const int *powers,
+                               const int *mext, int compat_nr, double *out);
 
 /* ---------------- engine: basis, geometry, pair data ---------------- */
 

@@ -28,8 +28,16 @@ def _seq(x, n=None):
     return v
 
 
-def contr_coulomb_batch(quartets):
-    """quartets: list of 4-tuples of (exps, coefs, norms, origin, powers)."""
+RYS, HGP = 1, 2
+
+
+def contr_coulomb_batch(quartets, method=RYS, compat_nr=1, mext=None):
+    """quartets: list of 4-tuples of (exps, coefs, norms, origin, powers).
+
+    method RYS evaluates with the Rys engine (full precision); HGP with the
+    Head-Gordon-Pople kernel, whose compat_nr = 1 reproduces chgp.c's values
+    (its EPS = 3e-7 Boys function).  mext: per-quartet auxiliary order for
+    primitive VRR leaves (HGP only)."""
     _abi.require_gpu()
     nprim, poff, exps, coefs, norms, org, pw = [], [], [], [], [], [], []
     for q in quartets:
@@ -56,30 +64,47 @@ def contr_coulomb_batch(quartets):
                                            (coefs, np.float64), (norms, np.float64), (org, np.float64),
                                            (pw, np.int32))]
     try:
-        check(lib.pq_contr_coulomb_batch(n, pi(a[0]), pi(a[1]), len(exps), pd(a[2]), pd(a[3]), pd(a[4]),
-                                         pd(a[5]), pi(a[6]), pd(out)))
+        if method == HGP:
+            m = None if mext is None else np.ascontiguousarray(np.asarray(mext, dtype=np.int32).reshape(n))
+            check(lib.pq_hgp_contr_coulomb_batch(n, pi(a[0]), pi(a[1]), len(exps), pd(a[2]), pd(a[3]), pd(a[4]),
+                                                 pd(a[5]), pi(a[6]), None if m is None else pi(m), int(compat_nr),
+                                                 pd(out)))
+        else:
+            check(lib.pq_contr_coulomb_batch(n, pi(a[0]), pi(a[1]), len(exps), pd(a[2]), pd(a[3]), pd(a[4]),
+                                             pd(a[5]), pi(a[6]), pd(out)))
     except _abi.PQError as e:
-        if e.code == -3:
+        if e.code in (-2, -3):
             raise ValueError(str(e))
         raise
     return out
 
 
-def contr_coulomb(*args):
+def contr_coulomb(*args, method=RYS):
     if len(args) != 20:
         raise SystemError("error return without exception set")
     q = tuple(tuple(args[5 * f:5 * f + 5]) for f in range(4))
-    return float(contr_coulomb_batch([q])[0])
+    return float(contr_coulomb_batch([q], method)[0])
 
 
-def coulomb_repulsion(*args):
+def coulomb_repulsion(*args, method=RYS):
     if len(args) != 16:
         raise SystemError("error return without exception set")
     q = []
     for f in range(4):
         o, nrm, p, a = args[4 * f:4 * f + 4]
         q.append(([float(a)], [1.0], [float(nrm)], o, p))
-    return float(contr_coulomb_batch([tuple(q)])[0])
+    return float(contr_coulomb_batch([tuple(q)], method)[0])
+
+
+def hgp_vrr(*args):
+    """chgp.vrr (chgp.c:857-890): (A, norma, (la,ma,na), alphaa, B, normb,
+    alphab, C, normc, (lc,mc,nc), alphac, D, normd, alphad, m) -> [a0|c0]^(m)."""
+    if len(args) != 15:
+        raise SystemError("error return without exception set")
+    A, na, pa, aa, B, nb, ab, C, nc, pc, ac, D, nd, ad, m = args
+    q = (([float(aa)], [1.0], [float(na)], A, pa), ([float(ab)], [1.0], [float(nb)], B, (0, 0, 0)),
+         ([float(ac)], [1.0], [float(nc)], C, pc), ([float(ad)], [1.0], [float(nd)], D, (0, 0, 0)))
+    return float(contr_coulomb_batch([q], HGP, 1, [int(m)])[0])
 
 
 def intindex(i, j, k, l):

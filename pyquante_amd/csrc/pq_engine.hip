@@ -512,6 +512,61 @@ int pq_contr_coulomb_batch(int n, const int* nprim_host, const int* poff_host, i
   return PQ_OK;
 }
 
+int pq_hgp_contr_coulomb_batch(int n, const int* nprim_host, const int* poff_host, int nprim_total,
+                               const double* exps_host, const double* coefs_host, const double* norms_host,
+                               const double* origins_host, const int* powers_host, const int* mext_host,
+                               int compat_nr, double* out_host) {
+  if (n < 0 || nprim_total < 0) return fail(PQ_E_ARG, "negative size");
+  if (n == 0) return PQ_OK;
+  if (!nprim_host || !poff_host || !exps_host || !coefs_host || !norms_host || !origins_host || !powers_host ||
+      !out_host)
+    return fail(PQ_E_ARG, "null pointer");
+  for (int q = 0; q < n; ++q) {
+    int pw[4][3];
+    for (int f = 0; f < 4; ++f) {
+      const int np = nprim_host[4 * q + f], po = poff_host[4 * q + f];
+      if (np < 1 || po < 0 || po + np > nprim_total) return fail(PQ_E_ARG, "primitive range out of bounds");
+      for (int d = 0; d < 3; ++d) {
+        pw[f][d] = powers_host[12 * q + 3 * f + d];
+        if (pw[f][d] < 0) return fail(PQ_E_ARG, "negative power");
+      }
+    }
+    const int mext = mext_host ? mext_host[q] : 0;
+    if (mext < 0) return fail(PQ_E_ARG, "negative auxiliary order");
+    // the largest leaves are a+b and c+d: box sizes and total order
+    int boxe = 1, boxf = 1, ltot = mext;
+    for (int d = 0; d < 3; ++d) {
+      boxe *= pw[0][d] + pw[1][d] + 1;
+      boxf *= pw[2][d] + pw[3][d] + 1;
+      ltot += pw[0][d] + pw[1][d] + pw[2][d] + pw[3][d];
+    }
+    if (boxe > 12 || boxf > 12 || ltot > 8)
+      return fail(PQ_E_AM, "HGP leaf beyond the supported size (box <= 12 per side, total order <= 8)");
+  }
+  DevBuf b_np, b_po, b_e, b_c, b_n, b_o, b_p, b_m, b_out;
+  PQ_HIP(hipMalloc(&b_np.p, 4 * n * sizeof(int)));
+  PQ_HIP(hipMalloc(&b_po.p, 4 * n * sizeof(int)));
+  PQ_HIP(hipMalloc(&b_e.p, nprim_total * sizeof(double)));
+  PQ_HIP(hipMalloc(&b_c.p, nprim_total * sizeof(double)));
+  PQ_HIP(hipMalloc(&b_n.p, nprim_total * sizeof(double)));
+  PQ_HIP(hipMalloc(&b_o.p, 12 * n * sizeof(double)));
+  PQ_HIP(hipMalloc(&b_p.p, 12 * n * sizeof(int)));
+  PQ_HIP(hipMalloc(&b_out.p, n * sizeof(double)));
+  if (mext_host) PQ_HIP(hipMalloc(&b_m.p, n * sizeof(int)));
+  PQ_HIP(hipMemcpy(b_np.p, nprim_host, 4 * n * sizeof(int), hipMemcpyHostToDevice));
+  PQ_HIP(hipMemcpy(b_po.p, poff_host, 4 * n * sizeof(int), hipMemcpyHostToDevice));
+  PQ_HIP(hipMemcpy(b_e.p, exps_host, nprim_total * sizeof(double), hipMemcpyHostToDevice));
+  PQ_HIP(hipMemcpy(b_c.p, coefs_host, nprim_total * sizeof(double), hipMemcpyHostToDevice));
+  PQ_HIP(hipMemcpy(b_n.p, norms_host, nprim_total * sizeof(double), hipMemcpyHostToDevice));
+  PQ_HIP(hipMemcpy(b_o.p, origins_host, 12 * n * sizeof(double), hipMemcpyHostToDevice));
+  PQ_HIP(hipMemcpy(b_p.p, powers_host, 12 * n * sizeof(int), hipMemcpyHostToDevice));
+  if (mext_host) PQ_HIP(hipMemcpy(b_m.p, mext_host, n * sizeof(int), hipMemcpyHostToDevice));
+  PQ_HIP(launch_hgp_contr(n, (int*)b_np.p, (int*)b_po.p, (double*)b_e.p, (double*)b_c.p, (double*)b_n.p,
+                          (double*)b_o.p, (int*)b_p.p, (int*)b_m.p, compat_nr, (double*)b_out.p, 0));
+  PQ_HIP(hipMemcpy(out_host, b_out.p, n * sizeof(double), hipMemcpyDeviceToHost));
+  return PQ_OK;
+}
+
 int pq_engine_create(int nshell, const int* shell_l, const int* shell_nprim, const int* shell_poff,
                      const int* shell_atom, const double* exps, const double* coefs, int natom,
                      const double* atom_xyz, const double* atom_z, pq_engine** out) {

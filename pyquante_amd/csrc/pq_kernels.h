@@ -51,6 +51,11 @@ hipError_t launch_onee(int la, int lb, const EngineView& ev, const int* pair_ids
                        double* V, hipStream_t st);
 
 // scalar-compatibility batches
+// HGP contracted ERIs per function quartet (batch layout of launch_contr_eri);
+// mext (nullable) selects the auxiliary order of a primitive VRR leaf
+hipError_t launch_hgp_contr(int n, const int* nprim, const int* poff, const double* exps, const double* coefs,
+                            const double* norms, const double* origins, const int* powers, const int* mext,
+                            int compat_nr, double* out, hipStream_t st);
 hipError_t launch_boys(int n, const double* m, const double* x, double eps, double* out, hipStream_t st);
 hipError_t launch_rys_roots(int n, int nr, const double* X, double* u, double* w, hipStream_t st);
 hipError_t launch_contr_eri(int n, const int* nprim, const int* poff, const double* exps,

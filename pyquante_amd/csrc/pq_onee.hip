@@ -388,12 +388,8 @@ hipError_t launch_grad_finish(const double* gacc, int ncopies, int natom, const 
 // reference's Fgamma does (cints.c:340-411); eps is the relative stopping
 // tolerance (3e-7 reproduces the reference, 1e-16 gives full precision).
 // ---------------------------------------------------------------------
-__global__ void k_boys(int n, const double* m_, const double* x_, double eps, int itmax, double* out) {
-  const int t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= n) return;
-  const double m = m_[t];
-  double x = x_[t];
-  if (fabs(x) < 1e-8) x = 1e-8;
+__device__ double fgamma_dev(double m, double x, double eps, int itmax) {
+  if (x < 0.0) x = 0.0;
   if (eps < 1e-10 && (x < 50.0 || m == floor(m))) {
     // full-precision mode: F_m(x) = e^-x sum_k (2x)^k / prod_{j<=k}(2m+2j+1)
     // below x = 50; upward recursion from F_0 = sqrt(pi/x) erf(sqrt x)/2 above
@@ -405,15 +401,15 @@ __global__ void k_boys(int n, const double* m_, const double* x_, double eps, in
         sum += term;
         if (term < sum * 1e-17) break;
       }
-      out[t] = exp(-x) * sum;
+      return exp(-x) * sum;
     } else {
       const double ex = exp(-x);
       double f = 0.5 * sqrt(M_PI / x) * erf(sqrt(x));
       for (int k = 0; k < (int)m; ++k) f = ((2.0 * k + 1.0) * f - ex) / (2.0 * x);
-      out[t] = f;
+      return f;
     }
-    return;
   }
+  if (x < 1e-8) x = 1e-8;  // the reference's clamp (cints.c:340-411)
   const double a = m + 0.5;
   const double gln = lgamma(a);
   double gam;
@@ -443,7 +439,13 @@ __global__ void k_boys(int n, const double* m_, const double* x_, double eps, in
     }
     gam = exp(gln) * (1.0 - exp(-x + a * log(x) - gln) * h);
   }
-  out[t] = 0.5 * pow(x, -m - 0.5) * gam;
+  return 0.5 * pow(x, -m - 0.5) * gam;
+}
+
+__global__ void k_boys(int n, const double* m_, const double* x_, double eps, int itmax, double* out) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  out[t] = fgamma_dev(m_[t], x_[t], eps, itmax);
 }
 
 hipError_t launch_boys(int n, const double* m, const double* x, double eps, double* out, hipStream_t st) {
@@ -560,6 +562,177 @@ __global__ void k_contr_eri(int n, const int* __restrict__ nprim, const int* __r
                  norms[po[1] + j] * norms[po[2] + k] * norms[po[3] + l] * pre * sum;
         }
   out[t] = val;
+}
+
+// ---------------------------------------------------------------------
+// Head-Gordon-Pople ERIs (chgp.c:78-433), one contracted function quartet
+// per lane.  (ab|cd) is expanded by the horizontal recurrence
+// (a, b+1_i| = (a+1_i, b| + (A-B)_i (a, b| in closed form,
+//   (ab|cd) = sum_{k<=b, l<=d} prod_i C(b_i,k_i) AB_i^(b_i-k_i) C(d_i,l_i) CD_i^(d_i-l_i) [a+k,0|c+l,0],
+// and every leaf [e0|f0] is a contracted Obara-Saika vertical recurrence
+// over its own box e' <= e, f' <= f with the Boys values of order |e|+|f|
+// (chgp.c:244-419): the reference starts the downward Boys recursion at each
+// leaf's own total order, and so does this kernel, which keeps its values
+// equal to the reference's also with the reference's EPS = 3e-7 Boys
+// function (eps).  Leaves are limited to |e|, |f| <= 4 (box <= 12 entries)
+// and |e| + |f| + mext <= 8.
+// mext (optional, per quartet) asks for the auxiliary order m of a single
+// primitive leaf [a0|c0]^(m) instead (chgp.c:857-890, vrr).
+// ---------------------------------------------------------------------
+constexpr int kHgpBox = 12, kHgpM = 9;
+
+__device__ __forceinline__ double binom(int n, int k) {
+  double c = 1.0;
+  for (int i = 1; i <= k; ++i) c = c * (n - k + i) / i;
+  return c;
+}
+
+__device__ double hgp_leaf(const int e[3], const int f[3], int mext, const int np[4], const int po[4],
+                           const double o[4][3], const double* __restrict__ exps,
+                           const double* __restrict__ coefs, const double* __restrict__ norms, double eps,
+                           int itmax, double (&V)[kHgpM][kHgpBox][kHgpBox]) {
+  const int ne[3] = {e[0] + 1, e[1] + 1, e[2] + 1}, nf[3] = {f[0] + 1, f[1] + 1, f[2] + 1};
+  const int le = e[0] + e[1] + e[2], lf = f[0] + f[1] + f[2];
+  const int mtot = le + lf + mext;
+  const int nE = ne[0] * ne[1] * ne[2], nF = nf[0] * nf[1] * nf[2];
+  const int se[3] = {ne[1] * ne[2], ne[2], 1}, sf[3] = {nf[1] * nf[2], nf[2], 1};
+  double acc = 0.0;
+  for (int i = 0; i < np[0]; ++i)
+    for (int j = 0; j < np[1]; ++j)
+      for (int k = 0; k < np[2]; ++k)
+        for (int l = 0; l < np[3]; ++l) {
+          const double aa = exps[po[0] + i], ab = exps[po[1] + j], ac = exps[po[2] + k], ad = exps[po[3] + l];
+          const double zeta = aa + ab, eta = ac + ad, ze = zeta + eta, rho = zeta * eta / ze;
+          double P[3], Q[3], W[3], rab2 = 0.0, rcd2 = 0.0, rpq2 = 0.0;
+          for (int d = 0; d < 3; ++d) {
+            P[d] = (aa * o[0][d] + ab * o[1][d]) / zeta;
+            Q[d] = (ac * o[2][d] + ad * o[3][d]) / eta;
+            W[d] = (zeta * P[d] + eta * Q[d]) / ze;
+            rab2 += (o[0][d] - o[1][d]) * (o[0][d] - o[1][d]);
+            rcd2 += (o[2][d] - o[3][d]) * (o[2][d] - o[3][d]);
+            rpq2 += (P[d] - Q[d]) * (P[d] - Q[d]);
+          }
+          const double T = rho * rpq2;
+          // K = sqrt(2) pi^(5/4) / zeta exp(-ab/zeta |AB|^2) (chgp.c:273-277)
+          const double kpi = 5.914967172795613;  // sqrt(2) pi^(5/4)
+          const double Kab = kpi / zeta * exp(-aa * ab / zeta * rab2);
+          const double Kcd = kpi / eta * exp(-ac * ad / eta * rcd2);
+          const double pref = norms[po[0] + i] * norms[po[1] + j] * norms[po[2] + k] * norms[po[3] + l] * Kab *
+                              Kcd / sqrt(ze);
+          // Boys F_m(T), m = 0..mtot: F_mtot, then downward (chgp.c:287-291)
+          double F[kHgpM];
+          F[mtot] = fgamma_dev((double)mtot, T, eps, itmax);
+          const double ex = exp(-T);
+          for (int m = mtot - 1; m >= 0; --m) F[m] = (2.0 * T * F[m + 1] + ex) / (2.0 * m + 1.0);
+          for (int m = 0; m <= mtot; ++m) V[m][0][0] = pref * F[m];
+          // bra: [e'0|00]^m for e' in the box, lexicographic order; the step
+          // goes along the first nonzero direction of e'
+          for (int ex_ = 0; ex_ < ne[0]; ++ex_)
+            for (int ey = 0; ey < ne[1]; ++ey)
+              for (int ez = 0; ez < ne[2]; ++ez) {
+                const int ep[3] = {ex_, ey, ez};
+                const int tot = ex_ + ey + ez;
+                if (tot == 0) continue;
+                const int dir = ex_ > 0 ? 0 : (ey > 0 ? 1 : 2);
+                const int ie = ex_ * se[0] + ey * se[1] + ez;
+                const int i1 = ie - se[dir], i2 = ie - 2 * se[dir];
+                const double PA = P[dir] - o[0][dir], WP = W[dir] - P[dir];
+                const double c2 = (ep[dir] - 1) / (2.0 * zeta);
+                for (int m = 0; m <= mtot - tot; ++m) {
+                  double v = PA * V[m][i1][0] + WP * V[m + 1][i1][0];
+                  if (ep[dir] >= 2) v += c2 * (V[m][i2][0] - rho / zeta * V[m + 1][i2][0]);
+                  V[m][ie][0] = v;
+                }
+              }
+          // ket: [e'0|f'0]^m, f' lexicographic outside, every e' inside
+          for (int fx = 0; fx < nf[0]; ++fx)
+            for (int fy = 0; fy < nf[1]; ++fy)
+              for (int fz = 0; fz < nf[2]; ++fz) {
+                const int fp[3] = {fx, fy, fz};
+                const int ftot = fx + fy + fz;
+                if (ftot == 0) continue;
+                const int dir = fx > 0 ? 0 : (fy > 0 ? 1 : 2);
+                const int jf = fx * sf[0] + fy * sf[1] + fz;
+                const int j1 = jf - sf[dir], j2 = jf - 2 * sf[dir];
+                const double QC = Q[dir] - o[2][dir], WQ = W[dir] - Q[dir];
+                const double c2 = (fp[dir] - 1) / (2.0 * eta), c3 = 1.0 / (2.0 * ze);
+                for (int ex_ = 0; ex_ < ne[0]; ++ex_)
+                  for (int ey = 0; ey < ne[1]; ++ey)
+                    for (int ez = 0; ez < ne[2]; ++ez) {
+                      const int ep[3] = {ex_, ey, ez};
+                      const int ie = ex_ * se[0] + ey * se[1] + ez;
+                      for (int m = 0; m <= mtot - ftot - (ex_ + ey + ez); ++m) {
+                        double v = QC * V[m][ie][j1] + WQ * V[m + 1][ie][j1];
+                        if (fp[dir] >= 2) v += c2 * (V[m][ie][j2] - rho / eta * V[m + 1][ie][j2]);
+                        if (ep[dir] >= 1) v += ep[dir] * c3 * V[m + 1][ie - se[dir]][j1];
+                        V[m][ie][jf] = v;
+                      }
+                    }
+              }
+          acc += coefs[po[0] + i] * coefs[po[1] + j] * coefs[po[2] + k] * coefs[po[3] + l] *
+                 V[mext][nE - 1][nF - 1];
+        }
+  return acc;
+}
+
+__global__ void k_hgp_contr(int n, const int* __restrict__ nprim, const int* __restrict__ poff,
+                            const double* __restrict__ exps, const double* __restrict__ coefs,
+                            const double* __restrict__ norms, const double* __restrict__ org,
+                            const int* __restrict__ pw, const int* __restrict__ mext_, double eps, int itmax,
+                            double* out) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  int np[4], po[4], p[4][3];
+  double o[4][3];
+  for (int f = 0; f < 4; ++f) {
+    np[f] = nprim[4 * t + f];
+    po[f] = poff[4 * t + f];
+    for (int d = 0; d < 3; ++d) {
+      p[f][d] = pw[12 * t + 3 * f + d];
+      o[f][d] = org[12 * t + 3 * f + d];
+    }
+  }
+  const int mext = mext_ ? mext_[t] : 0;
+  double V[kHgpM][kHgpBox][kHgpBox];
+  double val = 0.0;
+  // leaves a+k, c+l of the horizontal expansion
+  for (int kx = 0; kx <= p[1][0]; ++kx)
+    for (int ky = 0; ky <= p[1][1]; ++ky)
+      for (int kz = 0; kz <= p[1][2]; ++kz) {
+        const int kk[3] = {kx, ky, kz};
+        double hb = 1.0;
+        int e[3];
+        for (int d = 0; d < 3; ++d) {
+          hb *= binom(p[1][d], kk[d]);
+          for (int r = kk[d]; r < p[1][d]; ++r) hb *= o[0][d] - o[1][d];
+          e[d] = p[0][d] + kk[d];
+        }
+        for (int lx = 0; lx <= p[3][0]; ++lx)
+          for (int ly = 0; ly <= p[3][1]; ++ly)
+            for (int lz = 0; lz <= p[3][2]; ++lz) {
+              const int ll[3] = {lx, ly, lz};
+              double hk = 1.0;
+              int f[3];
+              for (int d = 0; d < 3; ++d) {
+                hk *= binom(p[3][d], ll[d]);
+                for (int r = ll[d]; r < p[3][d]; ++r) hk *= o[2][d] - o[3][d];
+                f[d] = p[2][d] + ll[d];
+              }
+              val += hb * hk * hgp_leaf(e, f, mext, np, po, o, exps, coefs, norms, eps, itmax, V);
+            }
+      }
+  out[t] = val;
+}
+
+hipError_t launch_hgp_contr(int n, const int* nprim, const int* poff, const double* exps, const double* coefs,
+                            const double* norms, const double* origins, const int* powers, const int* mext,
+                            int compat_nr, double* out, hipStream_t st) {
+  if (n <= 0) return hipSuccess;
+  const double eps = compat_nr ? 3.0e-7 : 1e-17;
+  const int itmax = compat_nr ? 100 : 2000;
+  hipLaunchKernelGGL(k_hgp_contr, dim3((n + 63) / 64), dim3(64), 0, st, n, nprim, poff, exps, coefs, norms,
+                     origins, powers, mext, eps, itmax, out);
+  return hipGetLastError();
 }
 
 hipError_t launch_contr_eri(int n, const int* nprim, const int* poff, const double* exps, const double* coefs,
