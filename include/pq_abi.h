@@ -156,6 +156,12 @@ int pq_engine_class_stats(const pq_engine *eng, int c, int l[4], uint64_t counts
  * npair = nbf(nbf+1)/2), FP64, unscreened. */
 int pq_get2ints(pq_engine *eng, double *out_dev, uint64_t len, void *stream);
 
+/* pq_get2ints by Head-Gordon-Pople recurrences: the reference's get2ints with
+ * CGBF.coulomb bound to chgp (CGBF.py:23-25, Ints.py:146-167), one function
+ * quartet per lane as chgp evaluates them (chgp.c:78-419).  compat_nr = 1
+ * uses chgp's EPS = 3e-7 Boys function, 0 the full-precision one. */
+int pq_get2ints_hgp(pq_engine *eng, double *out_dev, uint64_t len, int compat_nr, void *stream);
+
 /* Ints.getJ / getK / get2JmK (PyQuante/Ints.py:195-238) in direct mode:
  * J_ij = sum_kl (ij|kl) D_kl and K_ij = sum_kl (ik|jl) D_kl from integrals
  * computed on the fly over the screened work lists.  D_dev is a symmetric

@@ -25,6 +25,11 @@ from .engine import Engine, digest_stored
 sorted = True
 # Schwarz cutoff of the direct Fock build behind getJ/getK/get2JmK
 schwarz_cut = 1e-12
+# integral backend of Integrals/get2ints, as the reference switches CGBF.coulomb
+# between crys and chgp (CGBF.py:23-25): "rys" or "hgp"
+method = "rys"
+# the HGP backend keeps chgp's Boys tolerance (EPS = 3e-7) and so its values
+hgp_compat_boys = True
 
 
 class _Nucleus(object):
@@ -47,10 +52,19 @@ def _atoms_for(bfs, atoms):
 
 
 class Integrals(object):
-    """Two-electron integrals of one basis, held by the GPU engine."""
+    """Two-electron integrals of one basis, held by the GPU engine.
 
-    def __init__(self, bfs, atoms=None, engine=None):
+    method "rys" (default, the reference's crys backend) computes J/K
+    directly from screened shell quartets; method "hgp" (the reference's chgp
+    backend, CGBF.py:23-25) materialises the flat array with the GPU HGP
+    kernel, as the reference's get2ints does, and digests it for J/K.
+    hgp_compat_boys keeps chgp's EPS = 3e-7 Boys function (its values)."""
+
+    def __init__(self, bfs, atoms=None, engine=None, method=None):
         self.bfs = bfs
+        self.method = method or globals()["method"]
+        if self.method not in ("rys", "hgp"):
+            raise ValueError("method must be 'rys' or 'hgp'")
         self.engine = engine if engine is not None else Engine(bfs, _atoms_for(bfs, atoms))
         self.nbf = self.engine.nbf
         npair = self.nbf * (self.nbf + 1) // 2
@@ -65,7 +79,7 @@ class Integrals(object):
     def device_array(self):
         """Flat unique-ERI array (reference layout) as a device tensor."""
         if self._dev is None:
-            self._dev = self.engine.get2ints()
+            self._dev = self.engine.get2ints(self.method, 1 if hgp_compat_boys else 0)
         return self._dev
 
     def host_array(self):
@@ -84,6 +98,8 @@ class Integrals(object):
         return a if dtype is None else a.astype(dtype)
 
     def jk(self, D, want_j=True, want_k=True, precision=_abi.PQ_PREC_FP64):
+        if self.method == "hgp":
+            return digest_stored(self.device_array(), D, want_j, want_k)
         if self._built_cut != schwarz_cut or self.engine.fp32cut:
             self.engine.build(schwarz_cut, 0.0)
             self._built_cut = schwarz_cut
@@ -124,9 +140,9 @@ def getV(bfs, atoms):
     return eng.get1ints()[2].cpu().numpy()
 
 
-def get2ints(bfs, atoms=None):
+def get2ints(bfs, atoms=None, method=None):
     """Ints.get2ints (Ints.py:146-167): the unique ERIs, reference layout."""
-    return Integrals(bfs, atoms)
+    return Integrals(bfs, atoms, method=method)
 
 
 def getints(bfs, atoms):

@@ -309,6 +309,14 @@ __host__ __device__ inline uint64_t ijkl2intindex(int64_t i, int64_t j, int64_t 
 }
 
 // Wavefront-wide sum (64 lanes) by butterfly exchange.
+// r with r(r+1)/2 <= t < (r+1)(r+2)/2: row of triangular index t.
+__device__ __forceinline__ int64_t tri_root(uint64_t t) {
+  int64_t r = (int64_t)((sqrt(8.0 * (double)t + 1.0) - 1.0) * 0.5);
+  while ((uint64_t)(r + 1) * (uint64_t)(r + 2) / 2 <= t) ++r;
+  while (r > 0 && (uint64_t)r * (uint64_t)(r + 1) / 2 > t) --r;
+  return r;
+}
+
 // Load of read-only data at a wavefront-uniform address through the constant
 // address space, so that the compiler can use a scalar load (no vector memory
 // instruction per lane).  Only for memory no kernel writes while it runs.

@@ -138,6 +138,25 @@ struct pq_engine {
 
   EngineView view() const { return EngineView{d_pairs, d_prims, d_prim_a, nbf}; }
 
+  // per-function primitive data for the HGP path (built on first use, dropped
+  // with the geometry)
+  int* d_fnp = nullptr;
+  int* d_foff = nullptr;
+  int* d_fpw = nullptr;
+  double* d_forg = nullptr;
+  double* d_fexps = nullptr;
+  double* d_fcoefs = nullptr;
+  double* d_fnorms = nullptr;
+  void free_functions() {
+    dev_free(d_fnp);
+    dev_free(d_foff);
+    dev_free(d_fpw);
+    dev_free(d_forg);
+    dev_free(d_fexps);
+    dev_free(d_fcoefs);
+    dev_free(d_fnorms);
+  }
+
   void free_work(std::vector<ClassWork>& w) {
     for (auto& c : w) {
       dev_free(c.d_t64);
@@ -168,6 +187,7 @@ struct pq_engine {
       (void)hipEventDestroy(fork_ev);
     }
     free_pairs();
+    free_functions();
     dev_free(d_atom_xyz);
     dev_free(d_atom_z);
     dev_free(d_Jacc);
@@ -629,6 +649,7 @@ int pq_engine_set_geometry(pq_engine* eng, const double* atom_xyz_host) {
   eng->atom_xyz.assign(atom_xyz_host, atom_xyz_host + 3 * eng->natom);
   PQ_HIP(hipMemcpy(eng->d_atom_xyz, atom_xyz_host, 3 * eng->natom * sizeof(double), hipMemcpyHostToDevice));
   eng->free_pairs();
+  eng->free_functions();
   return PQ_OK;
 }
 
@@ -773,6 +794,61 @@ int pq_get2ints(pq_engine* eng, double* out_dev, uint64_t len, void* stream) {
                             eng->view(), c.d_t64, (int)c.t64.size(), eng->d_sorted[c.Y], nullptr, nullptr,
                             nullptr, out_dev, nullptr, st));
   }
+  return PQ_OK;
+}
+
+namespace {
+// Per-function primitive arrays for the HGP kernels: exponents, raw
+// coefficients and the full primitive norm for the function's own powers
+// (primitive-gto.c:76-83), with the function's centre and powers.
+int build_functions(pq_engine* e) {
+  if (e->d_fnp) return PQ_OK;
+  std::vector<int> fnp, foff, fpw;
+  std::vector<double> forg, fexps, fcoefs, fnorms;
+  for (int s = 0; s < e->nshell; ++s) {
+    const int L = e->sh_l[s];
+    for (int c = 0; c < ncart(L); ++c) {
+      fnp.push_back(e->sh_np[s]);
+      foff.push_back((int)fexps.size());
+      int df = 1;
+      for (int d = 0; d < 3; ++d) {
+        fpw.push_back(cart_pow(L, c, d));
+        forg.push_back(e->atom_xyz[3 * e->sh_atom[s] + d]);
+        df *= dfact(2 * cart_pow(L, c, d) - 1);
+      }
+      for (int i = 0; i < e->sh_np[s]; ++i) {
+        const double a = e->exps[e->sh_off[s] + i];
+        fexps.push_back(a);
+        fcoefs.push_back(e->coefs[e->sh_off[s] + i]);
+        fnorms.push_back(nrad(a, L) / std::sqrt((double)df));
+      }
+    }
+  }
+  auto up = [](auto** d, const auto& v) -> hipError_t {
+    hipError_t r = hipMalloc((void**)d, std::max<size_t>(1, v.size()) * sizeof(v[0]));
+    if (r == hipSuccess && !v.empty()) r = hipMemcpy(*d, v.data(), v.size() * sizeof(v[0]), hipMemcpyHostToDevice);
+    return r;
+  };
+  hipError_t r;
+  if ((r = up(&e->d_fnp, fnp)) != hipSuccess || (r = up(&e->d_foff, foff)) != hipSuccess ||
+      (r = up(&e->d_fpw, fpw)) != hipSuccess || (r = up(&e->d_forg, forg)) != hipSuccess ||
+      (r = up(&e->d_fexps, fexps)) != hipSuccess || (r = up(&e->d_fcoefs, fcoefs)) != hipSuccess ||
+      (r = up(&e->d_fnorms, fnorms)) != hipSuccess) {
+    e->free_functions();
+    return fail(PQ_E_HIP, hipGetErrorString(r));
+  }
+  return PQ_OK;
+}
+}  // namespace
+
+int pq_get2ints_hgp(pq_engine* eng, double* out_dev, uint64_t len, int compat_nr, void* stream) {
+  if (!eng || !out_dev) return fail(PQ_E_ARG, "null argument");
+  const uint64_t n = eng->nbf, npair = n * (n + 1) / 2;
+  if (len != npair * (npair + 1) / 2) return fail(PQ_E_ARG, "output length must be npair*(npair+1)/2");
+  int rc = build_functions(eng);
+  if (rc) return rc;
+  PQ_HIP(launch_hgp_store(len, eng->d_fnp, eng->d_foff, eng->d_forg, eng->d_fpw, eng->d_fexps, eng->d_fcoefs,
+                          eng->d_fnorms, compat_nr, out_dev, (hipStream_t)stream));
   return PQ_OK;
 }
 

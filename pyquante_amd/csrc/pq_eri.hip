@@ -86,12 +86,6 @@ __global__ void k_symmetrize(int n, const double* __restrict__ Jacc, const doubl
 // same accumulation rule as the direct path applies with the function-level
 // degeneracy factor.
 // ---------------------------------------------------------------------
-__device__ __forceinline__ int64_t tri_root(uint64_t t) {
-  int64_t r = (int64_t)((sqrt(8.0 * (double)t + 1.0) - 1.0) * 0.5);
-  while ((uint64_t)(r + 1) * (uint64_t)(r + 2) / 2 <= t) ++r;
-  while (r > 0 && (uint64_t)r * (uint64_t)(r + 1) / 2 > t) --r;
-  return r;
-}
 
 __global__ void k_digest_stored(int n, const double* __restrict__ ints, uint64_t len, const double* __restrict__ D,
                                 double* Jacc, double* Kacc) {

@@ -56,6 +56,10 @@ hipError_t launch_onee(int la, int lb, const EngineView& ev, const int* pair_ids
 hipError_t launch_hgp_contr(int n, const int* nprim, const int* poff, const double* exps, const double* coefs,
                             const double* norms, const double* origins, const int* powers, const int* mext,
                             int compat_nr, double* out, hipStream_t st);
+// HGP for every unique function quartet of a basis, reference flat layout
+hipError_t launch_hgp_store(uint64_t len, const int* fnp, const int* foff, const double* forg, const int* fpw,
+                            const double* fexps, const double* fcoefs, const double* fnorms, int compat_nr,
+                            double* out, hipStream_t st);
 hipError_t launch_boys(int n, const double* m, const double* x, double eps, double* out, hipStream_t st);
 hipError_t launch_rys_roots(int n, int nr, const double* X, double* u, double* w, hipStream_t st);
 hipError_t launch_contr_eri(int n, const int* nprim, const int* poff, const double* exps,

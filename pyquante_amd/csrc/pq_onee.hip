@@ -2,6 +2,8 @@
 // kernels behind the drop-in cints/crys/chgp functions.
 #include <math.h>
 
+#include <algorithm>
+
 #include "pq_device.h"
 #include "pq_kernels.h"
 
@@ -675,27 +677,12 @@ __device__ double hgp_leaf(const int e[3], const int f[3], int mext, const int n
   return acc;
 }
 
-__global__ void k_hgp_contr(int n, const int* __restrict__ nprim, const int* __restrict__ poff,
-                            const double* __restrict__ exps, const double* __restrict__ coefs,
-                            const double* __restrict__ norms, const double* __restrict__ org,
-                            const int* __restrict__ pw, const int* __restrict__ mext_, double eps, int itmax,
-                            double* out) {
-  const int t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= n) return;
-  int np[4], po[4], p[4][3];
-  double o[4][3];
-  for (int f = 0; f < 4; ++f) {
-    np[f] = nprim[4 * t + f];
-    po[f] = poff[4 * t + f];
-    for (int d = 0; d < 3; ++d) {
-      p[f][d] = pw[12 * t + 3 * f + d];
-      o[f][d] = org[12 * t + 3 * f + d];
-    }
-  }
-  const int mext = mext_ ? mext_[t] : 0;
-  double V[kHgpM][kHgpBox][kHgpBox];
+// One contracted function quartet: leaves a+k, c+l of the horizontal expansion.
+__device__ double hgp_quartet(const int np[4], const int po[4], const double o[4][3], const int p[4][3], int mext,
+                              const double* __restrict__ exps, const double* __restrict__ coefs,
+                              const double* __restrict__ norms, double eps, int itmax,
+                              double (&V)[kHgpM][kHgpBox][kHgpBox]) {
   double val = 0.0;
-  // leaves a+k, c+l of the horizontal expansion
   for (int kx = 0; kx <= p[1][0]; ++kx)
     for (int ky = 0; ky <= p[1][1]; ++ky)
       for (int kz = 0; kz <= p[1][2]; ++kz) {
@@ -721,7 +708,69 @@ __global__ void k_hgp_contr(int n, const int* __restrict__ nprim, const int* __r
               val += hb * hk * hgp_leaf(e, f, mext, np, po, o, exps, coefs, norms, eps, itmax, V);
             }
       }
-  out[t] = val;
+  return val;
+}
+
+__global__ void k_hgp_contr(int n, const int* __restrict__ nprim, const int* __restrict__ poff,
+                            const double* __restrict__ exps, const double* __restrict__ coefs,
+                            const double* __restrict__ norms, const double* __restrict__ org,
+                            const int* __restrict__ pw, const int* __restrict__ mext_, double eps, int itmax,
+                            double* out) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  int np[4], po[4], p[4][3];
+  double o[4][3];
+  for (int f = 0; f < 4; ++f) {
+    np[f] = nprim[4 * t + f];
+    po[f] = poff[4 * t + f];
+    for (int d = 0; d < 3; ++d) {
+      p[f][d] = pw[12 * t + 3 * f + d];
+      o[f][d] = org[12 * t + 3 * f + d];
+    }
+  }
+  double V[kHgpM][kHgpBox][kHgpBox];
+  out[t] = hgp_quartet(np, po, o, p, mext_ ? mext_[t] : 0, exps, coefs, norms, eps, itmax, V);
+}
+
+// Every unique function quartet of a basis in the reference's flat layout
+// (index ij(ij+1)/2 + kl, Ints.py:146-167), by HGP.  Function f has fnp[f]
+// primitives at foff[f] in fexps/fcoefs/fnorms (norms for its own powers).
+__global__ void k_hgp_store(uint64_t len, const int* __restrict__ fnp, const int* __restrict__ foff,
+                            const double* __restrict__ forg, const int* __restrict__ fpw,
+                            const double* __restrict__ fexps, const double* __restrict__ fcoefs,
+                            const double* __restrict__ fnorms, double eps, int itmax, double* out) {
+  uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  double V[kHgpM][kHgpBox][kHgpBox];
+  for (; t < len; t += stride) {
+    const int64_t ij = tri_root(t), kl = (int64_t)(t - (uint64_t)ij * (ij + 1) / 2);
+    const int64_t i = tri_root(ij), j = ij - i * (i + 1) / 2;
+    const int64_t k = tri_root(kl), l = kl - k * (k + 1) / 2;
+    const int64_t fs[4] = {i, j, k, l};
+    int np[4], po[4], p[4][3];
+    double o[4][3];
+    for (int f = 0; f < 4; ++f) {
+      np[f] = fnp[fs[f]];
+      po[f] = foff[fs[f]];
+      for (int d = 0; d < 3; ++d) {
+        p[f][d] = fpw[3 * fs[f] + d];
+        o[f][d] = forg[3 * fs[f] + d];
+      }
+    }
+    out[t] = hgp_quartet(np, po, o, p, 0, fexps, fcoefs, fnorms, eps, itmax, V);
+  }
+}
+
+hipError_t launch_hgp_store(uint64_t len, const int* fnp, const int* foff, const double* forg, const int* fpw,
+                            const double* fexps, const double* fcoefs, const double* fnorms, int compat_nr,
+                            double* out, hipStream_t st) {
+  if (len == 0) return hipSuccess;
+  const double eps = compat_nr ? 3.0e-7 : 1e-17;
+  const int itmax = compat_nr ? 100 : 2000;
+  const uint64_t blocks = std::min<uint64_t>((len + 63) / 64, 1u << 16);
+  hipLaunchKernelGGL(k_hgp_store, dim3((unsigned)blocks), dim3(64), 0, st, len, fnp, foff, forg, fpw, fexps,
+                     fcoefs, fnorms, eps, itmax, out);
+  return hipGetLastError();
 }
 
 hipError_t launch_hgp_contr(int n, const int* nprim, const int* poff, const double* exps, const double* coefs,

@@ -206,13 +206,21 @@ class Engine(object):
                             ctypes.c_void_p(g.data_ptr()), _stream()))
         return g
 
-    def get2ints(self):
-        """The reference's flat unique-ERI array as a device tensor (pq_get2ints)."""
+    def get2ints(self, method="rys", compat_nr=0):
+        """The reference's flat unique-ERI array as a device tensor:
+        method "rys" (pq_get2ints, shell-quartet Rys kernels) or "hgp"
+        (pq_get2ints_hgp; compat_nr = 1 keeps chgp's Boys tolerance)."""
         n = self.nbf
         npair = n * (n + 1) // 2
         length = npair * (npair + 1) // 2
         out = torch.empty(length, dtype=torch.float64, device=self.device)
-        check(lib.pq_get2ints(self.handle, ctypes.c_void_p(out.data_ptr()), ctypes.c_uint64(length), _stream()))
+        if method == "rys":
+            check(lib.pq_get2ints(self.handle, ctypes.c_void_p(out.data_ptr()), ctypes.c_uint64(length), _stream()))
+        elif method == "hgp":
+            check(lib.pq_get2ints_hgp(self.handle, ctypes.c_void_p(out.data_ptr()), ctypes.c_uint64(length),
+                                      int(compat_nr), _stream()))
+        else:
+            raise ValueError("method must be 'rys' or 'hgp'")
         return out
 
     def get1ints(self):

@@ -87,7 +87,11 @@ class DIIS(object):
 
 
 def rhf(atoms, **opts):
-    """rhf(atoms, **opts) -> (energy, orbe, orbs) (hartree_fock.py:66-149)."""
+    """rhf(atoms, **opts) -> (energy, orbe, orbs) (hartree_fock.py:66-149).
+
+    Extra options: method "rys" (direct J/K) or "hgp" (stored HGP integrals,
+    as with the reference's chgp backend); precision PQ_PREC_MIXED (direct
+    Rys with FP32 quartets below fp32_cut)."""
     ConvCriteria = opts.get("ConvCriteria", 1e-4)
     MaxIter = opts.get("MaxIter", 20)
     DoAveraging = opts.get("DoAveraging", False)
@@ -109,7 +113,7 @@ def rhf(atoms, **opts):
         dev = engine.device
         Sd, Td, Vd = engine.get1ints()
         S, h = Sd, Td + Vd
-        Ints = Integrals(bfs, atoms, engine=engine)
+        Ints = Integrals(bfs, atoms, engine=engine, method=opts.get("method", None))
     S = _t(S, dev)
     h = _t(h, dev)
     X = chol_orth(S)

@@ -127,3 +127,40 @@ def test_chgp_module_and_vrr_closed_form():
     z = O.prim_eri(O.HGP, A, 0.7, (1, 0, 0), aa, B, 1.1, (0, 0, 0), ab, C, 0.9, (0, 1, 1), ac, D, 1.3,
                    (0, 0, 0), ad, compat_nr=1)
     assert abs(w - z) < 1e-12 * abs(z)
+
+
+def test_get2ints_hgp_h2o_631ss(golden):
+    from pyquante_amd.basis import BasisSet
+    from pyquante_amd.engine import Engine
+    from pyquante_amd.molecule import Molecule
+    r = 1 / 0.52918
+    mol = Molecule("h2o", [(8, (0, 0, 0)), (1, (r, 0, 0)), (1, (0, r, 0))])
+    eng = Engine(BasisSet(mol, "6-31g**"), mol)
+    g = golden("h2o_631ss")
+    exact = eng.get2ints("hgp", 0).cpu().numpy()
+    assert len(exact) == 52975
+    assert np.max(np.abs(exact - g["crys"])) < 1e-12
+    compat = eng.get2ints("hgp", 1).cpu().numpy()
+    assert np.max(np.abs(compat[g["index5"]] - g["chgp5"])) < 1e-12
+
+
+def test_rhf_hgp_backend_matches_rys():
+    from pyquante_amd import Ints
+    from pyquante_amd.hartree_fock import rhf
+    from pyquante_amd.molecule import water_cluster
+    mol = water_cluster((2, 1, 1), seed=4)
+    e_rys, _, _ = rhf(mol, basis_data="6-31g**", ConvCriteria=1e-10, MaxIter=60, DoAveraging=True)
+    old = Ints.hgp_compat_boys
+    try:
+        Ints.hgp_compat_boys = False
+        e_hgp, _, _ = rhf(mol, basis_data="6-31g**", ConvCriteria=1e-10, MaxIter=60, DoAveraging=True,
+                          method="hgp")
+        Ints.hgp_compat_boys = True
+        e_hgp_nr, _, _ = rhf(mol, basis_data="6-31g**", ConvCriteria=1e-10, MaxIter=60, DoAveraging=True,
+                             method="hgp")
+    finally:
+        Ints.hgp_compat_boys = old
+    print("rhf water dimer: rys %.12f hgp %.12f hgp(NR Boys) %.12f" % (e_rys, e_hgp, e_hgp_nr), flush=True)
+    assert abs(e_hgp - e_rys) < 1e-9
+    # the reference's EPS = 3e-7 Boys function moves the energy by far less than 1e-4 Eh
+    assert abs(e_hgp_nr - e_rys) < 1e-4
