@@ -18,6 +18,12 @@ FETCH_SIZE/WRITE_SIZE runs (profiles/traffic.json), when recorded.  The
 cpu_baseline block times the compiled reference crys.c (oracle/_ref) on a
 seeded sample of the same workload's unique function quartets,
 single-threaded.
+
+Outside the timed region the line also carries: (H2O)_16 FP64 against mixed
+precision (times, energy error), forces per second on (H2O)_8, and the HGP
+path on (H2O)_8 (all 202,015,050 unique ERIs by the GPU HGP kernel against
+the Rys kernels, RHF energies through both, the compiled reference chgp.c on
+a CPU sample).
 """
 import argparse
 import json
@@ -31,18 +37,20 @@ sys.path.insert(0, ROOT)
 FP64_PEAK_TFLOPS = 78.6
 
 
-def cpu_baseline(bfs, seconds=10.0, seed=0):
-    """Time the reference's crys.c contr_coulomb (compiled from the reference
-    sources into oracle/_ref) on random unique function quartets."""
+def cpu_baseline(bfs, seconds=10.0, seed=0, method="crys"):
+    """Time the reference's crys.c contr_coulomb (or chgp.c contr_hrr),
+    compiled from the reference sources into oracle/_ref, on random unique
+    function quartets."""
     import ctypes
 
     import numpy as np
-    path = os.path.join(ROOT, "oracle", "_ref", "libref_crys.so")
+    path = os.path.join(ROOT, "oracle", "_ref", "libref_%s.so" % method)
     if not os.path.exists(path):
         return None
     lib = ctypes.CDLL(path)
     PD, PI = ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_int)
-    lib.pqref_crys_contr_batch.argtypes = [ctypes.c_int, PI, PI, PD, PD, PD, PD, PI, PD]
+    batch = getattr(lib, "pqref_%s_contr_batch" % method)
+    batch.argtypes = [ctypes.c_int, PI, PI, PD, PD, PD, PD, PI, PD]
     rng = np.random.default_rng(seed)
     n = len(bfs)
     nprim = np.array([len(f.pexps) for f in bfs], dtype=np.int32)
@@ -63,14 +71,14 @@ def cpu_baseline(bfs, seconds=10.0, seed=0):
         a, b, o, p = sample(m)
         out = np.zeros(m)
         t0 = time.perf_counter()
-        lib.pqref_crys_contr_batch(m, a.ctypes.data_as(PI), b.ctypes.data_as(PI), e.ctypes.data_as(PD),
-                                   c.ctypes.data_as(PD), nr.ctypes.data_as(PD), o.ctypes.data_as(PD),
-                                   p.ctypes.data_as(PI), out.ctypes.data_as(PD))
+        batch(m, a.ctypes.data_as(PI), b.ctypes.data_as(PI), e.ctypes.data_as(PD), c.ctypes.data_as(PD),
+              nr.ctypes.data_as(PD), o.ctypes.data_as(PD), p.ctypes.data_as(PI), out.ctypes.data_as(PD))
         t_used += time.perf_counter() - t0
         done += m
+    fn = "crys.c contr_coulomb" if method == "crys" else "chgp.c contr_hrr"
     return {"value": done / t_used, "unit": "quartets/s", "cores": 1, "kind": "reference",
             "sample": "%d random unique function quartets of the (H2O)_8 6-31G** basis, reference "
-                      "crys.c contr_coulomb compiled -O2, %.1f s, 1 thread" % (done, t_used)}
+                      "%s compiled -O2, %.1f s, 1 thread" % (done, fn, t_used)}
 
 
 def recorded_traffic(key):
@@ -165,7 +173,47 @@ def extra_workloads(args):
     tg = _timed(lambda: eng.gradient(D, W), args.steps)
     out.update({"h2o8_ms_per_gradient": tg * 1e3, "h2o8_forces_per_s": 1.0 / tg})
     print("bench: (H2O)_8 gradient %.2f ms (%.1f forces/s)" % (tg * 1e3, 1.0 / tg), file=sys.stderr, flush=True)
+    # HGP path: every unique function quartet of (H2O)_8 by the GPU HGP
+    # kernel, against the Rys kernels' array; RHF through the stored HGP
+    # integrals against the direct Rys RHF
+    if not args.no_hgp:
+        from pyquante_amd.hartree_fock import rhf
+        rys = eng.get2ints("rys")
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        hgp = eng.get2ints("hgp", 0)
+        torch.cuda.synchronize()
+        th = time.perf_counter() - t0
+        dmax = float((hgp - rys).abs().max())
+        del hgp, rys
+        from pyquante_amd import Ints
+        e_r, _, _ = rhf(mol, basis_data="6-31g**", ConvCriteria=1e-9, MaxIter=60, DoAveraging=True)
+        compat = Ints.hgp_compat_boys
+        try:
+            Ints.hgp_compat_boys = False  # exact Boys values: the two algorithms must agree
+            e_h, _, _ = rhf(mol, basis_data="6-31g**", ConvCriteria=1e-9, MaxIter=60, DoAveraging=True,
+                            method="hgp")
+            Ints.hgp_compat_boys = True   # chgp.c's EPS = 3e-7 Boys function, as the reference computes
+            e_hn, _, _ = rhf(mol, basis_data="6-31g**", ConvCriteria=1e-9, MaxIter=60, DoAveraging=True,
+                             method="hgp")
+        finally:
+            Ints.hgp_compat_boys = compat
+        out.update({"h2o8_hgp_get2ints_s": th, "h2o8_hgp_quartets_per_s": eng_len(eng) / th,
+                    "h2o8_hgp_vs_rys_max_abs": dmax, "h2o8_rhf_energy_rys_eh": e_r, "h2o8_rhf_energy_hgp_eh": e_h,
+                    "h2o8_rhf_hgp_minus_rys_eh": e_h - e_r, "h2o8_rhf_hgp_reference_boys_minus_rys_eh": e_hn - e_r})
+        if not args.no_cpu:
+            from pyquante_amd.basis import BasisSet as _BS
+            cpu_h = cpu_baseline(_BS(mol, "6-31g**").bfs, seconds=3.0, method="chgp")
+            if cpu_h:
+                out["h2o8_hgp_cpu_reference_quartets_per_s"] = cpu_h["value"]
+        print("bench: (H2O)_8 HGP get2ints %.2f s (%.2e quartets/s, max |HGP - Rys| %.1e); RHF HGP - Rys %.1e Eh"
+              % (th, eng_len(eng) / th, dmax, e_h - e_r), file=sys.stderr, flush=True)
     return out
+
+
+def eng_len(eng):
+    npair = eng.nbf * (eng.nbf + 1) // 2
+    return npair * (npair + 1) // 2
 
 
 def main():
@@ -176,7 +224,8 @@ def main():
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--no-extra", action="store_true", help="skip the (H2O)_16 and forces workloads")
+    ap.add_argument("--no-extra", action="store_true", help="skip the (H2O)_16, forces and HGP workloads")
+    ap.add_argument("--no-hgp", action="store_true", help="skip the HGP workload")
     ap.add_argument("--streams", type=int, default=4, help="engine streams for the class kernels (0-4)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run one 256 MiB device copy (reference for rocprofv3 FETCH/WRITE_SIZE)")
