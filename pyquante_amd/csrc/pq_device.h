@@ -120,20 +120,23 @@ struct RysPoint {
   double sv;            // local variable on the interval
 };
 
-template <int NR>
-__device__ __forceinline__ RysPoint<NR> rys_prepare(double X) {
+// LT: tab is the order-NR block of the table (rys_coef + rys_off[NR - 1])
+// staged in LDS; otherwise the global table is read
+template <int NR, bool LT = false>
+__device__ __forceinline__ RysPoint<NR> rys_prepare(double X, const double* tab = nullptr) {
   RysPoint<NR> p;
+  const double* base = LT ? tab : rys_coef + rys_off[NR - 1];
   p.asym = X >= rys_xasym[NR];
   if (p.asym) {
     p.rx = 1.0 / X;
     p.rs = rsqrt(X);
-    p.c = nullptr;
+    p.c = base;
     p.sv = 0.0;
   } else {
     const int k = (int)X;
     p.rx = p.rs = 0.0;
     p.sv = 2.0 * (X - (double)k) - 1.0;
-    p.c = rys_coef + rys_off[NR - 1] + k * (2 * NR * (RYS_DEG + 1));
+    p.c = base + k * (2 * NR * (RYS_DEG + 1));
   }
   return p;
 }
@@ -169,20 +172,21 @@ struct RysPoint32 {
   float sv;
 };
 
-template <int NR>
-__device__ __forceinline__ RysPoint32<NR> rys_prepare32(double X) {
+template <int NR, bool LT = false>
+__device__ __forceinline__ RysPoint32<NR> rys_prepare32(double X, const float* tab = nullptr) {
   RysPoint32<NR> p;
+  const float* base = LT ? tab : rys_coef32 + rys_off32[NR - 1];
   p.asym = X >= rys_xasym32[NR];
   if (p.asym) {
     p.rx = 1.0 / X;
     p.rs = rsqrt(X);
-    p.c = nullptr;
+    p.c = base;
     p.sv = 0.f;
   } else {
     const int k = (int)X;
     p.rx = p.rs = 0.0;
     p.sv = (float)(2.0 * (X - (double)k) - 1.0);
-    p.c = rys_coef32 + rys_off32[NR - 1] + k * (2 * NR * RYS_STRIDE32);
+    p.c = base + k * (2 * NR * RYS_STRIDE32);
   }
   return p;
 }
@@ -215,13 +219,19 @@ __device__ __forceinline__ void rys_root32(const RysPoint32<NR>& p, int r, float
 template <int NR, typename T>
 struct RysRule {
   using Point = RysPoint<NR>;
-  static __device__ __forceinline__ Point prepare(double X) { return rys_prepare<NR>(X); }
+  template <bool LT>
+  static __device__ __forceinline__ Point prepare(double X, const void* tab) {
+    return rys_prepare<NR, LT>(X, static_cast<const double*>(tab));
+  }
   static __device__ __forceinline__ void root(const Point& p, int r, T& u, T& w) { rys_root<NR, T>(p, r, u, w); }
 };
 template <int NR>
 struct RysRule<NR, float> {
   using Point = RysPoint32<NR>;
-  static __device__ __forceinline__ Point prepare(double X) { return rys_prepare32<NR>(X); }
+  template <bool LT>
+  static __device__ __forceinline__ Point prepare(double X, const void* tab) {
+    return rys_prepare32<NR, LT>(X, static_cast<const float*>(tab));
+  }
   static __device__ __forceinline__ void root(const Point& p, int r, float& u, float& w) {
     rys_root32<NR>(p, r, u, w);
   }

@@ -73,9 +73,11 @@ __device__ __forceinline__ PrimQ prim_setup(const PairInfo& bp, const PrimPair& 
 // the per-component normalisation factor.
 // UB: the bra pair is the same for the whole wavefront (scalar loads of its
 // primitive pairs).
-template <int LA, int LB, int LC, int LD, int A0, int NAS, int B0, int NBS, typename T, bool UB = true>
+template <int LA, int LB, int LC, int LD, int A0, int NAS, int B0, int NBS, typename T, bool UB = true,
+          bool LT = false>
 __device__ __forceinline__ void eri_block(const EngineView& ev, const PairInfo& bp, const PairInfo& kp, int i0,
-                                          int i1, T (&out)[NAS * NBS * ncart(LC) * ncart(LD)]) {
+                                          int i1, T (&out)[NAS * NBS * ncart(LC) * ncart(LD)],
+                                          const void* rtab = nullptr) {
   using C = Cls<LA, LB, LC, LD>;
   constexpr int NR = C::NR;
 #pragma unroll
@@ -85,7 +87,7 @@ __device__ __forceinline__ void eri_block(const EngineView& ev, const PairInfo& 
   // one primitive quartet of the contraction, accumulated into out
   auto quartet = [&](const PrimPair& b, const PrimPair& q) __attribute__((always_inline)) {
     const PrimQ s = prim_setup(bp, b, kp, q);
-    const typename RysRule<NR, T>::Point rp = RysRule<NR, T>::prepare(s.X);
+    const typename RysRule<NR, T>::Point rp = RysRule<NR, T>::template prepare<LT>(s.X, rtab);
     const T pref = (T)s.pref, qfac = (T)s.qfac, pfac = (T)s.pfac;
     const T half_rpq = (T)(0.5 * s.rpq), i2p = (T)s.i2p, i2q = (T)s.i2q;
     // roots unrolled for the small orders; rolled above to bound code size
@@ -280,10 +282,11 @@ __device__ __forceinline__ void atomic_max_nonneg(double* p, double v) {
 }
 
 // One lane, one shell quartet, bra components [A0, A0+NAS) x [B0, B0+NBS).
-template <int LA, int LB, int LC, int LD, typename T, int MODE, int A0, int NAS, int B0, int NBS>
+template <int LA, int LB, int LC, int LD, typename T, int MODE, int A0, int NAS, int B0, int NBS, bool LT = false>
 __device__ __forceinline__ void lane_body(const EngineView& ev, const Tile& tl, bool active, int bra, int ket,
                                           const double* __restrict__ D, double* Jacc, double* Kacc,
-                                          double* ints, double* qout, double* krow, double* jsum) {
+                                          double* ints, double* qout, double* krow, double* jsum,
+                                          const void* rtab = nullptr) {
   using C = Cls<LA, LB, LC, LD>;
   constexpr bool UB = MODE != MODE_DIAG;  // Schwarz diagonal: bra = ket per lane
   const PairInfo bp = UB ? ld_uniform(ev.pairs + bra) : ev.pairs[bra];
@@ -292,7 +295,7 @@ __device__ __forceinline__ void lane_body(const EngineView& ev, const Tile& tl, 
   T out[NO];
   if (active) {
     const int i0 = tl.prim & 0xffff, ni = tl.prim >> 16;
-    eri_block<LA, LB, LC, LD, A0, NAS, B0, NBS, T, UB>(ev, bp, kp, i0, ni ? i0 + ni : bp.pp_n, out);
+    eri_block<LA, LB, LC, LD, A0, NAS, B0, NBS, T, UB, LT>(ev, bp, kp, i0, ni ? i0 + ni : bp.pp_n, out, rtab);
   } else {
 #pragma unroll
     for (int c = 0; c < NO; ++c) out[c] = (T)0;
@@ -349,6 +352,15 @@ __device__ __forceinline__ void static_switch(int i, F&& f) {
 //     the ket-side J accumulator.
 constexpr int kWaveSplit = 4;
 
+// Root table of order NR staged in LDS by k_eri_group, in doubles of LDS:
+// the FP64 table for orders 1-3 (7.1, 16.9 and 29.4 KB), the FP32 table for
+// every order (1.2-13.4 KB); 0 = read the global table.
+template <int NR, typename T>
+constexpr int kLdsTableDoubles =
+    std::is_same<T, double>::value
+        ? (NR <= 3 ? (NR == 1 ? 37 : NR == 2 ? 44 : 51) * 2 * NR * (RYS_DEG + 1) : 0)  // rys_nint[1..3]
+        : (NR == 1 ? 19 : NR == 2 ? 25 : NR == 3 ? 31 : NR == 4 ? 37 : 42) * 2 * NR * RYS_STRIDE32 / 2;
+
 // Unsplit classes with bra groups: one block of 4 wavefronts walks a run of
 // tiles that share the bra pair (wavefront w takes tiles w, w+4, ...), so the
 // LDS K rows are zeroed and flushed once per run.
@@ -364,8 +376,18 @@ __global__ __launch_bounds__(256) void k_eri_group(EngineView ev, const Tile* __
   const int wv = threadIdx.x >> 6;
   const int n = ev.nbf;
   const int2 g = groups[blockIdx.x];
-  double* krow = pq_lds;
-  for (int i = threadIdx.x; i < (C::NA + C::NB) * n; i += blockDim.x) pq_lds[i] = 0.0;
+  // light FP64 classes read the root table from LDS (one copy per block)
+  // instead of gathering it from L1/L2 per lane
+  constexpr int NT = kLdsTableDoubles<C::NR, T>;
+  void* rtab = NT ? pq_lds : nullptr;
+  if constexpr (std::is_same<T, double>::value) {
+    for (int i = threadIdx.x; i < NT; i += blockDim.x) pq_lds[i] = rys_coef[rys_off[C::NR - 1] + i];
+  } else {
+    float* t32 = reinterpret_cast<float*>(pq_lds);
+    for (int i = threadIdx.x; i < 2 * NT; i += blockDim.x) t32[i] = rys_coef32[rys_off32[C::NR - 1] + i];
+  }
+  double* krow = pq_lds + NT;
+  for (int i = threadIdx.x; i < (C::NA + C::NB) * n; i += blockDim.x) krow[i] = 0.0;
   __syncthreads();
   for (int tt = wv; tt < g.y; tt += 4) {
     Tile tl = ld_uniform(tiles + g.x + tt);
@@ -375,8 +397,8 @@ __global__ __launch_bounds__(256) void k_eri_group(EngineView ev, const Tile* __
     tl.prim = __builtin_amdgcn_readfirstlane(tl.prim);
     const bool active = lane < tl.nket;
     const int ket = kets[tl.ket0 + (active ? lane : 0)];
-    lane_body<LA, LB, LC, LD, T, MODE_JK, 0, C::NA, 0, C::NB>(ev, tl, active, tl.bra, ket, D, Jacc, Kacc, nullptr,
-                                                             nullptr, krow, nullptr);
+    lane_body<LA, LB, LC, LD, T, MODE_JK, 0, C::NA, 0, C::NB, (NT > 0)>(ev, tl, active, tl.bra, ket, D, Jacc, Kacc,
+                                                                       nullptr, nullptr, krow, nullptr, rtab);
   }
   __syncthreads();
   const PairInfo bp = ev.pairs[tiles[g.x].bra];
@@ -475,8 +497,9 @@ static hipError_t launch_one(const EngineView& ev, const Tile* tiles, int ntiles
   using C = Cls<LA, LB, LC, LD>;
   const size_t lds = ((size_t)(C::NA + C::NB) * ev.nbf + (C::SPLIT ? 64 * C::NC * C::ND : 0)) * sizeof(double);
   if constexpr (C::SPLIT == 0 && MODE == MODE_JK) {
-    if (groups && ngroups > 0 && lds <= 65536) {
-      hipLaunchKernelGGL((k_eri_group<LA, LB, LC, LD, T, MODE>), dim3(ngroups), dim3(256), lds, st, ev, tiles,
+    const size_t glds = lds + kLdsTableDoubles<C::NR, T> * sizeof(double);
+    if (groups && ngroups > 0 && glds <= 65536) {
+      hipLaunchKernelGGL((k_eri_group<LA, LB, LC, LD, T, MODE>), dim3(ngroups), dim3(256), glds, st, ev, tiles,
                          groups, ngroups, kets, D, Jacc, Kacc);
       return hipGetLastError();
     }
