@@ -70,9 +70,31 @@ struct __align__(16) PairInfo {
 struct Tile {
   int bra;   // pair id
   int ket0;  // first entry of the ket pair list covered by this tile
-  int nket;  // number of kets
+  int nket;  // number of kets (bits 0-7) and log2 of the ket-primitive split ks (bits 8-10)
   int prim;  // bra primitive-pair range: first (low 16 bits), count (high 16 bits; 0 = all)
 };
+
+// Lanes of a tile: with nk kets and a ket-primitive split ks, lane l takes
+// ket l % nk and the (l / nk)-th of ks near-equal chunks of that ket's
+// primitive pairs; each lane's partial block digests on its own (J and K are
+// linear in the integrals).  ks = 1 for stored integrals and Schwarz bounds.
+struct TileLane {
+  bool active;
+  int slot;  // ket index within the tile
+  int kc;    // ket-primitive chunk
+  int ks;
+  int nk;
+};
+__device__ __forceinline__ TileLane tile_lane(int nket_field, int lane) {
+  const int nk = nket_field & 0xff, ks = 1 << ((nket_field >> 8) & 7);
+  TileLane t;
+  t.active = lane < nk * ks;
+  t.slot = t.active ? lane % nk : 0;
+  t.kc = t.active ? lane / nk : 0;
+  t.ks = ks;
+  t.nk = nk;
+  return t;
+}
 
 // Everything a kernel needs about the basis and the pair data.
 struct EngineView {

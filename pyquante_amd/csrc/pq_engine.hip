@@ -392,11 +392,23 @@ int build_work(pq_engine* e, std::vector<ClassWork>& out, double cut, double fp3
           const int kpn = e->pairs[kets[g0]].pp_n;
           int chunk = bpn;
           if (split_prims) chunk = std::max(1, std::min(bpn, kPrimBudget / std::max(1, kpn)));
+          // a tile with fewer than 64 kets spreads each ket's primitive
+          // pairs over ks lanes (ks a power of two, ks * kets <= 64), whose
+          // blocks are summed by shuffles before digestion; (ss|ss) has too
+          // little work per lane for that to pay (measured)
+          const bool ksplit = split_prims && la + lb + lc + ld > 0;
+          auto nket_field = [&](int cnt) {
+            int lg = 0;
+            while (ksplit && lg < 6 && (cnt << (lg + 1)) <= tile && (1 << (lg + 1)) <= kpn) ++lg;
+            return cnt | (lg << 8);
+          };
           for (int p0 = 0; p0 < bpn; p0 += chunk) {
             const int np = std::min(chunk, bpn - p0);
             const int prim = np == bpn ? 0 : (p0 | (np << 16));
-            for (int k = g0; k < e64; k += tile) cw.t64.push_back(Tile{bras[i], k, std::min(tile, e64 - k), prim});
-            for (int k = e64; k < end; k += tile) cw.t32.push_back(Tile{bras[i], k, std::min(tile, end - k), prim});
+            for (int k = g0; k < e64; k += tile)
+              cw.t64.push_back(Tile{bras[i], k, nket_field(std::min(tile, e64 - k)), prim});
+            for (int k = e64; k < end; k += tile)
+              cw.t32.push_back(Tile{bras[i], k, nket_field(std::min(tile, end - k)), prim});
           }
           cw.nq64 += e64 - g0;
           cw.nq32 += end - e64;

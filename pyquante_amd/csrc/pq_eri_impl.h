@@ -76,7 +76,7 @@ __device__ __forceinline__ PrimQ prim_setup(const PairInfo& bp, const PrimPair& 
 template <int LA, int LB, int LC, int LD, int A0, int NAS, int B0, int NBS, typename T, bool UB = true,
           bool LT = false>
 __device__ __forceinline__ void eri_block(const EngineView& ev, const PairInfo& bp, const PairInfo& kp, int i0,
-                                          int i1, T (&out)[NAS * NBS * ncart(LC) * ncart(LD)],
+                                          int i1, int k0, int k1, T (&out)[NAS * NBS * ncart(LC) * ncart(LD)],
                                           const void* rtab = nullptr) {
   using C = Cls<LA, LB, LC, LD>;
   constexpr int NR = C::NR;
@@ -128,7 +128,7 @@ __device__ __forceinline__ void eri_block(const EngineView& ev, const PairInfo& 
   // ket primitives (per lane) outside, bra primitives (shared by the
   // wavefront, scalar loads) inside
 #pragma unroll 1
-  for (int k = 0; k < kp.pp_n; ++k) {
+  for (int k = k0; k < k1; ++k) {
     const PrimPair q = ev.prims[kp.pp_off + k];
 #pragma unroll 1
     for (int i = i0; i < i1; ++i) quartet(bra_prim(i), q);
@@ -286,7 +286,7 @@ template <int LA, int LB, int LC, int LD, typename T, int MODE, int A0, int NAS,
 __device__ __forceinline__ void lane_body(const EngineView& ev, const Tile& tl, bool active, int bra, int ket,
                                           const double* __restrict__ D, double* Jacc, double* Kacc,
                                           double* ints, double* qout, double* krow, double* jsum,
-                                          const void* rtab = nullptr) {
+                                          const void* rtab = nullptr, int kc = 0, int ks = 1, int nk = 64) {
   using C = Cls<LA, LB, LC, LD>;
   constexpr bool UB = MODE != MODE_DIAG;  // Schwarz diagonal: bra = ket per lane
   const PairInfo bp = UB ? ld_uniform(ev.pairs + bra) : ev.pairs[bra];
@@ -295,12 +295,26 @@ __device__ __forceinline__ void lane_body(const EngineView& ev, const Tile& tl, 
   T out[NO];
   if (active) {
     const int i0 = tl.prim & 0xffff, ni = tl.prim >> 16;
-    eri_block<LA, LB, LC, LD, A0, NAS, B0, NBS, T, UB, LT>(ev, bp, kp, i0, ni ? i0 + ni : bp.pp_n, out, rtab);
+    const int k0 = kc * kp.pp_n / ks, k1 = (kc + 1) * kp.pp_n / ks;
+    eri_block<LA, LB, LC, LD, A0, NAS, B0, NBS, T, UB, LT>(ev, bp, kp, i0, ni ? i0 + ni : bp.pp_n, k0, k1, out,
+                                                           rtab);
   } else {
 #pragma unroll
     for (int c = 0; c < NO; ++c) out[c] = (T)0;
   }
   if constexpr (MODE == MODE_JK) {
+    // ket-primitive chunks of one quartet (lanes slot + j*nk) are summed into
+    // the chunk-0 lane, which alone digests
+    if (ks > 1) {
+      for (int step = ks >> 1; step >= 1; step >>= 1) {
+#pragma unroll
+        for (int c = 0; c < NO; ++c) {
+          const T v = __shfl_down(out[c], step * nk);
+          if (kc < step) out[c] += v;
+        }
+      }
+      active = active && kc == 0;
+    }
     const double f = active ? quartet_degeneracy(bp, kp, bra, ket) : 0.0;
     digest_lane<LA, LB, LC, LD, A0, NAS, B0, NBS, T>(ev, bp, kp, out, f, active, D, Jacc, Kacc, krow, jsum);
   } else if constexpr (MODE == MODE_STORE) {
@@ -395,10 +409,11 @@ __global__ __launch_bounds__(256) void k_eri_group(EngineView ev, const Tile* __
     tl.ket0 = __builtin_amdgcn_readfirstlane(tl.ket0);
     tl.nket = __builtin_amdgcn_readfirstlane(tl.nket);
     tl.prim = __builtin_amdgcn_readfirstlane(tl.prim);
-    const bool active = lane < tl.nket;
-    const int ket = kets[tl.ket0 + (active ? lane : 0)];
-    lane_body<LA, LB, LC, LD, T, MODE_JK, 0, C::NA, 0, C::NB, (NT > 0)>(ev, tl, active, tl.bra, ket, D, Jacc, Kacc,
-                                                                       nullptr, nullptr, krow, nullptr, rtab);
+    const TileLane tn = tile_lane(tl.nket, lane);
+    const int ket = kets[tl.ket0 + tn.slot];
+    lane_body<LA, LB, LC, LD, T, MODE_JK, 0, C::NA, 0, C::NB, (NT > 0)>(ev, tl, tn.active, tl.bra, ket, D, Jacc,
+                                                                       Kacc, nullptr, nullptr, krow, nullptr, rtab,
+                                                                       tn.kc, tn.ks, tn.nk);
   }
   __syncthreads();
   const PairInfo bp = ev.pairs[tiles[g.x].bra];
@@ -439,24 +454,26 @@ __global__ __launch_bounds__(256) void k_eri_lane(EngineView ev, const Tile* __r
   tl.ket0 = __builtin_amdgcn_readfirstlane(tl.ket0);
   tl.nket = __builtin_amdgcn_readfirstlane(tl.nket);
   tl.prim = __builtin_amdgcn_readfirstlane(tl.prim);
-  const bool active = lane < tl.nket;
-  const int ket = kets[tl.ket0 + (active ? lane : 0)];
+  const TileLane tn = tile_lane(tl.nket, lane);
+  const bool active = tn.active;
+  const int ket = kets[tl.ket0 + tn.slot];
   const int bra = (MODE == MODE_DIAG) ? ket : tl.bra;
   if constexpr (C::SPLIT == 0) {
     lane_body<LA, LB, LC, LD, T, MODE, 0, C::NA, 0, C::NB>(ev, tl, active, bra, ket, D, Jacc, Kacc, ints, qout,
-                                                          krow, jsum);
+                                                          krow, jsum, nullptr, tn.kc, tn.ks, tn.nk);
   } else {
     const int split = lds_mode ? (int)blockIdx.y * kWaveSplit + wv : (int)blockIdx.y;
     if constexpr (C::SPLIT == 1) {
       static_switch<0, C::NA>(split, [&](auto ia) {
-        lane_body<LA, LB, LC, LD, T, MODE, decltype(ia)::value, 1, 0, C::NB>(ev, tl, active, bra, ket, D, Jacc,
-                                                                              Kacc, ints, qout, krow, jsum);
+        lane_body<LA, LB, LC, LD, T, MODE, decltype(ia)::value, 1, 0, C::NB>(
+            ev, tl, active, bra, ket, D, Jacc, Kacc, ints, qout, krow, jsum, nullptr, tn.kc, tn.ks, tn.nk);
       });
     } else {
       static_switch<0, C::NA * C::NB>(split, [&](auto iab) {
         constexpr int v = decltype(iab)::value;
         lane_body<LA, LB, LC, LD, T, MODE, v / C::NB, 1, v % C::NB, 1>(ev, tl, active, bra, ket, D, Jacc, Kacc,
-                                                                        ints, qout, krow, jsum);
+                                                                        ints, qout, krow, jsum, nullptr, tn.kc,
+                                                                        tn.ks, tn.nk);
       });
     }
   }
@@ -476,10 +493,11 @@ __global__ __launch_bounds__(256) void k_eri_lane(EngineView ev, const Tile* __r
     constexpr int NCD = C::NC * C::ND;
     for (int i = threadIdx.x; i < 64 * NCD; i += blockDim.x) {
       const int l = i / NCD, cd = i - l * NCD;
-      if (l >= tl.nket) break;
+      const TileLane tl_l = tile_lane(tl.nket, l);
+      if (!tl_l.active) break;
       const double v = jsum[i];
       if (v != 0.0) {
-        const PairInfo kq = ev.pairs[kets[tl.ket0 + l]];
+        const PairInfo kq = ev.pairs[kets[tl.ket0 + tl_l.slot]];
         unsafeAtomicAdd(&Jacc[(kq.bfA + cd / C::ND) * n + kq.bfB + cd % C::ND], v);
       }
     }

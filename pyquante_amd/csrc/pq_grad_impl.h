@@ -27,7 +27,8 @@ struct GCls {
 
 template <int LA, int LB, int LC, int LD, int A0, int NAS, int B0, int NBS>
 __device__ __forceinline__ void grad_lane(const EngineView& ev, const Tile& tl, bool active, int bra, int ket,
-                                          const double* __restrict__ D, double* gcopy, double* gl) {
+                                          const double* __restrict__ D, double* gcopy, double* gl, int kc,
+                                          int ks) {
   using C = Cls<LA, LB, LC, LD>;
   constexpr int NR = GCls<LA, LB, LC, LD>::NR;
   constexpr int NO = NAS * NBS * C::NC * C::ND;
@@ -63,7 +64,7 @@ __device__ __forceinline__ void grad_lane(const EngineView& ev, const Tile& tl, 
     const double AB[3] = {bp.ABx, bp.ABy, bp.ABz};
     const double CD[3] = {kp.ABx, kp.ABy, kp.ABz};
 #pragma unroll 1
-    for (int k = 0; k < kp.pp_n; ++k) {
+    for (int k = kc * kp.pp_n / ks; k < (kc + 1) * kp.pp_n / ks; ++k) {
       const PrimPair q = ev.prims[kp.pp_off + k];
       const double gam = ev.prim_a[kp.pp_off + k];
 #pragma unroll 1
@@ -161,19 +162,22 @@ __global__ __launch_bounds__(64) void k_eri_grad(EngineView ev, const Tile* __re
   tl.ket0 = __builtin_amdgcn_readfirstlane(tl.ket0);
   tl.nket = __builtin_amdgcn_readfirstlane(tl.nket);
   tl.prim = __builtin_amdgcn_readfirstlane(tl.prim);
-  const bool active = lane < tl.nket;
-  const int ket = kets[tl.ket0 + (active ? lane : 0)];
+  const TileLane tn = tile_lane(tl.nket, lane);
+  const bool active = tn.active;
+  const int ket = kets[tl.ket0 + tn.slot];
   double* gcopy = gacc + (size_t)((t * GC::NSPLIT + blockIdx.y) % kGradCopies) * 3 * natom;
   if constexpr (GC::SPLIT == 0) {
-    grad_lane<LA, LB, LC, LD, 0, C::NA, 0, C::NB>(ev, tl, active, tl.bra, ket, D, gcopy, gl);
+    grad_lane<LA, LB, LC, LD, 0, C::NA, 0, C::NB>(ev, tl, active, tl.bra, ket, D, gcopy, gl, tn.kc, tn.ks);
   } else if constexpr (GC::SPLIT == 1) {
     static_switch<0, C::NA>(blockIdx.y, [&](auto ia) {
-      grad_lane<LA, LB, LC, LD, decltype(ia)::value, 1, 0, C::NB>(ev, tl, active, tl.bra, ket, D, gcopy, gl);
+      grad_lane<LA, LB, LC, LD, decltype(ia)::value, 1, 0, C::NB>(ev, tl, active, tl.bra, ket, D, gcopy, gl, tn.kc,
+                                                                  tn.ks);
     });
   } else {
     static_switch<0, C::NA * C::NB>(blockIdx.y, [&](auto iab) {
       constexpr int v = decltype(iab)::value;
-      grad_lane<LA, LB, LC, LD, v / C::NB, 1, v % C::NB, 1>(ev, tl, active, tl.bra, ket, D, gcopy, gl);
+      grad_lane<LA, LB, LC, LD, v / C::NB, 1, v % C::NB, 1>(ev, tl, active, tl.bra, ket, D, gcopy, gl, tn.kc,
+                                                            tn.ks);
     });
   }
   __syncthreads();
