@@ -20,7 +20,8 @@ seeded sample of the same workload's unique function quartets,
 single-threaded.
 
 Outside the timed region the line also carries: (H2O)_16 FP64 against mixed
-precision (times, energy error), forces per second on (H2O)_8, and the HGP
+precision (times, energy error), forces per second on (H2O)_8, MP2 on
+(H2O)_8 (stored ERIs, AO->MO transform as device GEMMs), and the HGP
 path on (H2O)_8 (all 202,015,050 unique ERIs by the GPU HGP kernel against
 the Rys kernels, RHF energies through both, the compiled reference chgp.c on
 a CPU sample).
@@ -133,6 +134,7 @@ def _timed(fn, steps):
 def extra_workloads(args):
     """Mixed precision on (H2O)_16 and forces on (H2O)_8 (outside the
     headline timed region; reported alongside it)."""
+    import numpy as np
     import torch
 
     from pyquante_amd import _abi
@@ -169,10 +171,32 @@ def extra_workloads(args):
     mol = water_cluster((2, 2, 2), seed=args.seed)
     eng = Engine(BasisSet(mol, "6-31g**"), mol)
     eng.build(1e-12, 0.0)
-    _, _, _, D, W = scf(eng, mol, ConvCriteria=1e-9)
+    e8, orbe8, C8, D, W = scf(eng, mol, ConvCriteria=1e-9)
     tg = _timed(lambda: eng.gradient(D, W), args.steps)
     out.update({"h2o8_ms_per_gradient": tg * 1e3, "h2o8_forces_per_s": 1.0 / tg})
     print("bench: (H2O)_8 gradient %.2f ms (%.1f forces/s)" % (tg * 1e3, 1.0 / tg), file=sys.stderr, flush=True)
+    # MP2 on (H2O)_8: stored ERIs, row-block unpacking and device GEMMs
+    if not args.no_mp2:
+        from pyquante_amd.Ints import Integrals
+        from pyquante_amd.MP import MP2
+        nocc = mol.get_closedopen()[0]
+        nbf = eng.nbf
+        orbs = C8.cpu().numpy() if torch.is_tensor(C8) else np.asarray(C8)
+        oe = orbe8.cpu().numpy() if torch.is_tensor(orbe8) else np.asarray(orbe8)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        ints = Integrals(None, None, engine=eng)
+        e2 = MP2(ints, orbs, oe, nocc, nbf - nocc)
+        torch.cuda.synchronize()
+        tm = time.perf_counter() - t0
+        nv = nbf - nocc
+        gflop = (2.0 * nocc * nbf ** 4 + 2.0 * nocc ** 2 * nbf ** 3 + 2.0 * nocc ** 2 * nbf ** 2 * nv +
+                 2.0 * nocc ** 2 * nv ** 2 * nbf) / 1e9
+        out.update({"h2o8_mp2_s": tm, "h2o8_mp2_correlation_eh": e2, "h2o8_rhf_plus_mp2_eh": e8 + e2,
+                    "h2o8_mp2_transform_gflop": gflop})
+        print("bench: (H2O)_8 MP2 %.3f s (E2 = %.8f Eh, transform %.0f GFLOP)" % (tm, e2, gflop),
+              file=sys.stderr, flush=True)
+        del ints
     # HGP path: every unique function quartet of (H2O)_8 by the GPU HGP
     # kernel, against the Rys kernels' array; RHF through the stored HGP
     # integrals against the direct Rys RHF
@@ -226,6 +250,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extra", action="store_true", help="skip the (H2O)_16, forces and HGP workloads")
     ap.add_argument("--no-hgp", action="store_true", help="skip the HGP workload")
+    ap.add_argument("--no-mp2", action="store_true", help="skip the MP2 workload")
     ap.add_argument("--streams", type=int, default=4, help="engine streams for the class kernels (0-4)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run one 256 MiB device copy (reference for rocprofv3 FETCH/WRITE_SIZE)")

@@ -185,6 +185,12 @@ int pq_fock_jk_timed(pq_engine *eng, const double *D_dev, double *J_dev, double 
 int pq_digest_stored(int nbf, const double *ints_dev, uint64_t len, const double *D_dev, double *J_dev,
                      double *K_dev, void *stream);
 
+/* Rows mu0 .. mu0+nmu-1 of the full 4-index tensor (mu nu|lam sig) from the
+ * flat unique array (len = npair(npair+1)/2), row-major into out_dev
+ * (nmu*nbf^3 doubles): the unpacking step of the reference's AO->MO transform
+ * (PyQuante/MP.py:38-46), done once per row block instead of per element. */
+int pq_unpack_eri(int nbf, const double *ints_dev, uint64_t len, int mu0, int nmu, double *out_dev, void *stream);
+
 /* Ints.get1ints (PyQuante/Ints.py:73-87): S, T and V = sum_A Z_A V_A
  * (h = T + V), nbf x nbf row-major FP64, on the device. */
 int pq_get1ints(pq_engine *eng, double *S_dev, double *T_dev, double *V_dev, void *stream);

@@ -1014,6 +1014,15 @@ int pq_digest_stored(int nbf, const double* ints_dev, uint64_t len, const double
   return PQ_OK;
 }
 
+int pq_unpack_eri(int nbf, const double* ints_dev, uint64_t len, int mu0, int nmu, double* out_dev, void* stream) {
+  if (nbf <= 0 || !ints_dev || !out_dev) return fail(PQ_E_ARG, "bad arguments");
+  const uint64_t n = nbf, npair = n * (n + 1) / 2;
+  if (len != npair * (npair + 1) / 2) return fail(PQ_E_ARG, "integral array length mismatch");
+  if (mu0 < 0 || nmu < 0 || mu0 + nmu > nbf) return fail(PQ_E_ARG, "row range out of bounds");
+  PQ_HIP(launch_unpack_eri(nbf, ints_dev, mu0, nmu, out_dev, (hipStream_t)stream));
+  return PQ_OK;
+}
+
 int pq_forces(pq_engine* eng, const double* D_dev, const double* W_dev, double* grad_dev, void* stream) {
   if (!eng || !D_dev || !W_dev || !grad_dev) return fail(PQ_E_ARG, "null argument");
   if (!eng->work_ok) return fail(PQ_E_STATE, "pq_build_pairs must be called before pq_forces");

@@ -120,6 +120,31 @@ hipError_t launch_digest_stored(int n, const double* ints, uint64_t len, const d
   return hipGetLastError();
 }
 
+// Rows mu0 .. mu0+nmu-1 of the full 4-index tensor (mu nu|lam sig), row-major,
+// from the flat unique array: out[((mu-mu0) n + nu) n^2 + lam n + sig] =
+// ints[ijkl2intindex(mu, nu, lam, sig)] (the reference's unpacking,
+// MP.py:38-46).  Writes are coalesced; reads gather.
+__global__ void k_unpack_eri(int n, const double* __restrict__ ints, int mu0, uint64_t total, double* out) {
+  uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  const uint64_t n2 = (uint64_t)n * n, n3 = n2 * n;
+  for (; t < total; t += stride) {
+    const int mu = mu0 + (int)(t / n3);
+    const uint64_t r = t % n3;
+    const int nu = (int)(r / n2), lam = (int)((r / n) % n), sig = (int)(r % n);
+    out[t] = ints[ijkl2intindex(mu, nu, lam, sig)];
+  }
+}
+
+hipError_t launch_unpack_eri(int n, const double* ints, int mu0, int nmu, double* out, hipStream_t st) {
+  const uint64_t total = (uint64_t)nmu * n * n * n;
+  if (total == 0) return hipSuccess;
+  uint64_t blocks = (total + 255) / 256;
+  if (blocks > 65536) blocks = 65536;
+  hipLaunchKernelGGL(k_unpack_eri, dim3((unsigned)blocks), dim3(256), 0, st, n, ints, mu0, total, out);
+  return hipGetLastError();
+}
+
 static hipError_t launch_diag_0000(const EngineView& ev, const Tile* tiles, int ntiles, const int* kets, const double* D,
                                    double* Jacc, double* Kacc, double* ints, double* qout, hipStream_t st) {
   return launch_cls_0000_m3(ev, tiles, ntiles, kets, D, Jacc, Kacc, ints, qout, st, nullptr, 0);

@@ -60,6 +60,8 @@ hipError_t launch_hgp_contr(int n, const int* nprim, const int* poff, const doub
 hipError_t launch_hgp_store(uint64_t len, const int* fnp, const int* foff, const double* forg, const int* fpw,
                             const double* fexps, const double* fcoefs, const double* fnorms, int compat_nr,
                             double* out, hipStream_t st);
+// rows [mu0, mu0+nmu) of the full (mu nu|lam sig) tensor from the flat array
+hipError_t launch_unpack_eri(int n, const double* ints, int mu0, int nmu, double* out, hipStream_t st);
 hipError_t launch_boys(int n, const double* m, const double* x, double eps, double* out, hipStream_t st);
 hipError_t launch_rys_roots(int n, int nr, const double* X, double* u, double* w, hipStream_t st);
 hipError_t launch_contr_eri(int n, const int* nprim, const int* poff, const double* exps,

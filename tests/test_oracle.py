@@ -232,3 +232,22 @@ def test_h2o_sto3g_onee_and_norms(golden):
             assert abs(nrm - O.lib().pqo_prim_norm(e, *f.powers)) < 1e-15 * nrm
     d = np.diag(S)
     assert np.all(np.abs(d - 1) < 1e-2) and np.any(np.abs(d - 1) > 1e-9)
+
+
+def test_mp2_restatement_h2_golden():
+    """H2 at +-1 bohr, 6-31G**: the reference's MP2 test energy
+    (Tests/h2_mp2.py:14, -1.11591861694, tolerance 1e-6 there) and
+    GAMESS-UK's -1.11591790 (h2_mp2.py:12)."""
+    from oracle import mp2, scf
+    from pyquante_amd.basis import BasisSet
+    from pyquante_amd.molecule import Molecule
+    mol = Molecule("h2", [(1, (1.0, 0, 0)), (1, (-1.0, 0, 0))])
+    bfs = BasisSet(mol, None).bfs
+    e, orbe, orbs, _ = scf.rhf(bfs, mol, ConvCriteria=1e-12, MaxIter=100)
+    ints = O.get2ints(bfs)
+    e2 = mp2.mp2(ints, orbs, orbe, 1, len(bfs) - 1)
+    assert abs(e + e2 - (-1.11591861694)) < 1e-6
+    assert abs(e + e2 - (-1.11591790)) < 1e-7
+    # the unpacked tensor has the 8-fold permutational symmetry
+    A = mp2.ao_tensor(ints, len(bfs))
+    assert np.array_equal(A, A.transpose(1, 0, 2, 3)) and np.array_equal(A, A.transpose(2, 3, 0, 1))
