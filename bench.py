@@ -20,7 +20,8 @@ seeded sample of the same workload's unique function quartets,
 single-threaded.
 
 Outside the timed region the line also carries: (H2O)_16 FP64 against mixed
-precision (times, energy error), forces per second on (H2O)_8, MP2 on
+precision (times, energy error), forces per second on (H2O)_8, J/K of two
+densities in one pass against two builds (the unrestricted path), MP2 on
 (H2O)_8 (stored ERIs, AO->MO transform as device GEMMs), and the HGP
 path on (H2O)_8 (all 202,015,050 unique ERIs by the GPU HGP kernel against
 the Rys kernels, RHF energies through both, the compiled reference chgp.c on
@@ -175,6 +176,13 @@ def extra_workloads(args):
     tg = _timed(lambda: eng.gradient(D, W), args.steps)
     out.update({"h2o8_ms_per_gradient": tg * 1e3, "h2o8_forces_per_s": 1.0 / tg})
     print("bench: (H2O)_8 gradient %.2f ms (%.1f forces/s)" % (tg * 1e3, 1.0 / tg), file=sys.stderr, flush=True)
+    # two densities in one pass (the unrestricted J/K) against two builds
+    Db_ = (0.9 * D + 0.1 * torch.eye(D.shape[0], dtype=D.dtype, device=D.device)).contiguous()
+    t2 = _timed(lambda: eng.fock_jk_multi([D, Db_]), args.steps)
+    t1 = _timed(lambda: (eng.fock_jk(D), eng.fock_jk(Db_)), args.steps)
+    out.update({"h2o8_two_density_ms": t2 * 1e3, "h2o8_two_single_builds_ms": t1 * 1e3})
+    print("bench: (H2O)_8 J/K of two densities %.2f ms in one pass, %.2f ms as two builds" % (t2 * 1e3, t1 * 1e3),
+          file=sys.stderr, flush=True)
     # MP2 on (H2O)_8: stored ERIs, row-block unpacking and device GEMMs
     if not args.no_mp2:
         from pyquante_amd.Ints import Integrals

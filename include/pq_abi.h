@@ -171,6 +171,14 @@ int pq_get2ints_hgp(pq_engine *eng, double *out_dev, uint64_t len, int compat_nr
 int pq_fock_jk(pq_engine *eng, const double *D_dev, double *J_dev, double *K_dev, int precision,
                void *stream);
 
+/* J and K of ndens (1 or 2) densities from one pass over the integrals:
+ * D_dev, J_dev, K_dev are host arrays of ndens device pointers (entries of
+ * J_dev / K_dev may be NULL).  Replaces the four getJ/getK calls of an
+ * unrestricted SCF iteration (hartree_fock.py:233-236, PyQuante2.py:320-343)
+ * with one integral evaluation.  ndens = 2 is FP64 only. */
+int pq_fock_jk_multi(pq_engine *eng, int ndens, const double *const *D_dev, double *const *J_dev,
+                     double *const *K_dev, int precision, void *stream);
+
 /* pq_fock_jk with HIP events around every class launch, recorded on the
  * stream the kernel is launched on (the engine streams of
  * pq_engine_set_streams, so concurrent kernels are timed as they run; 0

@@ -116,8 +116,9 @@ struct pq_engine {
   std::vector<ClassWork> work;
   std::vector<ClassWork> store_work;  // unscreened, for get2ints
   bool store_ok = false;
-  double* d_Jacc = nullptr;
+  double* d_Jacc = nullptr;  // 2 n^2: room for two densities
   double* d_Kacc = nullptr;
+  double* d_D2 = nullptr;    // two stacked densities (allocated on first use)
   double* d_gacc = nullptr;  // private gradient copies
   size_t gacc_n = 0;
 
@@ -192,6 +193,7 @@ struct pq_engine {
     dev_free(d_atom_z);
     dev_free(d_Jacc);
     dev_free(d_Kacc);
+    dev_free(d_D2);
     dev_free(d_gacc);
   }
 };
@@ -632,7 +634,8 @@ int pq_engine_create(int nshell, const int* shell_l, const int* shell_nprim, con
   }
   e->nbf = nbf;
   if (dev_alloc(&e->d_atom_xyz, 3 * natom) != hipSuccess || dev_alloc(&e->d_atom_z, natom) != hipSuccess ||
-      dev_alloc(&e->d_Jacc, (size_t)nbf * nbf) != hipSuccess || dev_alloc(&e->d_Kacc, (size_t)nbf * nbf) != hipSuccess) {
+      dev_alloc(&e->d_Jacc, 2 * (size_t)nbf * nbf) != hipSuccess ||
+      dev_alloc(&e->d_Kacc, 2 * (size_t)nbf * nbf) != hipSuccess) {
     delete e;
     return fail(PQ_E_ALLOC, "device allocation failed");
   }
@@ -886,22 +889,30 @@ int pq_engine_class_stats(const pq_engine* eng, int c, int l[4], uint64_t counts
   return PQ_OK;
 }
 
-static int fock_jk_impl(pq_engine* eng, const double* D_dev, double* J_dev, double* K_dev, int precision,
-                        void* stream, float* ms);
+static int fock_jk_impl(pq_engine* eng, int nd, const double* const* Ds, double* const* Js, double* const* Ks,
+                        int precision, void* stream, float* ms);
 
 int pq_fock_jk(pq_engine* eng, const double* D_dev, double* J_dev, double* K_dev, int precision, void* stream) {
-  return fock_jk_impl(eng, D_dev, J_dev, K_dev, precision, stream, nullptr);
+  return fock_jk_impl(eng, 1, &D_dev, &J_dev, &K_dev, precision, stream, nullptr);
 }
 
 int pq_fock_jk_timed(pq_engine* eng, const double* D_dev, double* J_dev, double* K_dev, int precision,
                      void* stream, float* ms) {
   if (!ms) return fail(PQ_E_ARG, "null timing array");
-  return fock_jk_impl(eng, D_dev, J_dev, K_dev, precision, stream, ms);
+  return fock_jk_impl(eng, 1, &D_dev, &J_dev, &K_dev, precision, stream, ms);
 }
 
-static int fock_jk_impl(pq_engine* eng, const double* D_dev, double* J_dev, double* K_dev, int precision,
-                        void* stream, float* ms) {
-  if (!eng || !D_dev) return fail(PQ_E_ARG, "null argument");
+int pq_fock_jk_multi(pq_engine* eng, int ndens, const double* const* D_dev, double* const* J_dev,
+                     double* const* K_dev, int precision, void* stream) {
+  if (!D_dev || !J_dev || !K_dev) return fail(PQ_E_ARG, "null pointer array");
+  if (ndens < 1 || ndens > 2) return fail(PQ_E_ARG, "ndens must be 1 or 2");
+  if (ndens == 2 && precision != PQ_PREC_FP64) return fail(PQ_E_ARG, "two-density builds are FP64");
+  return fock_jk_impl(eng, ndens, D_dev, J_dev, K_dev, precision, stream, nullptr);
+}
+
+static int fock_jk_impl(pq_engine* eng, int nd, const double* const* Ds, double* const* Js, double* const* Ks,
+                        int precision, void* stream, float* ms) {
+  if (!eng || !Ds[0] || (nd == 2 && !Ds[1])) return fail(PQ_E_ARG, "null argument");
   if (!eng->work_ok) return fail(PQ_E_STATE, "pq_build_pairs must be called before pq_fock_jk");
   if (precision != PQ_PREC_FP64 && precision != PQ_PREC_MIXED) return fail(PQ_E_ARG, "bad precision");
   if (precision == PQ_PREC_MIXED && !(eng->fp32cut > 0.0))
@@ -930,8 +941,16 @@ static int fock_jk_impl(pq_engine* eng, const double* D_dev, double* J_dev, doub
     break;                                         \
   }
     if (ms) PQ_TRY(mark(st));
-    PQ_TRY(launch_fill(eng->d_Jacc, nn, 0.0, st));
-    PQ_TRY(launch_fill(eng->d_Kacc, nn, 0.0, st));
+    // two densities: stacked copies, digested by the MODE_JK2 kernels
+    const double* D_dev = Ds[0];
+    if (nd == 2) {
+      if (!eng->d_D2) PQ_TRY(dev_alloc(&eng->d_D2, 2 * (size_t)nn));
+      PQ_TRY(hipMemcpyAsync(eng->d_D2, Ds[0], nn * sizeof(double), hipMemcpyDeviceToDevice, st));
+      PQ_TRY(hipMemcpyAsync(eng->d_D2 + nn, Ds[1], nn * sizeof(double), hipMemcpyDeviceToDevice, st));
+      D_dev = eng->d_D2;
+    }
+    PQ_TRY(launch_fill(eng->d_Jacc, nd * nn, 0.0, st));
+    PQ_TRY(launch_fill(eng->d_Kacc, nd * nn, 0.0, st));
     const int nside = eng->nside;
     if (nside > 0) {
       // fork: class kernels spread over the side streams, joined before the
@@ -959,7 +978,8 @@ static int fock_jk_impl(pq_engine* eng, const double* D_dev, double* J_dev, doub
         hipStream_t ks = nside > 0 ? eng->side[next] : st;
         if (nside > 0) next = (next + 1) % nside;
         if (ms) PQ_TRY(mark(ks));
-        PQ_TRY(launch_eri_class(la, lb, lc, ld, MODE_JK, prec, eng->view(), prec ? c.d_t32 : c.d_t64,
+        PQ_TRY(launch_eri_class(la, lb, lc, ld, nd == 2 ? MODE_JK2 : MODE_JK, prec, eng->view(),
+                                prec ? c.d_t32 : c.d_t64,
                                 (int)tl.size(), eng->d_sorted[c.Y], D_dev, eng->d_Jacc, eng->d_Kacc, nullptr,
                                 nullptr, ks, prec ? c.d_g32 : c.d_g64, (int)(prec ? c.g32 : c.g64).size()));
         if (ms) {
@@ -974,7 +994,9 @@ static int fock_jk_impl(pq_engine* eng, const double* D_dev, double* J_dev, doub
       if (err == hipSuccess) err = hipStreamWaitEvent(st, eng->join_ev[i], 0);
     }
     PQ_TRY(err);
-    PQ_TRY(launch_symmetrize(eng->nbf, eng->d_Jacc, eng->d_Kacc, J_dev, K_dev, 1.0, 1.0, st));
+    for (int d = 0; d < nd && rc == PQ_OK; ++d)
+      PQ_TRY(launch_symmetrize(eng->nbf, eng->d_Jacc + d * nn, eng->d_Kacc + d * nn, Js[d], Ks[d], 1.0, 1.0, st));
+    if (rc != PQ_OK) break;
     if (ms) {
       PQ_TRY(mark(st));
       PQ_TRY(hipStreamSynchronize(st));

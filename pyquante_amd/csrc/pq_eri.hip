@@ -11,7 +11,8 @@ namespace pq {
                                               const int* kets, const double* D, double* Jacc, double* Kacc, \
                                               double* ints, double* qout, hipStream_t st, const int2* groups, \
                                               int ngroups);
-#define PQ_DECL(A, B, C_, D_) PQ_DECL1(A, B, C_, D_, 0) PQ_DECL1(A, B, C_, D_, 1) PQ_DECL1(A, B, C_, D_, 2)
+#define PQ_DECL(A, B, C_, D_) \
+  PQ_DECL1(A, B, C_, D_, 0) PQ_DECL1(A, B, C_, D_, 1) PQ_DECL1(A, B, C_, D_, 2) PQ_DECL1(A, B, C_, D_, 5)
 PQ_DECL(0, 0, 0, 0)
 PQ_DECL(1, 0, 0, 0)
 PQ_DECL(1, 0, 1, 0)
@@ -242,6 +243,8 @@ hipError_t launch_eri_class(int la, int lb, int lc, int ld, int mode, int fp32, 
       return launch_cls_##A##B##C_##D_##_m1(ev, tiles, ntiles, kets, D, Jacc, Kacc, ints, qout, st, groups, ngroups); \
     if (mode == MODE_STORE)                                                                        \
       return launch_cls_##A##B##C_##D_##_m2(ev, tiles, ntiles, kets, D, Jacc, Kacc, ints, qout, st, nullptr, 0); \
+    if (mode == MODE_JK2)                                                                          \
+      return launch_cls_##A##B##C_##D_##_m5(ev, tiles, ntiles, kets, D, Jacc, Kacc, ints, qout, st, groups, ngroups); \
     return launch_diag_##A##B##C_##D_(ev, tiles, ntiles, kets, D, Jacc, Kacc, ints, qout, st);      \
   }
   PQ_CASE(0, 0, 0, 0)

@@ -14,6 +14,8 @@
 //
 // Modes: MODE_JK digests the block into J and K (the reference's
 // getJ/getK/get2JmK, Ints.py:195-238, done directly from the integrals),
+// MODE_JK2 digests one pass of integrals into J and K of two densities (the
+// four getJ/getK calls of an unrestricted iteration, hartree_fock.py:233-236),
 // MODE_STORE writes the reference's flat unique-ERI array (Ints.py:146-167,
 // index cints.c:413-425), MODE_DIAG produces Schwarz bounds from (ab|ab).
 #pragma once
@@ -302,7 +304,7 @@ __device__ __forceinline__ void lane_body(const EngineView& ev, const Tile& tl, 
 #pragma unroll
     for (int c = 0; c < NO; ++c) out[c] = (T)0;
   }
-  if constexpr (MODE == MODE_JK) {
+  if constexpr (mode_is_jk(MODE)) {
     // ket-primitive chunks of one quartet (lanes slot + j*nk) are summed into
     // the chunk-0 lane, which alone digests
     if (ks > 1) {
@@ -316,7 +318,14 @@ __device__ __forceinline__ void lane_body(const EngineView& ev, const Tile& tl, 
       active = active && kc == 0;
     }
     const double f = active ? quartet_degeneracy(bp, kp, bra, ket) : 0.0;
-    digest_lane<LA, LB, LC, LD, A0, NAS, B0, NBS, T>(ev, bp, kp, out, f, active, D, Jacc, Kacc, krow, jsum);
+    // one block of integrals, digested once per density (stacked n x n
+    // matrices; LDS K rows and J sums stacked the same way)
+    const size_t nn = (size_t)ev.nbf * ev.nbf;
+#pragma unroll
+    for (int d = 0; d < mode_ndens(MODE); ++d)
+      digest_lane<LA, LB, LC, LD, A0, NAS, B0, NBS, T>(
+          ev, bp, kp, out, f, active, D + d * nn, Jacc + d * nn, Kacc + d * nn,
+          krow ? krow + d * (C::NA + C::NB) * ev.nbf : nullptr, jsum ? jsum + d * 64 * C::NC * C::ND : nullptr);
   } else if constexpr (MODE == MODE_STORE) {
     if (!active) return;
     const bool same = bra == ket;
@@ -400,8 +409,9 @@ __global__ __launch_bounds__(256) void k_eri_group(EngineView ev, const Tile* __
     float* t32 = reinterpret_cast<float*>(pq_lds);
     for (int i = threadIdx.x; i < 2 * NT; i += blockDim.x) t32[i] = rys_coef32[rys_off32[C::NR - 1] + i];
   }
+  constexpr int ND = mode_ndens(MODE);
   double* krow = pq_lds + NT;
-  for (int i = threadIdx.x; i < (C::NA + C::NB) * n; i += blockDim.x) krow[i] = 0.0;
+  for (int i = threadIdx.x; i < ND * (C::NA + C::NB) * n; i += blockDim.x) krow[i] = 0.0;
   __syncthreads();
   for (int tt = wv; tt < g.y; tt += 4) {
     Tile tl = ld_uniform(tiles + g.x + tt);
@@ -411,18 +421,20 @@ __global__ __launch_bounds__(256) void k_eri_group(EngineView ev, const Tile* __
     tl.prim = __builtin_amdgcn_readfirstlane(tl.prim);
     const TileLane tn = tile_lane(tl.nket, lane);
     const int ket = kets[tl.ket0 + tn.slot];
-    lane_body<LA, LB, LC, LD, T, MODE_JK, 0, C::NA, 0, C::NB, (NT > 0)>(ev, tl, tn.active, tl.bra, ket, D, Jacc,
+    lane_body<LA, LB, LC, LD, T, MODE, 0, C::NA, 0, C::NB, (NT > 0)>(ev, tl, tn.active, tl.bra, ket, D, Jacc,
                                                                        Kacc, nullptr, nullptr, krow, nullptr, rtab,
                                                                        tn.kc, tn.ks, tn.nk);
   }
   __syncthreads();
   const PairInfo bp = ev.pairs[tiles[g.x].bra];
-  for (int i = threadIdx.x; i < (C::NA + C::NB) * n; i += blockDim.x) {
+  const int rows = (C::NA + C::NB) * n;
+  for (int i = threadIdx.x; i < ND * rows; i += blockDim.x) {
     const double v = krow[i];
     if (v != 0.0) {
-      const int r = i / n, col = i - r * n;
+      const int d = i / rows, ir = i - d * rows;
+      const int r = ir / n, col = ir - r * n;
       const int grow = r < C::NA ? bp.bfA + r : bp.bfB + (r - C::NA);
-      unsafeAtomicAdd(&Kacc[grow * n + col], v);
+      unsafeAtomicAdd(&Kacc[(size_t)d * n * n + grow * n + col], v);
     }
   }
 }
@@ -440,10 +452,11 @@ __global__ __launch_bounds__(256) void k_eri_lane(EngineView ev, const Tile* __r
   const int t = __builtin_amdgcn_readfirstlane(lds_mode ? (int)blockIdx.x : (int)(blockIdx.x * 4 + wv));
   double* krow = nullptr;
   double* jsum = nullptr;
+  constexpr int ND = mode_ndens(MODE);
   if (lds_mode) {
     krow = pq_lds;
-    if (C::SPLIT) jsum = pq_lds + (C::NA + C::NB) * n;
-    const int tot = (C::NA + C::NB) * n + (C::SPLIT ? 64 * C::NC * C::ND : 0);
+    if (C::SPLIT) jsum = pq_lds + ND * (C::NA + C::NB) * n;
+    const int tot = ND * ((C::NA + C::NB) * n + (C::SPLIT ? 64 * C::NC * C::ND : 0));
     for (int i = threadIdx.x; i < tot; i += blockDim.x) pq_lds[i] = 0.0;
     __syncthreads();
   } else if (t >= ntiles) {
@@ -481,26 +494,29 @@ __global__ __launch_bounds__(256) void k_eri_lane(EngineView ev, const Tile* __r
   __syncthreads();
   // flush the K rows (coalesced) and the ket-side J accumulator
   const PairInfo bp = ev.pairs[tl.bra];
-  for (int i = threadIdx.x; i < (C::NA + C::NB) * n; i += blockDim.x) {
+  const int rows = (C::NA + C::NB) * n;
+  for (int i = threadIdx.x; i < ND * rows; i += blockDim.x) {
     const double v = krow[i];
     if (v != 0.0) {
-      const int r = i / n, col = i - r * n;
+      const int d = i / rows, ir = i - d * rows;
+      const int r = ir / n, col = ir - r * n;
       const int grow = r < C::NA ? bp.bfA + r : bp.bfB + (r - C::NA);
-      unsafeAtomicAdd(&Kacc[grow * n + col], v);
+      unsafeAtomicAdd(&Kacc[(size_t)d * n * n + grow * n + col], v);
     }
   }
   if constexpr (C::SPLIT != 0) {
     constexpr int NCD = C::NC * C::ND;
-    for (int i = threadIdx.x; i < 64 * NCD; i += blockDim.x) {
-      const int l = i / NCD, cd = i - l * NCD;
-      const TileLane tl_l = tile_lane(tl.nket, l);
-      if (!tl_l.active) break;
-      const double v = jsum[i];
-      if (v != 0.0) {
-        const PairInfo kq = ev.pairs[kets[tl.ket0 + tl_l.slot]];
-        unsafeAtomicAdd(&Jacc[(kq.bfA + cd / C::ND) * n + kq.bfB + cd % C::ND], v);
+    for (int d = 0; d < ND; ++d)
+      for (int i = threadIdx.x; i < 64 * NCD; i += blockDim.x) {
+        const int l = i / NCD, cd = i - l * NCD;
+        const TileLane tl_l = tile_lane(tl.nket, l);
+        if (!tl_l.active) break;
+        const double v = jsum[d * 64 * NCD + i];
+        if (v != 0.0) {
+          const PairInfo kq = ev.pairs[kets[tl.ket0 + tl_l.slot]];
+          unsafeAtomicAdd(&Jacc[(size_t)d * n * n + (kq.bfA + cd / C::ND) * n + kq.bfB + cd % C::ND], v);
+        }
       }
-    }
   }
 }
 
@@ -513,8 +529,9 @@ static hipError_t launch_one(const EngineView& ev, const Tile* tiles, int ntiles
                              hipStream_t st, const int2* groups, int ngroups) {
   if (ntiles <= 0) return hipSuccess;
   using C = Cls<LA, LB, LC, LD>;
-  const size_t lds = ((size_t)(C::NA + C::NB) * ev.nbf + (C::SPLIT ? 64 * C::NC * C::ND : 0)) * sizeof(double);
-  if constexpr (C::SPLIT == 0 && MODE == MODE_JK) {
+  const size_t lds = mode_ndens(MODE) * ((size_t)(C::NA + C::NB) * ev.nbf + (C::SPLIT ? 64 * C::NC * C::ND : 0)) *
+                     sizeof(double);
+  if constexpr (C::SPLIT == 0 && mode_is_jk(MODE)) {
     const size_t glds = lds + kLdsTableDoubles<C::NR, T> * sizeof(double);
     if (groups && ngroups > 0 && glds <= 65536) {
       hipLaunchKernelGGL((k_eri_group<LA, LB, LC, LD, T, MODE>), dim3(ngroups), dim3(256), glds, st, ev, tiles,
@@ -522,7 +539,7 @@ static hipError_t launch_one(const EngineView& ev, const Tile* tiles, int ntiles
       return hipGetLastError();
     }
   }
-  if (MODE == MODE_JK && lds <= 65536) {
+  if (mode_is_jk(MODE) && lds <= 65536) {
     const int waves = C::SPLIT ? kWaveSplit : 1;
     const int chunks = (C::NSPLIT + waves - 1) / waves;
     hipLaunchKernelGGL((k_eri_lane<LA, LB, LC, LD, T, MODE>), dim3(ntiles, chunks), dim3(64 * waves), lds, st, ev,

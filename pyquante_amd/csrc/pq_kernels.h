@@ -6,7 +6,11 @@
 
 namespace pq {
 
-enum { MODE_JK = 0, MODE_STORE = 1, MODE_DIAG = 2 };
+enum { MODE_JK = 0, MODE_STORE = 1, MODE_DIAG = 2, MODE_JK2 = 3 };
+// J/K modes and the number of densities they digest (MODE_JK2: two stacked
+// n x n densities into two stacked J/K accumulators)
+constexpr bool mode_is_jk(int m) { return m == MODE_JK || m == MODE_JK2; }
+constexpr int mode_ndens(int m) { return m == MODE_JK2 ? 2 : 1; }
 
 // classes with at most this many Cartesian integrals run one lane per quartet
 constexpr int kLaneMaxComp = 81;

@@ -190,6 +190,24 @@ class Engine(object):
                              _stream()))
         return J, K
 
+    def fock_jk_multi(self, Ds):
+        """[(J, K) for D in Ds] for one or two symmetric densities from one
+        pass over the integrals (pq_fock_jk_multi, FP64)."""
+        if self.cut is None:
+            self.build()
+        if self.fp32cut:
+            self.build(self.cut, 0.0)
+        Ds = [self._dtensor(D) for D in Ds]
+        if not 1 <= len(Ds) <= 2:
+            raise ValueError("one or two densities")
+        Js = [torch.empty_like(D) for D in Ds]
+        Ks = [torch.empty_like(D) for D in Ds]
+        arr = ctypes.c_void_p * len(Ds)
+        check(lib.pq_fock_jk_multi(self.handle, len(Ds), arr(*[D.data_ptr() for D in Ds]),
+                                   arr(*[J.data_ptr() for J in Js]), arr(*[K.data_ptr() for K in Ks]),
+                                   _abi.PQ_PREC_FP64, _stream()))
+        return list(zip(Js, Ks))
+
     def capture_fock_jk(self, D, precision=_abi.PQ_PREC_FP64):
         """Captures one direct Fock build into a HIP graph (FockGraph)."""
         return FockGraph(self, D, precision)

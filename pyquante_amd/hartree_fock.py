@@ -145,3 +145,60 @@ def rhf(atoms, **opts):
             break
         eold = energy
     return energy, orbe.cpu().numpy(), orbs.cpu().numpy()
+
+
+def uhf(atoms, **opts):
+    """uhf(atoms, **opts) -> (energy, (orbea, orbeb), (orbsa, orbsb))
+    (hartree_fock.py:150-249): unrestricted Hartree-Fock with the reference's
+    density averaging (DoAveraging, averaging = 0.5).  Each iteration's four
+    getJ/getK calls are one two-density pass over the integrals on the GPU
+    (pq_fock_jk_multi).  The reference's finite-temperature occupations
+    (ETemp) are not supported."""
+    if opts.get("ETemp", False):
+        raise NotImplementedError("ETemp occupations are not supported")
+    ConvCriteria = opts.get("ConvCriteria", 1e-5)
+    MaxIter = opts.get("MaxIter", 40)
+    DoAveraging = opts.get("DoAveraging", True)
+    averaging = opts.get("averaging", 0.5)
+    bfs = opts.get("bfs", None)
+    if not bfs:
+        bfs = getbasis(atoms, opts.get("basis_data", None))
+    nalpha, nbeta = atoms.get_alphabeta()
+    engine = Engine(bfs, atoms)
+    engine.build(opts.get("schwarz_cut", 1e-12), 0.0)
+    dev = engine.device
+    Sd, Td, Vd = engine.get1ints()
+    S, h = Sd, Td + Vd
+    X = chol_orth(S)
+    orbs = opts.get("orbs", None)
+    if orbs is not None:
+        orbsa, orbsb = _t(orbs[0], dev), _t(orbs[1], dev)
+        orbea = orbeb = None
+    else:
+        orbe, orbs = geigh(h, S, X)
+        orbea = orbeb = orbe
+        orbsa = orbsb = orbs
+    enuke = atoms.get_enuke()
+    eold = 0.0
+    energy = 0.0
+    Da0 = Db0 = None
+    for i in range(MaxIter):
+        Da = mkdens(orbsa, 0, nalpha)
+        Db = mkdens(orbsb, 0, nbeta)
+        if DoAveraging:
+            if i:
+                Da = averaging * Da + (1 - averaging) * Da0
+                Db = averaging * Db + (1 - averaging) * Db0
+            Da0, Db0 = Da, Db
+        (Ja, Ka), (Jb, Kb) = engine.fock_jk_multi([0.5 * (Da + Da.T), 0.5 * (Db + Db.T)])
+        Fa = h + Ja + Jb - Ka
+        Fb = h + Ja + Jb - Kb
+        orbea, orbsa = geigh(Fa, S, X)
+        orbeb, orbsb = geigh(Fb, S, X)
+        energy = (get_energy(h, Fa, Da) + get_energy(h, Fb, Db)) / 2 + enuke
+        logger.debug("%d %f", i, energy)
+        if abs(energy - eold) < ConvCriteria:
+            break
+        eold = energy
+    return energy, (orbea.cpu().numpy(), orbeb.cpu().numpy()), (orbsa.cpu().numpy(), orbsb.cpu().numpy())
+

@@ -91,6 +91,11 @@ class Molecule(object):
             raise ValueError("inconsistent charge/multiplicity")
         return nclosed, nopen
 
+    def get_alphabeta(self):
+        """(nalpha, nbeta) (Molecule.py:199-201)."""
+        nclosed, nopen = self.get_closedopen()
+        return nclosed + nopen, nclosed
+
     def coords(self):
         return np.array([a.r for a in self.atoms])
 
