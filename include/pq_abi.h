@@ -162,6 +162,17 @@ int pq_get2ints(pq_engine *eng, double *out_dev, uint64_t len, void *stream);
  * uses chgp's EPS = 3e-7 Boys function, 0 the full-precision one. */
 int pq_get2ints_hgp(pq_engine *eng, double *out_dev, uint64_t len, int compat_nr, void *stream);
 
+/* Shell-quartet blocks in the libint layout, the interface of the
+ * reference's libint branch (Src/lib/clibint.h:37-52 shell_compute_eri,
+ * Src/PyQuante/clibint.pyx:27-60, Ints.py:125-143): for each of nq quartets
+ * of shell indices (shells_host[4q..4q+3], engine shell order) a row-major
+ * block ((i*nb + j)*nc + k)*nd + l over the shells' functions in canonical
+ * Cartesian order, blocks concatenated in request order into out_dev
+ * (out_len doubles available).  The engine keeps the flat unique array it
+ * gathers from (computed on first use, dropped with the geometry). */
+int pq_shell_quartet_blocks(pq_engine *eng, int nq, const int *shells_host, double *out_dev, uint64_t out_len,
+                            void *stream);
+
 /* Ints.getJ / getK / get2JmK (PyQuante/Ints.py:195-238) in direct mode:
  * J_ij = sum_kl (ij|kl) D_kl and K_ij = sum_kl (ik|jl) D_kl from integrals
  * computed on the fly over the screened work lists.  D_dev is a symmetric

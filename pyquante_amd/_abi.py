@@ -58,6 +58,7 @@ _proto("pq_engine_autotune", [_VP, _VP, _I, _VP, _PI])
 _proto("pq_engine_stats", [_VP, _PU64, _PD])
 _proto("pq_get2ints", [_VP, _VP, _U64, _VP])
 _proto("pq_get2ints_hgp", [_VP, _VP, _U64, _I, _VP])
+_proto("pq_shell_quartet_blocks", [_VP, _I, _PI, _VP, _U64, _VP])
 _proto("pq_fock_jk", [_VP, _VP, _VP, _VP, _I, _VP])
 _proto("pq_fock_jk_multi", [_VP, _I, _c.POINTER(_VP), _c.POINTER(_VP), _c.POINTER(_VP), _I, _VP])
 _proto("pq_engine_class_stats", [_VP, _I, _PI, _PU64, _PD])

@@ -106,11 +106,13 @@ class BasisSet(object):
             basis_data = get_basis_data(basis_data)
         self.bfs = []
         self.shells = []
-        for atom in atoms:
+        self.atoms = list(atoms)
+        for atom in self.atoms:
             for sym, prims in basis_data[atom.atno]:
                 if sym not in sym2powerlist:
                     raise ValueError("only S, P and D shells are supported (got %s)" % sym)
                 sh = Shell(sym, atom.pos(), atom.atid, len(self.bfs), prims)
+                sh.basis, sh.index = self, len(self.shells)
                 for power in sym2powerlist[sym]:
                     bf = CGBF(atom.pos(), power, atom.atid)
                     for e, c in prims:

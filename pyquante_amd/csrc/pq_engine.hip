@@ -119,6 +119,7 @@ struct pq_engine {
   double* d_Jacc = nullptr;  // 2 n^2: room for two densities
   double* d_Kacc = nullptr;
   double* d_D2 = nullptr;    // two stacked densities (allocated on first use)
+  double* d_flat = nullptr;  // cached flat unique-ERI array for shell-quartet blocks
   double* d_gacc = nullptr;  // private gradient copies
   size_t gacc_n = 0;
 
@@ -194,6 +195,7 @@ struct pq_engine {
     dev_free(d_Jacc);
     dev_free(d_Kacc);
     dev_free(d_D2);
+    dev_free(d_flat);
     dev_free(d_gacc);
   }
 };
@@ -665,6 +667,7 @@ int pq_engine_set_geometry(pq_engine* eng, const double* atom_xyz_host) {
   PQ_HIP(hipMemcpy(eng->d_atom_xyz, atom_xyz_host, 3 * eng->natom * sizeof(double), hipMemcpyHostToDevice));
   eng->free_pairs();
   eng->free_functions();
+  dev_free(eng->d_flat);
   return PQ_OK;
 }
 
@@ -1042,6 +1045,49 @@ int pq_unpack_eri(int nbf, const double* ints_dev, uint64_t len, int mu0, int nm
   if (len != npair * (npair + 1) / 2) return fail(PQ_E_ARG, "integral array length mismatch");
   if (mu0 < 0 || nmu < 0 || mu0 + nmu > nbf) return fail(PQ_E_ARG, "row range out of bounds");
   PQ_HIP(launch_unpack_eri(nbf, ints_dev, mu0, nmu, out_dev, (hipStream_t)stream));
+  return PQ_OK;
+}
+
+int pq_shell_quartet_blocks(pq_engine* eng, int nq, const int* shells_host, double* out_dev, uint64_t out_len,
+                            void* stream) {
+  if (!eng || (nq > 0 && (!shells_host || !out_dev))) return fail(PQ_E_ARG, "null argument");
+  if (nq < 0) return fail(PQ_E_ARG, "negative quartet count");
+  if (nq == 0) return PQ_OK;
+  std::vector<int> first(4 * (size_t)nq), cnt(4 * (size_t)nq);
+  std::vector<int64_t> off(nq);
+  uint64_t tot = 0;
+  for (int q = 0; q < nq; ++q) {
+    uint64_t sz = 1;
+    for (int s = 0; s < 4; ++s) {
+      const int sh = shells_host[4 * q + s];
+      if (sh < 0 || sh >= eng->nshell) return fail(PQ_E_ARG, "shell index out of range");
+      first[4 * q + s] = eng->sh_bf[sh];
+      cnt[4 * q + s] = ncart(eng->sh_l[sh]);
+      sz *= cnt[4 * q + s];
+    }
+    off[q] = (int64_t)tot;
+    tot += sz;
+  }
+  if (out_len < tot) return fail(PQ_E_ARG, "output buffer too small for the requested blocks");
+  hipStream_t st = (hipStream_t)stream;
+  const uint64_t n = eng->nbf, npair = n * (n + 1) / 2, len = npair * (npair + 1) / 2;
+  if (!eng->d_flat) {
+    PQ_HIP(dev_alloc(&eng->d_flat, len));
+    const int rc = pq_get2ints(eng, eng->d_flat, len, stream);
+    if (rc) {
+      dev_free(eng->d_flat);
+      return rc;
+    }
+  }
+  DevBuf b_first, b_cnt, b_off;
+  PQ_HIP(hipMalloc(&b_first.p, first.size() * sizeof(int)));
+  PQ_HIP(hipMalloc(&b_cnt.p, cnt.size() * sizeof(int)));
+  PQ_HIP(hipMalloc(&b_off.p, off.size() * sizeof(int64_t)));
+  PQ_HIP(hipMemcpy(b_first.p, first.data(), first.size() * sizeof(int), hipMemcpyHostToDevice));
+  PQ_HIP(hipMemcpy(b_cnt.p, cnt.data(), cnt.size() * sizeof(int), hipMemcpyHostToDevice));
+  PQ_HIP(hipMemcpy(b_off.p, off.data(), off.size() * sizeof(int64_t), hipMemcpyHostToDevice));
+  PQ_HIP(launch_shell_blocks(nq, eng->d_flat, (int*)b_first.p, (int*)b_cnt.p, (int64_t*)b_off.p, out_dev, st));
+  PQ_HIP(hipStreamSynchronize(st));
   return PQ_OK;
 }
 

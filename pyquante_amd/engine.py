@@ -208,6 +208,27 @@ class Engine(object):
                                    _abi.PQ_PREC_FP64, _stream()))
         return list(zip(Js, Ks))
 
+    def shell_quartets(self, quartets):
+        """Blocks of the shell quartets [(a, b, c, d), ...] (engine shell
+        indices) in the libint layout (pq_shell_quartet_blocks): a list of
+        numpy arrays of shape (na, nb, nc, nd)."""
+        q = np.ascontiguousarray(np.asarray(quartets, dtype=np.int32).reshape(-1, 4))
+        if len(q) == 0:
+            return []
+        nf = np.array([(L + 1) * (L + 2) // 2 for L in self.shell_l], dtype=np.int64)
+        if q.min() < 0 or q.max() >= len(nf):
+            raise ValueError("shell index out of range")
+        sizes = np.prod(nf[q], axis=1)
+        out = torch.empty(int(sizes.sum()), dtype=torch.float64, device=self.device)
+        check(lib.pq_shell_quartet_blocks(self.handle, len(q), pi(q.reshape(-1)), ctypes.c_void_p(out.data_ptr()),
+                                          ctypes.c_uint64(out.numel()), _stream()))
+        host = out.cpu().numpy()
+        blocks, o = [], 0
+        for t, sz in enumerate(sizes):
+            blocks.append(host[o:o + sz].reshape(tuple(nf[q[t]])))
+            o += sz
+        return blocks
+
     def capture_fock_jk(self, D, precision=_abi.PQ_PREC_FP64):
         """Captures one direct Fock build into a HIP graph (FockGraph)."""
         return FockGraph(self, D, precision)

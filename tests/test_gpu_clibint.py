@@ -1,0 +1,53 @@
+"""GPU: the libint-style shell-quartet interface (clibint drop-in) against
+the compiled reference's golden ERIs and the oracle."""
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+def _h2o():
+    from pyquante_amd.basis import BasisSet
+    from pyquante_amd.molecule import Molecule
+    r = 1 / 0.52918
+    mol = Molecule("h2o", [(8, (0, 0, 0)), (1, (r, 0, 0)), (1, (0, r, 0))])
+    return BasisSet(mol, "6-31g**")
+
+
+def test_libint_get2ints_layout(golden):
+    from pyquante_amd import clibint
+    basis = _h2o()
+    n = len(basis.bfs)
+    Ints = clibint.get2ints(basis)
+    assert len(Ints) == n ** 4
+    ref = golden("h2o_631ss")["crys"]
+    assert np.max(np.abs(Ints[:len(ref)] - ref)) < 1e-12
+    assert not np.any(Ints[len(ref):])
+
+
+def test_shell_blocks_row_major_any_order():
+    from pyquante_amd import clibint
+    basis = _h2o()
+    bfs, sh = basis.bfs, basis.shells
+    d = [s for s in sh if s.L == 2][0]
+    p = [s for s in sh if s.L == 1][-1]
+    s0 = [s for s in sh if s.L == 0][2]
+    for quartet in ((d, p, s0, d), (s0, d, d, p), (p, p, d, s0)):
+        block = clibint._engine(basis).shell_quartets([[x.index for x in quartet]])[0]
+        assert block.shape == tuple(x.nfunc for x in quartet)
+        for i in range(block.shape[0]):
+            for j in range(block.shape[1]):
+                for k in range(block.shape[2]):
+                    for l in range(block.shape[3]):
+                        f = [quartet[0].first_bf + i, quartet[1].first_bf + j, quartet[2].first_bf + k,
+                             quartet[3].first_bf + l]
+                        ref = O.contr_coulomb(*[bfs[t] for t in f])
+                        assert abs(block[i, j, k, l] - ref) < 1e-12
+        # the reference's placement: every integral at ijkl2intindex of its functions
+        Ints = np.zeros(len(bfs) ** 4)
+        clibint.shell_compute_eri(*quartet, Ints)
+        for i, j, k, l in np.ndindex(block.shape):
+            f = [quartet[0].first_bf + i, quartet[1].first_bf + j, quartet[2].first_bf + k, quartet[3].first_bf + l]
+            assert Ints[O.ijkl2intindex(*f)] == block[i, j, k, l]
