@@ -39,9 +39,12 @@ class Wavefunction(object):
         return (c * self.orbe[:self.nclosed]) @ c.T
 
 
-def scf(engine, mol, ConvCriteria=1e-10, MaxIter=100, DoAveraging=True, precision=0):
+def scf(engine, mol, ConvCriteria=1e-10, MaxIter=100, DoAveraging=True, precision=0, graph=False):
     """RHF on an engine (hartree_fock.rhf semantics, DIIS on by default here
-    for tight convergence).  Returns (E, orbe, orbs, D, W) as device tensors."""
+    for tight convergence).  Returns (E, orbe, orbs, D, W) as device tensors.
+    graph=True replays each iteration's Fock build from one HIP graph
+    captured on the first iteration (Engine.capture_fock_jk); the
+    eigensolve stays outside the graph."""
     from .hartree_fock import DIIS
     S, T, V = engine.get1ints()
     h = T + V
@@ -52,9 +55,15 @@ def scf(engine, mol, ConvCriteria=1e-10, MaxIter=100, DoAveraging=True, precisio
     avg = DIIS(S) if DoAveraging else None
     eold = 0.0
     energy = 0.0
+    fg = None
     for _ in range(MaxIter):
         D = mkdens(C, 0, nocc)
-        J, K = engine.fock_jk(D, precision=precision)
+        if graph:
+            if fg is None:
+                fg = engine.capture_fock_jk(D, precision=precision)
+            J, K = fg(D)
+        else:
+            J, K = engine.fock_jk(D, precision=precision)
         F = h + 2.0 * J - K
         if avg is not None:
             F = avg.getF(F, D)

@@ -91,3 +91,15 @@ def test_graph_replay_matches_eager(streams):
         Je, _ = eng.fock_jk(D2)
         torch.cuda.synchronize()
         assert torch.max(torch.abs(Je - J2)).item() < 1e-11
+
+
+def test_scf_with_graph_replayed_fock_builds():
+    from pyquante_amd.force import scf
+    eng = _engine((2, 1, 1), 5)
+    eng.build(1e-12, 0.0)
+    from pyquante_amd.molecule import water_cluster
+    mol = water_cluster((2, 1, 1), seed=5)
+    e0 = scf(eng, mol, ConvCriteria=1e-10)[0]
+    e1 = scf(eng, mol, ConvCriteria=1e-10, graph=True)[0]
+    assert abs(e0 - e1) < 1e-9
+
