@@ -94,6 +94,25 @@ int pq_hgp_contr_coulomb_batch(int n, const int *nprim, const int *poff, int npr
 int pq_engine_create(int nshell, const int *shell_l, const int *shell_nprim, const int *shell_poff,
                      const int *shell_atom, const double *exps, const double *coefs, int natom,
                      const double *atom_xyz, const double *atom_z, pq_engine **out);
+
+/* A batch of independent molecules in one engine (one launch per class over
+ * all molecules' quartets): as pq_engine_create, plus atom_mol[a] = molecule
+ * of atom a (molecules numbered 0, 1, ... with contiguous atoms; shells
+ * ordered by molecule).  Pairs and quartets never mix molecules.  Matrices
+ * (D, J, K, S, T, V, W) are packed per molecule: molecule m's n_m x n_m
+ * block, row-major, starts at element mol_moff[m] (pq_engine_layout).
+ * pq_get2ints, pq_get2ints_hgp and pq_shell_quartet_blocks need a
+ * single-molecule engine.  No reference counterpart (the reference handles
+ * one molecule per call). */
+int pq_engine_create_batch(int nshell, const int *shell_l, const int *shell_nprim, const int *shell_poff,
+                           const int *shell_atom, const double *exps, const double *coefs, int natom,
+                           const double *atom_xyz, const double *atom_z, const int *atom_mol, pq_engine **out);
+
+/* Packed matrix layout: number of molecules, elements of one packed matrix,
+ * and per molecule (arrays of nmol entries, each optional) its function
+ * count, block offset, first atom and atom count. */
+int pq_engine_layout(const pq_engine *eng, int *nmol, int64_t *msize, int *mol_nbf, int64_t *mol_moff,
+                     int *mol_first_atom, int *mol_natom);
 int pq_engine_destroy(pq_engine *eng);
 /* Number of basis functions. */
 int pq_engine_nbf(const pq_engine *eng, int *nbf);

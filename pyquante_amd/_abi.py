@@ -47,6 +47,8 @@ _proto("pq_rys_roots_batch", [_I, _I, _PD, _PD, _PD])
 _proto("pq_contr_coulomb_batch", [_I, _PI, _PI, _I, _PD, _PD, _PD, _PD, _PI, _PD])
 _proto("pq_hgp_contr_coulomb_batch", [_I, _PI, _PI, _I, _PD, _PD, _PD, _PD, _PI, _PI, _I, _PD])
 _proto("pq_engine_create", [_I, _PI, _PI, _PI, _PI, _PD, _PD, _I, _PD, _PD, _c.POINTER(_VP)])
+_proto("pq_engine_create_batch", [_I, _PI, _PI, _PI, _PI, _PD, _PD, _I, _PD, _PD, _PI, _c.POINTER(_VP)])
+_proto("pq_engine_layout", [_VP, _PI, _c.POINTER(_I64), _PI, _c.POINTER(_I64), _PI, _PI])
 _proto("pq_engine_destroy", [_VP])
 _proto("pq_engine_nbf", [_VP, _PI])
 _proto("pq_engine_set_geometry", [_VP, _PD])

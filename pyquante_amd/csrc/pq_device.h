@@ -57,14 +57,19 @@ struct __align__(16) PrimPair {
   double Px, Py, Pz;
 };
 
+// Matrices (D, J, K, S, ...) are stored per molecule: molecule m's n_m x n_m
+// block starts at element moff and has row stride mstride = n_m; bfA/bfB are
+// function indices within the molecule.  A single-molecule engine has
+// moff = 0 and mstride = nbf.  Pairs never mix molecules.
 struct __align__(16) PairInfo {
   double Ax, Ay, Az;     // centre of shell A
   double ABx, ABy, ABz;  // A - B
   int pp_off, pp_n;      // primitive pairs [pp_off, pp_off + pp_n)
-  int bfA, bfB;          // first basis function of each shell
+  int bfA, bfB;          // first basis function of each shell (within its molecule)
   int shA, shB;          // shell indices
   int atA, atB;          // atom indices
-  int pad0, pad1;
+  int moff, mstride;     // the molecule's matrix block: first element, row stride
+  int mat0, mnat;        // the molecule's atoms [mat0, mat0 + mnat)
 };
 
 struct Tile {
@@ -101,7 +106,9 @@ struct EngineView {
   const PairInfo* pairs;
   const PrimPair* prims;
   const double* prim_a;  // alpha_a of each primitive pair (one-electron and derivative kernels)
-  int nbf;
+  int nbf;               // largest molecule's function count (LDS row length)
+  int64_t msize;         // elements of one packed matrix (sum of n_m^2)
+  int maxat;             // largest molecule's atom count
 };
 
 // ---------------------------------------------------------------------

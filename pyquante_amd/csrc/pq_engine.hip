@@ -138,7 +138,20 @@ struct pq_engine {
       for (auto& b : a) b[0] = b[1] = -1;
   }
 
-  EngineView view() const { return EngineView{d_pairs, d_prims, d_prim_a, nbf}; }
+  // molecules: atoms and shells of molecule m are contiguous; matrices are
+  // stored as per-molecule blocks (molecule m: mol_nbf[m]^2 elements from
+  // mol_moff[m]); one molecule (the usual case) is the plain n x n layout
+  int nmol = 1;
+  std::vector<int> atom_mol, sh_mol, mol_nbf, mol_fbase, mol_at0, mol_nat, mol_sh0;
+  std::vector<int64_t> mol_moff;
+  int64_t msize = 0;  // elements of one packed matrix
+  int maxn = 0;       // largest molecule's function count
+  int* d_mol_moff = nullptr;
+  int* d_mol_n = nullptr;
+  int* d_atom_mol = nullptr;
+
+  int maxat = 0;      // largest molecule's atom count
+  EngineView view() const { return EngineView{d_pairs, d_prims, d_prim_a, maxn, msize, maxat}; }
 
   // per-function primitive data for the HGP path (built on first use, dropped
   // with the geometry)
@@ -197,6 +210,9 @@ struct pq_engine {
     dev_free(d_D2);
     dev_free(d_flat);
     dev_free(d_gacc);
+    dev_free(d_mol_moff);
+    dev_free(d_mol_n);
+    dev_free(d_atom_mol);
   }
 };
 
@@ -209,7 +225,8 @@ int build_pair_data(pq_engine* e) {
   e->pcls.clear();
   e->members.assign(6, {});
   for (int A = 0; A < e->nshell; ++A)
-    for (int B = 0; B <= A; ++B) {
+    for (int B = e->mol_sh0[e->sh_mol[A]]; B <= A; ++B) {  // pairs within A's molecule
+      const int m = e->sh_mol[A];
       int s1 = A, s2 = B;
       if (e->sh_l[B] > e->sh_l[A]) std::swap(s1, s2);
       const double* c1 = &e->atom_xyz[3 * e->sh_atom[s1]];
@@ -222,12 +239,16 @@ int build_pair_data(pq_engine* e) {
       pi.ABy = c1[1] - c2[1];
       pi.ABz = c1[2] - c2[2];
       pi.pp_off = (int)e->prims.size();
-      pi.bfA = e->sh_bf[s1];
-      pi.bfB = e->sh_bf[s2];
+      pi.bfA = e->sh_bf[s1] - e->mol_fbase[m];
+      pi.bfB = e->sh_bf[s2] - e->mol_fbase[m];
       pi.shA = s1;
       pi.shB = s2;
       pi.atA = e->sh_atom[s1];
       pi.atB = e->sh_atom[s2];
+      pi.moff = (int)e->mol_moff[m];
+      pi.mstride = e->mol_nbf[m];
+      pi.mat0 = e->mol_at0[m];
+      pi.mnat = e->mol_nat[m];
       const double ab2 = pi.ABx * pi.ABx + pi.ABy * pi.ABy + pi.ABz * pi.ABz;
       const int l1 = e->sh_l[s1], l2 = e->sh_l[s2];
       for (int i = 0; i < e->sh_np[s1]; ++i)
@@ -292,8 +313,11 @@ int build_pair_data(pq_engine* e) {
   e->sorted.assign(6, {});
   for (int c = 0; c < 6; ++c) {
     auto v = e->members[c];
-    // contraction length first (uniform primitive loops within a tile), then Q
+    // molecule first (a bra's kets are its own molecule's), then contraction
+    // length (uniform primitive loops within a tile), then Q
     std::stable_sort(v.begin(), v.end(), [&](int a, int b) {
+      const int ma = e->sh_mol[e->pairs[a].shA], mb = e->sh_mol[e->pairs[b].shA];
+      if (ma != mb) return ma < mb;
       if (e->pairs[a].pp_n != e->pairs[b].pp_n) return e->pairs[a].pp_n > e->pairs[b].pp_n;
       return e->Q[a] > e->Q[b];
     });
@@ -367,11 +391,18 @@ int build_work(pq_engine* e, std::vector<ClassWork>& out, double cut, double fp3
         kpre[j + 1] = kpre[j] + e->pairs[kets[j]].pp_n;
         fpre[j + 1] = fpre[j] + ufun(kets[j], lc, ld);
       }
-      // ket groups: runs of equal contraction length, each sorted by Q descending
+      // ket groups: runs of one molecule and equal contraction length, each
+      // sorted by Q descending; gmol[m] = first group of molecule m
+      auto kmol = [&](int j) { return e->sh_mol[e->pairs[kets[j]].shA]; };
       std::vector<int> gs;
       for (size_t j = 0; j < kets.size(); ++j)
-        if (j == 0 || e->pairs[kets[j]].pp_n != e->pairs[kets[j - 1]].pp_n) gs.push_back((int)j);
+        if (j == 0 || e->pairs[kets[j]].pp_n != e->pairs[kets[j - 1]].pp_n || kmol(j) != kmol(j - 1))
+          gs.push_back((int)j);
       gs.push_back((int)kets.size());
+      std::vector<int> gmol(e->nmol + 1, (int)gs.size() - 1);
+      for (int g = (int)gs.size() - 2; g >= 0; --g) gmol[kmol(gs[g])] = g;
+      for (int m = e->nmol - 1; m >= 0; --m)
+        if (gmol[m] > gmol[m + 1]) gmol[m] = gmol[m + 1];
       auto count_ge = [&](int g0, int g1, double thr) -> int {  // entries of [g0,g1) with Q >= thr
         return (int)(std::upper_bound(kq.begin() + g0, kq.begin() + g1, thr,
                                       [](double t, double q) { return t > q; }) -
@@ -381,7 +412,8 @@ int build_work(pq_engine* e, std::vector<ClassWork>& out, double cut, double fp3
         const double qi = e->Q[bras[i]];
         const int bpn = e->pairs[bras[i]].pp_n;
         const double ua = ufun(bras[i], la, lb);
-        for (size_t g = 0; g + 1 < gs.size(); ++g) {
+        const int bm = e->sh_mol[e->pairs[bras[i]].shA];
+        for (int g = gmol[bm]; g < gmol[bm + 1]; ++g) {
           const int g0 = gs[g], g1 = gs[g + 1];
           if (X == Y && g0 > (int)i) break;
           int c;
@@ -431,7 +463,7 @@ int build_work(pq_engine* e, std::vector<ClassWork>& out, double cut, double fp3
       }
       for (int prec = 0; prec < 2 && split_prims; ++prec) {
         const int tuned = e->group_cap[X][Y][prec];
-        const int rc = set_groups(cw, prec, tuned >= 0 ? tuned : default_group_cap(e->nbf));
+        const int rc = set_groups(cw, prec, tuned >= 0 ? tuned : default_group_cap(e->maxn));
         if (rc) return rc;
       }
       PQ_HIP(dev_alloc(&cw.d_t64, cw.t64.size()));
@@ -606,9 +638,26 @@ int pq_hgp_contr_coulomb_batch(int n, const int* nprim_host, const int* poff_hos
 int pq_engine_create(int nshell, const int* shell_l, const int* shell_nprim, const int* shell_poff,
                      const int* shell_atom, const double* exps, const double* coefs, int natom,
                      const double* atom_xyz, const double* atom_z, pq_engine** out) {
+  return pq_engine_create_batch(nshell, shell_l, shell_nprim, shell_poff, shell_atom, exps, coefs, natom, atom_xyz,
+                                atom_z, nullptr, out);
+}
+
+int pq_engine_create_batch(int nshell, const int* shell_l, const int* shell_nprim, const int* shell_poff,
+                           const int* shell_atom, const double* exps, const double* coefs, int natom,
+                           const double* atom_xyz, const double* atom_z, const int* atom_mol, pq_engine** out) {
   if (!out || nshell <= 0 || natom <= 0 || !shell_l || !shell_nprim || !shell_poff || !shell_atom || !exps ||
       !coefs || !atom_xyz || !atom_z)
     return fail(PQ_E_ARG, "bad engine arguments");
+  if (atom_mol) {
+    if (atom_mol[0] != 0) return fail(PQ_E_ARG, "molecule ids must start at 0");
+    for (int a = 1; a < natom; ++a)
+      if (atom_mol[a] != atom_mol[a - 1] && atom_mol[a] != atom_mol[a - 1] + 1)
+        return fail(PQ_E_ARG, "atoms of a molecule must be contiguous, molecules numbered 0, 1, ...");
+    for (int s = 1; s < nshell; ++s)
+      if (shell_atom[s] >= 0 && shell_atom[s] < natom && shell_atom[s - 1] >= 0 && shell_atom[s - 1] < natom &&
+          atom_mol[shell_atom[s]] < atom_mol[shell_atom[s - 1]])
+        return fail(PQ_E_ARG, "shells must be ordered by molecule");
+  }
   *out = nullptr;
   int nprim_total = 0;
   for (int s = 0; s < nshell; ++s) {
@@ -635,11 +684,62 @@ int pq_engine_create(int nshell, const int* shell_l, const int* shell_nprim, con
     nbf += ncart(shell_l[s]);
   }
   e->nbf = nbf;
+  // molecules
+  e->atom_mol.assign(natom, 0);
+  if (atom_mol) e->atom_mol.assign(atom_mol, atom_mol + natom);
+  e->nmol = e->atom_mol[natom - 1] + 1;
+  e->sh_mol.resize(nshell);
+  e->mol_nbf.assign(e->nmol, 0);
+  e->mol_fbase.assign(e->nmol, -1);
+  e->mol_sh0.assign(e->nmol, -1);
+  e->mol_at0.assign(e->nmol, -1);
+  e->mol_nat.assign(e->nmol, 0);
+  for (int a = 0; a < natom; ++a) {
+    const int m = e->atom_mol[a];
+    if (e->mol_at0[m] < 0) e->mol_at0[m] = a;
+    e->mol_nat[m] += 1;
+  }
+  for (int sidx = 0; sidx < nshell; ++sidx) {
+    const int m = e->atom_mol[shell_atom[sidx]];
+    e->sh_mol[sidx] = m;
+    if (e->mol_sh0[m] < 0) {
+      e->mol_sh0[m] = sidx;
+      e->mol_fbase[m] = e->sh_bf[sidx];
+    }
+    e->mol_nbf[m] += ncart(shell_l[sidx]);
+  }
+  e->mol_moff.assign(e->nmol, 0);
+  e->msize = 0;
+  e->maxn = 0;
+  std::vector<int> moff32(e->nmol);
+  for (int m = 0; m < e->nmol; ++m) {
+    if (e->mol_sh0[m] < 0) {
+      delete e;
+      return fail(PQ_E_ARG, "a molecule has no shells");
+    }
+    e->mol_moff[m] = e->msize;
+    moff32[m] = (int)e->msize;
+    e->msize += (int64_t)e->mol_nbf[m] * e->mol_nbf[m];
+    e->maxn = std::max(e->maxn, e->mol_nbf[m]);
+    e->maxat = std::max(e->maxat, e->mol_nat[m]);
+  }
+  if (e->msize >= (int64_t)1 << 31) {
+    delete e;
+    return fail(PQ_E_ARG, "packed matrices beyond 2^31 elements");
+  }
   if (dev_alloc(&e->d_atom_xyz, 3 * natom) != hipSuccess || dev_alloc(&e->d_atom_z, natom) != hipSuccess ||
-      dev_alloc(&e->d_Jacc, 2 * (size_t)nbf * nbf) != hipSuccess ||
-      dev_alloc(&e->d_Kacc, 2 * (size_t)nbf * nbf) != hipSuccess) {
+      dev_alloc(&e->d_Jacc, 2 * (size_t)e->msize) != hipSuccess ||
+      dev_alloc(&e->d_Kacc, 2 * (size_t)e->msize) != hipSuccess ||
+      dev_alloc(&e->d_mol_moff, e->nmol) != hipSuccess || dev_alloc(&e->d_mol_n, e->nmol) != hipSuccess ||
+      dev_alloc(&e->d_atom_mol, natom) != hipSuccess) {
     delete e;
     return fail(PQ_E_ALLOC, "device allocation failed");
+  }
+  if (hipMemcpy(e->d_mol_moff, moff32.data(), e->nmol * sizeof(int), hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(e->d_mol_n, e->mol_nbf.data(), e->nmol * sizeof(int), hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(e->d_atom_mol, e->atom_mol.data(), natom * sizeof(int), hipMemcpyHostToDevice) != hipSuccess) {
+    delete e;
+    return fail(PQ_E_HIP, "copy of the molecule layout failed");
   }
   if (hipMemcpy(e->d_atom_xyz, atom_xyz, 3 * natom * sizeof(double), hipMemcpyHostToDevice) != hipSuccess ||
       hipMemcpy(e->d_atom_z, atom_z, natom * sizeof(double), hipMemcpyHostToDevice) != hipSuccess) {
@@ -652,6 +752,20 @@ int pq_engine_create(int nshell, const int* shell_l, const int* shell_nprim, con
 
 int pq_engine_destroy(pq_engine* eng) {
   delete eng;
+  return PQ_OK;
+}
+
+int pq_engine_layout(const pq_engine* eng, int* nmol, int64_t* msize, int* mol_nbf, int64_t* mol_moff,
+                     int* mol_first_atom, int* mol_natom) {
+  if (!eng || !nmol || !msize) return fail(PQ_E_ARG, "null argument");
+  *nmol = eng->nmol;
+  *msize = eng->msize;
+  for (int m = 0; m < eng->nmol; ++m) {
+    if (mol_nbf) mol_nbf[m] = eng->mol_nbf[m];
+    if (mol_moff) mol_moff[m] = eng->mol_moff[m];
+    if (mol_first_atom) mol_first_atom[m] = eng->mol_at0[m];
+    if (mol_natom) mol_natom[m] = eng->mol_nat[m];
+  }
   return PQ_OK;
 }
 
@@ -701,7 +815,7 @@ int pq_engine_set_group_cap(pq_engine* eng, int cap) {
   if (eng->work_ok)
     for (auto& c : eng->work)
       for (int prec = 0; prec < 2; ++prec) {
-        int rc = set_groups(c, prec, cap >= 0 ? cap : default_group_cap(eng->nbf));
+        int rc = set_groups(c, prec, cap >= 0 ? cap : default_group_cap(eng->maxn));
         if (rc) return rc;
       }
   return PQ_OK;
@@ -719,7 +833,7 @@ int pq_engine_set_group_caps(pq_engine* eng, const int* caps) {
     for (auto& c : eng->work)
       for (int prec = 0; prec < 2; ++prec) {
         const int cap = eng->group_cap[c.X][c.Y][prec];
-        int rc = set_groups(c, prec, cap >= 0 ? cap : default_group_cap(eng->nbf));
+        int rc = set_groups(c, prec, cap >= 0 ? cap : default_group_cap(eng->maxn));
         if (rc) return rc;
       }
   return PQ_OK;
@@ -793,6 +907,7 @@ int pq_engine_stats(const pq_engine* eng, uint64_t counts[6], double flops[2]) {
 
 int pq_get2ints(pq_engine* eng, double* out_dev, uint64_t len, void* stream) {
   if (!eng || !out_dev) return fail(PQ_E_ARG, "null argument");
+  if (eng->nmol > 1) return fail(PQ_E_STATE, "stored integrals need a single-molecule engine");
   const uint64_t n = eng->nbf, npair = n * (n + 1) / 2;
   if (len != npair * (npair + 1) / 2) return fail(PQ_E_ARG, "output length must be npair*(npair+1)/2");
   if (!eng->pairs_ok) {
@@ -861,6 +976,7 @@ int build_functions(pq_engine* e) {
 
 int pq_get2ints_hgp(pq_engine* eng, double* out_dev, uint64_t len, int compat_nr, void* stream) {
   if (!eng || !out_dev) return fail(PQ_E_ARG, "null argument");
+  if (eng->nmol > 1) return fail(PQ_E_STATE, "stored integrals need a single-molecule engine");
   const uint64_t n = eng->nbf, npair = n * (n + 1) / 2;
   if (len != npair * (npair + 1) / 2) return fail(PQ_E_ARG, "output length must be npair*(npair+1)/2");
   int rc = build_functions(eng);
@@ -923,7 +1039,7 @@ static int fock_jk_impl(pq_engine* eng, int nd, const double* const* Ds, double*
   if (precision == PQ_PREC_FP64 && eng->fp32cut > 0.0)
     return fail(PQ_E_STATE, "work lists were built for mixed precision; rebuild with fp32_cut = 0");
   hipStream_t st = (hipStream_t)stream;
-  const int64_t nn = (int64_t)eng->nbf * eng->nbf;
+  const int64_t nn = eng->msize;  // one packed matrix (n^2 for one molecule)
   // timing: one event pair per class launch, recorded on the stream the
   // kernel runs on (so concurrent launches are timed as they run)
   std::vector<hipEvent_t> ev;
@@ -998,7 +1114,11 @@ static int fock_jk_impl(pq_engine* eng, int nd, const double* const* Ds, double*
     }
     PQ_TRY(err);
     for (int d = 0; d < nd && rc == PQ_OK; ++d)
-      PQ_TRY(launch_symmetrize(eng->nbf, eng->d_Jacc + d * nn, eng->d_Kacc + d * nn, Js[d], Ks[d], 1.0, 1.0, st));
+      PQ_TRY(eng->nmol == 1
+                 ? launch_symmetrize(eng->nbf, eng->d_Jacc + d * nn, eng->d_Kacc + d * nn, Js[d], Ks[d], 1.0, 1.0,
+                                     st)
+                 : launch_symmetrize_packed(eng->nmol, eng->maxn, eng->d_mol_moff, eng->d_mol_n,
+                                            eng->d_Jacc + d * nn, eng->d_Kacc + d * nn, Js[d], Ks[d], st));
     if (rc != PQ_OK) break;
     if (ms) {
       PQ_TRY(mark(st));
@@ -1052,6 +1172,7 @@ int pq_shell_quartet_blocks(pq_engine* eng, int nq, const int* shells_host, doub
                             void* stream) {
   if (!eng || (nq > 0 && (!shells_host || !out_dev))) return fail(PQ_E_ARG, "null argument");
   if (nq < 0) return fail(PQ_E_ARG, "negative quartet count");
+  if (eng->nmol > 1) return fail(PQ_E_STATE, "shell-quartet blocks need a single-molecule engine");
   if (nq == 0) return PQ_OK;
   std::vector<int> first(4 * (size_t)nq), cnt(4 * (size_t)nq);
   std::vector<int64_t> off(nq);
@@ -1117,7 +1238,8 @@ int pq_forces(pq_engine* eng, const double* D_dev, const double* W_dev, double* 
     PQ_HIP(launch_onee_grad(kPairL[c][0], kPairL[c][1], eng->view(), eng->d_members[c], (int)eng->members[c].size(),
                             eng->d_atom_xyz, eng->d_atom_z, eng->natom, D_dev, W_dev, eng->d_gacc, st));
   }
-  PQ_HIP(launch_grad_finish(eng->d_gacc, ncopy, eng->natom, eng->d_atom_xyz, eng->d_atom_z, grad_dev, st));
+  PQ_HIP(launch_grad_finish(eng->d_gacc, ncopy, eng->natom, eng->d_atom_xyz, eng->d_atom_z,
+                            eng->nmol > 1 ? eng->d_atom_mol : nullptr, grad_dev, st));
   return PQ_OK;
 }
 

@@ -81,6 +81,28 @@ __global__ void k_symmetrize(int n, const double* __restrict__ Jacc, const doubl
   if (K) K[idx] = kscale * (Kacc[idx] + Kacc[(int64_t)j * n + i]);
 }
 
+// The same per molecule block of a packed set of matrices: block m starts
+// at moff[m] with n = mn[m] (blockIdx.y = m).
+__global__ void k_symmetrize_packed(const int* __restrict__ moff, const int* __restrict__ mn,
+                                    const double* __restrict__ Jacc, const double* __restrict__ Kacc, double* J,
+                                    double* K) {
+  const int m = blockIdx.y, n = mn[m];
+  const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= n * n) return;
+  const int i = idx / n, j = idx % n;
+  const int64_t o = moff[m];
+  if (J) J[o + idx] = Jacc[o + idx] + Jacc[o + j * n + i];
+  if (K) K[o + idx] = Kacc[o + idx] + Kacc[o + j * n + i];
+}
+
+hipError_t launch_symmetrize_packed(int nmol, int maxn, const int* moff, const int* mn, const double* Jacc,
+                                    const double* Kacc, double* J, double* K, hipStream_t st) {
+  if (nmol <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_symmetrize_packed, dim3((maxn * maxn + 255) / 256, nmol), dim3(256), 0, st, moff, mn, Jacc,
+                     Kacc, J, K);
+  return hipGetLastError();
+}
+
 // ---------------------------------------------------------------------
 // Digestion of a stored flat array Ints[ijkl2intindex(i,j,k,l)]: entry t
 // is decoded back to (ij, kl) with ij >= kl and then to i >= j, k >= l; the

@@ -159,7 +159,11 @@ __device__ __forceinline__ void digest_lane(const EngineView& ev, const PairInfo
                                             bool active, const double* __restrict__ D, double* Jacc,
                                             double* Kacc, double* krow, double* jsum) {
   using C = Cls<LA, LB, LC, LD>;
-  const int n = ev.nbf;
+  // the quartet's molecule block (bra and ket share it)
+  const int n = bp.mstride;
+  D += bp.moff;
+  Jacc += bp.moff;
+  Kacc += bp.moff;
   const int a0 = bp.bfA + A0, b0 = bp.bfB + B0, c0 = kp.bfA, d0 = kp.bfB;
   double jab[NAS][NBS], jcd[C::NC][C::ND], kac[NAS][C::NC], kad[NAS][C::ND], kbc[NBS][C::NC], kbd[NBS][C::ND];
   double Dab[NAS][NBS], Dcd[C::NC][C::ND], Dac[NAS][C::NC], Dad[NAS][C::ND], Dbc[NBS][C::NC], Dbd[NBS][C::ND];
@@ -320,7 +324,7 @@ __device__ __forceinline__ void lane_body(const EngineView& ev, const Tile& tl, 
     const double f = active ? quartet_degeneracy(bp, kp, bra, ket) : 0.0;
     // one block of integrals, digested once per density (stacked n x n
     // matrices; LDS K rows and J sums stacked the same way)
-    const size_t nn = (size_t)ev.nbf * ev.nbf;
+    const size_t nn = (size_t)ev.msize;
 #pragma unroll
     for (int d = 0; d < mode_ndens(MODE); ++d)
       digest_lane<LA, LB, LC, LD, A0, NAS, B0, NBS, T>(
@@ -427,14 +431,15 @@ __global__ __launch_bounds__(256) void k_eri_group(EngineView ev, const Tile* __
   }
   __syncthreads();
   const PairInfo bp = ev.pairs[tiles[g.x].bra];
-  const int rows = (C::NA + C::NB) * n;
+  const int ms = bp.mstride;
+  const int rows = (C::NA + C::NB) * ms, region = (C::NA + C::NB) * n;
   for (int i = threadIdx.x; i < ND * rows; i += blockDim.x) {
-    const double v = krow[i];
+    const int d = i / rows, ir = i - d * rows;
+    const double v = krow[d * region + ir];
     if (v != 0.0) {
-      const int d = i / rows, ir = i - d * rows;
-      const int r = ir / n, col = ir - r * n;
+      const int r = ir / ms, col = ir - r * ms;
       const int grow = r < C::NA ? bp.bfA + r : bp.bfB + (r - C::NA);
-      unsafeAtomicAdd(&Kacc[(size_t)d * n * n + grow * n + col], v);
+      unsafeAtomicAdd(&Kacc[(size_t)d * ev.msize + bp.moff + grow * ms + col], v);
     }
   }
 }
@@ -494,14 +499,15 @@ __global__ __launch_bounds__(256) void k_eri_lane(EngineView ev, const Tile* __r
   __syncthreads();
   // flush the K rows (coalesced) and the ket-side J accumulator
   const PairInfo bp = ev.pairs[tl.bra];
-  const int rows = (C::NA + C::NB) * n;
+  const int ms = bp.mstride;
+  const int rows = (C::NA + C::NB) * ms, region = (C::NA + C::NB) * n;
   for (int i = threadIdx.x; i < ND * rows; i += blockDim.x) {
-    const double v = krow[i];
+    const int d = i / rows, ir = i - d * rows;
+    const double v = krow[d * region + ir];
     if (v != 0.0) {
-      const int d = i / rows, ir = i - d * rows;
-      const int r = ir / n, col = ir - r * n;
+      const int r = ir / ms, col = ir - r * ms;
       const int grow = r < C::NA ? bp.bfA + r : bp.bfB + (r - C::NA);
-      unsafeAtomicAdd(&Kacc[(size_t)d * n * n + grow * n + col], v);
+      unsafeAtomicAdd(&Kacc[(size_t)d * ev.msize + bp.moff + grow * ms + col], v);
     }
   }
   if constexpr (C::SPLIT != 0) {
@@ -514,7 +520,8 @@ __global__ __launch_bounds__(256) void k_eri_lane(EngineView ev, const Tile* __r
         const double v = jsum[d * 64 * NCD + i];
         if (v != 0.0) {
           const PairInfo kq = ev.pairs[kets[tl.ket0 + tl_l.slot]];
-          unsafeAtomicAdd(&Jacc[(size_t)d * n * n + (kq.bfA + cd / C::ND) * n + kq.bfB + cd % C::ND], v);
+          unsafeAtomicAdd(&Jacc[(size_t)d * ev.msize + bp.moff + (kq.bfA + cd / C::ND) * ms + kq.bfB + cd % C::ND],
+                          v);
         }
       }
   }

@@ -34,7 +34,8 @@ __device__ __forceinline__ void grad_lane(const EngineView& ev, const Tile& tl, 
   constexpr int NO = NAS * NBS * C::NC * C::ND;
   const PairInfo bp = ld_uniform(ev.pairs + bra);
   const PairInfo kp = ev.pairs[ket];
-  const int n = ev.nbf;
+  const int n = bp.mstride;
+  D += bp.moff;
   const int a0 = bp.bfA + A0, b0 = bp.bfB + B0, c0 = kp.bfA, d0 = kp.bfB;
   // density factor per component (normalisation folded in)
   double G[NO];
@@ -136,10 +137,11 @@ __device__ __forceinline__ void grad_lane(const EngineView& ev, const Tile& tl, 
     }
   }
   if (active) {
+    // gl: the molecule's atoms, from its first atom
 #pragma unroll
     for (int x = 0; x < 3; ++x) {
-      atomicAdd(&gl[3 * kp.atA + x], gC[x]);
-      atomicAdd(&gl[3 * kp.atB + x], -(gA[x] + gB[x] + gC[x]));
+      atomicAdd(&gl[3 * (kp.atA - bp.mat0) + x], gC[x]);
+      atomicAdd(&gl[3 * (kp.atB - bp.mat0) + x], -(gA[x] + gB[x] + gC[x]));
     }
   }
 }
@@ -154,10 +156,11 @@ __global__ __launch_bounds__(64) void k_eri_grad(EngineView ev, const Tile* __re
   const int lane = threadIdx.x;
   const int t = blockIdx.x;
   if (t >= ntiles) return;
-  double* gl = pq_lds;
-  for (int i = lane; i < 3 * natom; i += 64) gl[i] = 0.0;
-  __syncthreads();
+  double* gl = pq_lds;  // 3 x (atoms of the tile's molecule)
   Tile tl = ld_uniform(tiles + t);
+  const PairInfo bpm = ld_uniform(ev.pairs + __builtin_amdgcn_readfirstlane(tl.bra));
+  for (int i = lane; i < 3 * bpm.mnat; i += 64) gl[i] = 0.0;
+  __syncthreads();
   tl.bra = __builtin_amdgcn_readfirstlane(tl.bra);
   tl.ket0 = __builtin_amdgcn_readfirstlane(tl.ket0);
   tl.nket = __builtin_amdgcn_readfirstlane(tl.nket);
@@ -181,9 +184,9 @@ __global__ __launch_bounds__(64) void k_eri_grad(EngineView ev, const Tile* __re
     });
   }
   __syncthreads();
-  for (int i = lane; i < 3 * natom; i += 64) {
+  for (int i = lane; i < 3 * bpm.mnat; i += 64) {
     const double v = gl[i];
-    if (v != 0.0) unsafeAtomicAdd(&gcopy[i], v);
+    if (v != 0.0) unsafeAtomicAdd(&gcopy[3 * bpm.mat0 + i], v);
   }
 }
 
@@ -193,7 +196,7 @@ static hipError_t launch_grad_one(const EngineView& ev, const Tile* tiles, int n
   if (ntiles <= 0) return hipSuccess;
   using GC = GCls<LA, LB, LC, LD>;
   hipLaunchKernelGGL((k_eri_grad<LA, LB, LC, LD>), dim3(ntiles, GC::NSPLIT), dim3(64),
-                     (size_t)3 * natom * sizeof(double), st, ev, tiles, ntiles, kets, D, gacc, natom);
+                     (size_t)3 * ev.maxat * sizeof(double), st, ev, tiles, ntiles, kets, D, gacc, natom);
   return hipGetLastError();
 }
 

@@ -94,9 +94,9 @@ __global__ __launch_bounds__(256) void k_onee(EngineView ev, const int* __restri
         if (bz >= 2) t2 += bz * (bz - 1) * O[0][ax][bx] * O[1][ay][by] * O[2][az][bz - 2];
         tk[a][b] += pre * (t0 - 2.0 * beta * beta * t1 - 0.5 * t2);
       }
-    // nuclear attraction
+    // nuclear attraction (the nuclei of the pair's own molecule)
     const double vpre = -2.0 * M_PI * pp.Kp;
-    for (int c = 0; c < natom; ++c) {
+    for (int c = bp.mat0; c < bp.mat0 + bp.mnat; ++c) {
       const double Cc[3] = {atom_xyz[3 * c], atom_xyz[3 * c + 1], atom_xyz[3 * c + 2]};
       const double PC[3] = {P[0] - Cc[0], P[1] - Cc[1], P[2] - Cc[2]};
       const double X = p * (PC[0] * PC[0] + PC[1] * PC[1] + PC[2] * PC[2]);
@@ -139,7 +139,10 @@ __global__ __launch_bounds__(256) void k_onee(EngineView ev, const int* __restri
       }
     }
   }
-  const int n = ev.nbf;
+  const int n = bp.mstride;
+  S += bp.moff;
+  T += bp.moff;
+  V += bp.moff;
 #pragma unroll
   for (int a = 0; a < NA; ++a)
 #pragma unroll
@@ -193,7 +196,9 @@ __global__ __launch_bounds__(128) void k_onee_grad(EngineView ev, const int* __r
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= npairs) return;
   const PairInfo bp = ev.pairs[pair_ids[t]];
-  const int n = ev.nbf;
+  const int n = bp.mstride;
+  D += bp.moff;
+  W += bp.moff;
   double* gcopy = gacc + (size_t)(t % kGradCopies) * 3 * natom;
   const double A[3] = {bp.Ax, bp.Ay, bp.Az};
   const double B[3] = {bp.Ax - bp.ABx, bp.Ay - bp.ABy, bp.Az - bp.ABz};
@@ -266,9 +271,9 @@ __global__ __launch_bounds__(128) void k_onee_grad(EngineView ev, const int* __r
           gB[x] -= g;
         }
       }
-    // nuclear attraction: derivatives on A, B and the nucleus
+    // nuclear attraction: derivatives on A, B and the nucleus (own molecule)
     const double vpre = -2.0 * M_PI * pp.Kp;
-    for (int c = 0; c < natom; ++c) {
+    for (int c = bp.mat0; c < bp.mat0 + bp.mnat; ++c) {
       const double Cc[3] = {atom_xyz[3 * c], atom_xyz[3 * c + 1], atom_xyz[3 * c + 2]};
       const double PC[3] = {P[0] - Cc[0], P[1] - Cc[1], P[2] - Cc[2]};
       const double X = p * (PC[0] * PC[0] + PC[1] * PC[1] + PC[2] * PC[2]);
@@ -359,15 +364,18 @@ hipError_t launch_onee_grad(int la, int lb, const EngineView& ev, const int* pai
 }
 
 // grad[i] = sum over the private copies + d/dR of sum_{A<B} Z_A Z_B / R_AB
+// atom_mol (nullable): molecule of each atom; nuclear repulsion acts within a
+// molecule only
 __global__ void k_grad_finish(const double* __restrict__ gacc, int ncopies, int natom,
-                              const double* __restrict__ xyz, const double* __restrict__ z, double* grad) {
+                              const double* __restrict__ xyz, const double* __restrict__ z,
+                              const int* __restrict__ atom_mol, double* grad) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= 3 * natom) return;
   double s = 0.0;
   for (int c = 0; c < ncopies; ++c) s += gacc[(size_t)c * 3 * natom + i];
   const int a = i / 3, x = i % 3;
   for (int b = 0; b < natom; ++b) {
-    if (b == a) continue;
+    if (b == a || (atom_mol && atom_mol[b] != atom_mol[a])) continue;
     const double dx = xyz[3 * a] - xyz[3 * b], dy = xyz[3 * a + 1] - xyz[3 * b + 1], dz = xyz[3 * a + 2] - xyz[3 * b + 2];
     const double r2 = dx * dx + dy * dy + dz * dz;
     const double r = sqrt(r2);
@@ -378,9 +386,9 @@ __global__ void k_grad_finish(const double* __restrict__ gacc, int ncopies, int 
 }
 
 hipError_t launch_grad_finish(const double* gacc, int ncopies, int natom, const double* atom_xyz,
-                              const double* atom_z, double* grad, hipStream_t st) {
+                              const double* atom_z, const int* atom_mol, double* grad, hipStream_t st) {
   hipLaunchKernelGGL(k_grad_finish, dim3((3 * natom + 127) / 128), dim3(128), 0, st, gacc, ncopies, natom, atom_xyz,
-                     atom_z, grad);
+                     atom_z, atom_mol, grad);
   return hipGetLastError();
 }
 

@@ -54,21 +54,25 @@ class Engine(object):
     """Engine(bfs, atoms): bfs is a BasisSet (or CGBF list), atoms a Molecule
     or a list of Atom."""
 
-    def __init__(self, bfs, atoms):
+    def __init__(self, bfs, atoms, atom_mol=None, shells=None, shell_atoms=None):
         self.device = _device()
         atoms = list(atoms)
         self.natom = len(atoms)
         self.atom_z = np.array([a.atno for a in atoms], dtype=np.float64)
         xyz = np.array([a.pos() for a in atoms], dtype=np.float64)
-        shells = shells_of(bfs)
-        # map each shell to an atom by its centre
+        shells = shells_of(bfs) if shells is None else shells
+        # map each shell to an atom by its centre (batches give the map: their
+        # molecules may overlap in space)
         self._shell_atom = []
-        for (L, e, c, atid, origin) in shells:
-            d = np.sum((xyz - np.array(origin)) ** 2, axis=1)
-            k = int(np.argmin(d))
-            if d[k] > 1e-20:
-                raise ValueError("shell centre does not coincide with an atom")
-            self._shell_atom.append(k)
+        if shell_atoms is not None:
+            self._shell_atom = [int(k) for k in shell_atoms]
+        else:
+            for (L, e, c, atid, origin) in shells:
+                d = np.sum((xyz - np.array(origin)) ** 2, axis=1)
+                k = int(np.argmin(d))
+                if d[k] > 1e-20:
+                    raise ValueError("shell centre does not coincide with an atom")
+                self._shell_atom.append(k)
         self.shell_l = np.array([s[0] for s in shells], dtype=np.int32)
         nprim = [len(s[1]) for s in shells]
         self.shell_nprim = np.array(nprim, dtype=np.int32)
@@ -77,14 +81,20 @@ class Engine(object):
         self.exps = np.array([e for s in shells for e in s[1]], dtype=np.float64)
         self.coefs = np.array([c for s in shells for c in s[2]], dtype=np.float64)
         h = ctypes.c_void_p()
-        check(lib.pq_engine_create(len(shells), pi(self.shell_l), pi(self.shell_nprim), pi(self.shell_poff),
-                                   pi(self.shell_atom), pd(self.exps), pd(self.coefs), self.natom,
-                                   pd(np.ascontiguousarray(xyz.reshape(-1))), pd(self.atom_z),
-                                   ctypes.byref(h)))
+        am = None if atom_mol is None else np.ascontiguousarray(np.asarray(atom_mol, dtype=np.int32))
+        check(lib.pq_engine_create_batch(len(shells), pi(self.shell_l), pi(self.shell_nprim), pi(self.shell_poff),
+                                         pi(self.shell_atom), pd(self.exps), pd(self.coefs), self.natom,
+                                         pd(np.ascontiguousarray(xyz.reshape(-1))), pd(self.atom_z),
+                                         None if am is None else pi(am), ctypes.byref(h)))
         self.handle = h
         n = ctypes.c_int()
         check(lib.pq_engine_nbf(h, ctypes.byref(n)))
         self.nbf = n.value
+        nm, ms = ctypes.c_int(), ctypes.c_int64()
+        check(lib.pq_engine_layout(h, ctypes.byref(nm), ctypes.byref(ms), None, None, None, None))
+        self.nmol, self.msize = nm.value, ms.value
+        # matrices: n x n for one molecule, packed per-molecule blocks otherwise
+        self.mshape = (self.nbf, self.nbf) if self.nmol == 1 else (self.msize,)
         self.xyz = xyz
         self.cut = None
         self.fp32cut = None
@@ -173,8 +183,8 @@ class Engine(object):
         if not torch.is_tensor(D):
             D = torch.as_tensor(np.asarray(D, dtype=np.float64))
         D = D.to(device=self.device, dtype=torch.float64).contiguous()
-        if D.shape != (self.nbf, self.nbf):
-            raise ValueError("density must be %d x %d" % (self.nbf, self.nbf))
+        if tuple(D.shape) != self.mshape:
+            raise ValueError("density must have shape %s" % (self.mshape,))
         return D
 
     def fock_jk(self, D, want_j=True, want_k=True, precision=_abi.PQ_PREC_FP64):
@@ -264,8 +274,7 @@ class Engine(object):
 
     def get1ints(self):
         """(S, T, V) device tensors (pq_get1ints); h = T + V."""
-        n = self.nbf
-        S = torch.empty((n, n), dtype=torch.float64, device=self.device)
+        S = torch.empty(self.mshape, dtype=torch.float64, device=self.device)
         T = torch.empty_like(S)
         V = torch.empty_like(S)
         check(lib.pq_get1ints(self.handle, ctypes.c_void_p(S.data_ptr()), ctypes.c_void_p(T.data_ptr()),
