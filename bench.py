@@ -22,7 +22,9 @@ single-threaded.
 Outside the timed region the line also carries: (H2O)_16 FP64 against mixed
 precision (times, energy error), forces per second on (H2O)_8, J/K of two
 densities in one pass against two builds (the unrestricted path), MP2 on
-(H2O)_8 (stored ERIs, AO->MO transform as device GEMMs), and the HGP
+(H2O)_8 (stored ERIs, AO->MO transform as device GEMMs), the batched
+workload (512 seeded (H2O)_n, n in 1..12, in one engine: Fock builds, SCF and
+forces in FP64 and mixed precision, the largest per-molecule errors), and the HGP
 path on (H2O)_8 (all 202,015,050 unique ERIs by the GPU HGP kernel against
 the Rys kernels, RHF energies through both, the compiled reference chgp.c on
 a CPU sample).
@@ -209,6 +211,8 @@ def extra_workloads(args):
         print("bench: (H2O)_8 MP2 %.3f s (E2 = %.8f Eh, transform %.0f GFLOP)" % (tm, e2, gflop),
               file=sys.stderr, flush=True)
         del ints
+    if args.batch_molecules > 0:
+        out.update(batch_workload(args))
     # HGP path: every unique function quartet of (H2O)_8 by the GPU HGP
     # kernel, against the Rys kernels' array; RHF through the stored HGP
     # integrals against the direct Rys RHF
@@ -253,6 +257,50 @@ def extra_workloads(args):
     return out
 
 
+def batch_workload(args):
+    """The batched workload: seeded water clusters (H2O)_n, n uniform in 1..12,
+    all in one engine (one launch per class over every molecule), RHF for all
+    of them at once in FP64 and in mixed precision, forces of all."""
+    import torch
+
+    from pyquante_amd import _abi
+    from pyquante_amd.batch import BatchEngine, batch_forces, batch_rhf, seeded_water_batch
+    mols = seeded_water_batch(nmol=args.batch_molecules, nmax=12, seed=args.seed)
+    be = BatchEngine(mols)
+    out = {"batch_molecules": be.nmol, "batch_functions": be.nbf}
+    for label, fp32, prec in (("fp64", 0.0, _abi.PQ_PREC_FP64), ("mixed", 1e-4, _abi.PQ_PREC_MIXED)):
+        be.build(1e-12, fp32)
+        st = be.stats()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        E, orbe, C, D, W, it = batch_rhf(be, ConvCriteria=1e-8, precision=prec)
+        torch.cuda.synchronize()
+        tscf = time.perf_counter() - t0
+        tb = _timed(be.capture_fock_jk(D, precision=prec), 3)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        F = batch_forces(be, D, W)
+        torch.cuda.synchronize()
+        tf = time.perf_counter() - t0
+        fq = st["fquartets_fp64"] + st["fquartets_fp32"]
+        out.update({"batch_%s_fock_build_ms" % label: tb * 1e3, "batch_%s_quartets_per_s" % label: fq / tb,
+                    "batch_%s_scf_s" % label: tscf, "batch_%s_scf_iterations" % label: it,
+                    "batch_%s_forces_ms" % label: tf * 1e3})
+        if label == "fp64":
+            E64, F64 = E, F
+            out["batch_function_quartets"] = fq
+        else:
+            out["batch_mixed_max_energy_error_eh"] = float((E - E64).abs().max())
+            out["batch_mixed_max_force_error"] = max(float((a - b).abs().max()) for a, b in zip(F, F64))
+            out["batch_mixed_fock_speedup"] = out["batch_fp64_fock_build_ms"] / out["batch_mixed_fock_build_ms"]
+        print("bench: batch of %d molecules (%d functions) %s: Fock build %.1f ms, SCF %.1f s (%d it), forces %.0f ms"
+              % (be.nmol, be.nbf, label, tb * 1e3, tscf, it, tf * 1e3), file=sys.stderr, flush=True)
+    print("bench: batch mixed vs FP64: Fock build x%.2f, max |dE| %.1e Eh, max |dF| %.1e"
+          % (out["batch_mixed_fock_speedup"], out["batch_mixed_max_energy_error_eh"], out["batch_mixed_max_force_error"]),
+          file=sys.stderr, flush=True)
+    return out
+
+
 def eng_len(eng):
     npair = eng.nbf * (eng.nbf + 1) // 2
     return npair * (npair + 1) // 2
@@ -269,6 +317,8 @@ def main():
     ap.add_argument("--no-extra", action="store_true", help="skip the (H2O)_16, forces and HGP workloads")
     ap.add_argument("--no-hgp", action="store_true", help="skip the HGP workload")
     ap.add_argument("--no-mp2", action="store_true", help="skip the MP2 workload")
+    ap.add_argument("--batch-molecules", type=int, default=512,
+                    help="molecules of the batched workload (0 skips it)")
     ap.add_argument("--streams", type=int, default=4, help="engine streams for the class kernels (0-4)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run one 256 MiB device copy (reference for rocprofv3 FETCH/WRITE_SIZE)")

@@ -84,6 +84,18 @@ class BatchEngine(Engine):
         return [packed[o:o + n * n].view(n, n) for o, n in zip(self.mol_moff, self.mol_nbf)]
 
 
+_CHUNK = 64  # molecules per batched linear-algebra call (library workspace limits)
+
+
+def _chunked(fn, *xs):
+    """fn over slices of at most _CHUNK molecules, results concatenated (a
+    tuple of outputs is concatenated element-wise)."""
+    outs = [fn(*(x[i:i + _CHUNK] for x in xs)) for i in range(0, xs[0].shape[0], _CHUNK)]
+    if isinstance(outs[0], tuple):
+        return tuple(torch.cat(o, 0) for o in zip(*outs))
+    return torch.cat(outs, 0)
+
+
 def _batched_diis_fock(Fs, Es, F, err, nkeep=8):
     """Pulay DIIS per molecule on padded batches: returns the extrapolated
     Fock matrices (nmol, nmax, nmax)."""
@@ -104,7 +116,7 @@ def _batched_diis_fock(Fs, Es, F, err, nkeep=8):
     # scale each molecule's system for conditioning
     scale = B[:, :k, :k].diagonal(dim1=1, dim2=2).abs().max(1).values.clamp_min(1e-300)
     B[:, :k, :k] /= scale[:, None, None]
-    c = torch.linalg.solve(B, rhs)[:, :k]
+    c = _chunked(torch.linalg.solve, B, rhs)[:, :k]
     return torch.einsum("mk,mkij->mij", c, torch.stack(Fs, 1))
 
 
@@ -117,14 +129,15 @@ def batch_rhf(be, ConvCriteria=1e-8, MaxIter=60, precision=_abi.PQ_PREC_FP64, gr
     S, T, V = be.get1ints()
     Sp = be.pad(S, fill_diag=1.0)
     hp = be.pad(T + V, fill_diag=pad_energy)
-    L = torch.linalg.cholesky(Sp)
-    X = torch.linalg.inv(L).transpose(1, 2)
+    L = _chunked(torch.linalg.cholesky, Sp)
+    eye = torch.eye(be.nmax, dtype=Sp.dtype, device=Sp.device).expand_as(Sp).contiguous()
+    X = _chunked(lambda a, b: torch.linalg.solve_triangular(a, b, upper=False), L, eye).transpose(1, 2)
     nocc = torch.as_tensor([m.get_closedopen()[0] for m in be.mols], device=be.device)
     occ = (torch.arange(be.nmax, device=be.device)[None, :] < nocc[:, None]).to(torch.float64)
     enuke = torch.as_tensor([m.get_enuke() for m in be.mols], dtype=torch.float64, device=be.device)
 
     def solve(Fp):
-        e, v = torch.linalg.eigh(X.transpose(1, 2) @ Fp @ X)
+        e, v = _chunked(torch.linalg.eigh, X.transpose(1, 2) @ Fp @ X)
         return e, X @ v
 
     orbe, C = solve(hp)
