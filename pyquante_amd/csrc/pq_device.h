@@ -149,8 +149,22 @@ struct RysPoint {
   double sv;            // local variable on the interval
 };
 
+// Interval stride of the order-NR tables in global memory and of their LDS
+// copies.  The LDS copy pads every interval by one double (FP64) or one float4
+// (FP32) so that the stride is 2 mod 64 dwords (FP64) or 4 mod 64 dwords
+// (FP32): lanes whose X falls in different intervals then read different
+// banks instead of the same two or four.
+template <int NR>
+constexpr int kRysStride = 2 * NR * (RYS_DEG + 1);
+template <int NR>
+constexpr int kRysLdsStride = kRysStride<NR> + 1;
+template <int NR>
+constexpr int kRysStride32 = 2 * NR * RYS_STRIDE32;
+template <int NR>
+constexpr int kRysLdsStride32 = kRysStride32<NR> + 4;
+
 // LT: tab is the order-NR block of the table (rys_coef + rys_off[NR - 1])
-// staged in LDS; otherwise the global table is read
+// staged in LDS with the padded stride; otherwise the global table is read
 template <int NR, bool LT = false>
 __device__ __forceinline__ RysPoint<NR> rys_prepare(double X, const double* tab = nullptr) {
   RysPoint<NR> p;
@@ -165,7 +179,7 @@ __device__ __forceinline__ RysPoint<NR> rys_prepare(double X, const double* tab 
     const int k = (int)X;
     p.rx = p.rs = 0.0;
     p.sv = 2.0 * (X - (double)k) - 1.0;
-    p.c = base + k * (2 * NR * (RYS_DEG + 1));
+    p.c = base + k * (LT ? kRysLdsStride<NR> : kRysStride<NR>);
   }
   return p;
 }
@@ -215,7 +229,7 @@ __device__ __forceinline__ RysPoint32<NR> rys_prepare32(double X, const float* t
     const int k = (int)X;
     p.rx = p.rs = 0.0;
     p.sv = (float)(2.0 * (X - (double)k) - 1.0);
-    p.c = base + k * (2 * NR * RYS_STRIDE32);
+    p.c = base + k * (LT ? kRysLdsStride32<NR> : kRysStride32<NR>);
   }
   return p;
 }

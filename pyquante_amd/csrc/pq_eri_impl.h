@@ -379,14 +379,39 @@ __device__ __forceinline__ void static_switch(int i, F&& f) {
 //     the ket-side J accumulator.
 constexpr int kWaveSplit = 4;
 
-// Root table of order NR staged in LDS by k_eri_group, in doubles of LDS:
-// the FP64 table for orders 1-3 (7.1, 16.9 and 29.4 KB), the FP32 table for
-// every order (1.2-13.4 KB); 0 = read the global table.
+// Root table of order NR staged in LDS by k_eri_group (padded interval
+// stride), in doubles of LDS: the FP64 table for orders 1-3 (7.4, 17.2 and
+// 29.8 KB), the FP32 table for every order (1.5-14.1 KB); 0 = read the
+// global table.
+template <int NR>
+constexpr int kRysIntervals = NR == 1 ? 37 : NR == 2 ? 44 : NR == 3 ? 51 : NR == 4 ? 58 : 64;  // rys_nint
+template <int NR>
+constexpr int kRysIntervals32 = NR == 1 ? 19 : NR == 2 ? 25 : NR == 3 ? 31 : NR == 4 ? 37 : 42;
 template <int NR, typename T>
-constexpr int kLdsTableDoubles =
-    std::is_same<T, double>::value
-        ? (NR <= 3 ? (NR == 1 ? 37 : NR == 2 ? 44 : 51) * 2 * NR * (RYS_DEG + 1) : 0)  // rys_nint[1..3]
-        : (NR == 1 ? 19 : NR == 2 ? 25 : NR == 3 ? 31 : NR == 4 ? 37 : 42) * 2 * NR * RYS_STRIDE32 / 2;
+constexpr int kLdsTableDoubles = std::is_same<T, double>::value
+                                     ? (NR <= 3 ? kRysIntervals<NR> * kRysLdsStride<NR> : 0)
+                                     : kRysIntervals32<NR> * kRysLdsStride32<NR> / 2;
+
+// Copies the order-NR table into LDS with the padded interval stride.
+template <int NR, typename T>
+__device__ __forceinline__ void stage_rys_table(double* lds) {
+  if constexpr (std::is_same<T, double>::value) {
+    constexpr int S = kRysStride<NR>, N = kRysIntervals<NR> * S;
+    const double* src = rys_coef + rys_off[NR - 1];
+    for (int i = threadIdx.x; i < N; i += blockDim.x) {
+      const int k = i / S;
+      lds[i + k * (kRysLdsStride<NR> - S)] = src[i];
+    }
+  } else {
+    constexpr int S = kRysStride32<NR>, N = kRysIntervals32<NR> * S;
+    const float* src = rys_coef32 + rys_off32[NR - 1];
+    float* t32 = reinterpret_cast<float*>(lds);
+    for (int i = threadIdx.x; i < N; i += blockDim.x) {
+      const int k = i / S;
+      t32[i + k * (kRysLdsStride32<NR> - S)] = src[i];
+    }
+  }
+}
 
 // Unsplit classes with bra groups: one block of 4 wavefronts walks a run of
 // tiles that share the bra pair (wavefront w takes tiles w, w+4, ...), so the
@@ -407,12 +432,7 @@ __global__ __launch_bounds__(256) void k_eri_group(EngineView ev, const Tile* __
   // instead of gathering it from L1/L2 per lane
   constexpr int NT = kLdsTableDoubles<C::NR, T>;
   void* rtab = NT ? pq_lds : nullptr;
-  if constexpr (std::is_same<T, double>::value) {
-    for (int i = threadIdx.x; i < NT; i += blockDim.x) pq_lds[i] = rys_coef[rys_off[C::NR - 1] + i];
-  } else {
-    float* t32 = reinterpret_cast<float*>(pq_lds);
-    for (int i = threadIdx.x; i < 2 * NT; i += blockDim.x) t32[i] = rys_coef32[rys_off32[C::NR - 1] + i];
-  }
+  if constexpr (NT > 0) stage_rys_table<C::NR, T>(pq_lds);
   constexpr int ND = mode_ndens(MODE);
   double* krow = pq_lds + NT;
   for (int i = threadIdx.x; i < ND * (C::NA + C::NB) * n; i += blockDim.x) krow[i] = 0.0;
