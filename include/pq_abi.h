@@ -135,8 +135,11 @@ int pq_engine_set_streams(pq_engine *eng, int nstreams);
  * components, one 256-thread block walks up to `cap` consecutive tiles that
  * share a bra shell pair and flushes its LDS copy of the bra K rows once.
  * cap = 0 disables runs, -1 restores the default (16 tiles when nbf >= 300,
- * else none).  Applies to every class and overrides tuned values.  No
- * reference counterpart (a launch knob). */
+ * else none).  A run length (1..1024) may be or-ed with PQ_CAP_STAGE_D: the
+ * block then also copies the bra rows of the density to LDS (when they fit)
+ * and reads them there.  Applies to every class and overrides tuned values.
+ * No reference counterpart (a launch knob). */
+enum { PQ_CAP_STAGE_D = 4096 };
 int pq_engine_set_group_cap(pq_engine *eng, int cap);
 
 /* Per-class bra-run caps: caps[2*c + prec] for class index c (the order of
@@ -147,7 +150,8 @@ int pq_engine_set_group_caps(pq_engine *eng, const int *caps);
 
 /* Autotuner: for each class and precision of the current work lists, times
  * the J/K kernel on D_dev (`reps` timed launches after one warm-up) for run
- * caps {0, 4, 8, 16, 32} on `stream` and keeps the fastest.  The choice
+ * caps {0, 4, 8, 16, 32}, each run length with and without PQ_CAP_STAGE_D,
+ * on `stream` and keeps the fastest.  The choice
  * persists across pq_build_pairs calls of this engine.  caps_out (optional,
  * 42 ints) receives the chosen cap per (class, precision); classes with more
  * than 81 components report their unused cap.  Overwrites the engine's J/K

@@ -12,6 +12,7 @@ LIB_PATH = os.path.join(_HERE, "libpqhip.so")
 
 PQ_PREC_FP64 = 0
 PQ_PREC_MIXED = 1
+PQ_CAP_STAGE_D = 4096  # bra-run cap flag: stage the bra rows of D in LDS
 
 if not os.path.exists(LIB_PATH):
     raise ImportError("libpqhip.so is not built (run __graft_entry__.build() or make -C pyquante_amd/csrc)")

@@ -80,7 +80,8 @@ struct ClassWork {
   std::vector<int2> g64, g32;  // runs of tiles with one bra pair
   int2* d_g64 = nullptr;
   int2* d_g32 = nullptr;
-  int cap[2] = {0, 0};         // tiles per run (0 = no runs) for FP64 / FP32 tiles
+  int cap[2] = {0, 0};         // tiles per run (0 = no runs) for FP64 / FP32 tiles, | PQ_CAP_STAGE_D
+  int gflags(int prec) const { return (cap[prec] & PQ_CAP_STAGE_D) ? kGroupStageD : 0; }
   uint64_t nq64 = 0, nq32 = 0;
   double np64 = 0, np32 = 0;
   double nf64 = 0, nf32 = 0;  // unique function quartets
@@ -349,9 +350,10 @@ int set_groups(ClassWork& cw, int prec, int cap) {
   g.clear();
   dev_free(d);
   cw.cap[prec] = cap;
-  if (cap <= 0) return PQ_OK;
+  const int run = cap > 0 ? cap & (PQ_CAP_STAGE_D - 1) : 0;
+  if (run <= 0) return PQ_OK;
   for (int i = 0; i < (int)t.size(); ++i) {
-    if (g.empty() || t[i].bra != t[g.back().x].bra || g.back().y >= cap) g.push_back(make_int2(i, 0));
+    if (g.empty() || t[i].bra != t[g.back().x].bra || g.back().y >= run) g.push_back(make_int2(i, 0));
     g.back().y += 1;
   }
   PQ_HIP(dev_alloc(&d, g.size()));
@@ -807,9 +809,16 @@ int pq_engine_set_streams(pq_engine* eng, int nstreams) {
   return PQ_OK;
 }
 
+// -1, 0, or a run length 1..1024 optionally or-ed with PQ_CAP_STAGE_D
+static bool valid_cap(int cap) {
+  if (cap == -1 || cap == 0) return true;
+  const int run = cap & ~PQ_CAP_STAGE_D;
+  return cap > 0 && run >= 1 && run <= 1024;
+}
+
 int pq_engine_set_group_cap(pq_engine* eng, int cap) {
   if (!eng) return fail(PQ_E_ARG, "null engine");
-  if (cap < -1 || cap > 1024) return fail(PQ_E_ARG, "cap must be -1..1024");
+  if (!valid_cap(cap)) return fail(PQ_E_ARG, "cap must be -1, 0 or 1..1024 (optionally | PQ_CAP_STAGE_D)");
   for (auto& a : eng->group_cap)
     for (auto& b : a) b[0] = b[1] = cap;
   if (eng->work_ok)
@@ -824,7 +833,7 @@ int pq_engine_set_group_cap(pq_engine* eng, int cap) {
 int pq_engine_set_group_caps(pq_engine* eng, const int* caps) {
   if (!eng || !caps) return fail(PQ_E_ARG, "null argument");
   for (int i = 0; i < 42; ++i)
-    if (caps[i] < -1 || caps[i] > 1024) return fail(PQ_E_ARG, "caps must be -1..1024");
+    if (!valid_cap(caps[i])) return fail(PQ_E_ARG, "caps must be -1, 0 or 1..1024 (optionally | PQ_CAP_STAGE_D)");
   int ci = 0;
   for (int X = 0; X < 6; ++X)
     for (int Y = 0; Y <= X; ++Y, ++ci)
@@ -843,7 +852,8 @@ int pq_engine_autotune(pq_engine* eng, const double* D_dev, int reps, void* stre
   if (!eng || !D_dev) return fail(PQ_E_ARG, "null argument");
   if (!eng->work_ok) return fail(PQ_E_STATE, "pq_build_pairs not called");
   if (reps < 1 || reps > 100) return fail(PQ_E_ARG, "reps must be 1..100");
-  static const int kCand[] = {0, 4, 8, 16, 32};
+  static const int kCand[] = {0,  4,  8,  16,  32, 4 | PQ_CAP_STAGE_D, 8 | PQ_CAP_STAGE_D, 16 | PQ_CAP_STAGE_D,
+                              32 | PQ_CAP_STAGE_D};
   hipStream_t st = (hipStream_t)stream;
   hipEvent_t e0 = nullptr, e1 = nullptr;
   PQ_HIP(hipEventCreate(&e0));
@@ -867,7 +877,8 @@ int pq_engine_autotune(pq_engine* eng, const double* D_dev, int reps, void* stre
           if (err == hipSuccess)
             err = launch_eri_class(la, lb, lc, ld, MODE_JK, prec, eng->view(), prec ? c.d_t32 : c.d_t64,
                                    (int)tl.size(), eng->d_sorted[c.Y], D_dev, eng->d_Jacc, eng->d_Kacc, nullptr,
-                                   nullptr, st, prec ? c.d_g32 : c.d_g64, (int)(prec ? c.g32 : c.g64).size());
+                                   nullptr, st, prec ? c.d_g32 : c.d_g64, (int)(prec ? c.g32 : c.g64).size(),
+                                   c.gflags(prec));
         }
         if (err == hipSuccess) err = hipEventRecord(e1, st);
         if (err == hipSuccess) err = hipEventSynchronize(e1);
@@ -1100,7 +1111,8 @@ static int fock_jk_impl(pq_engine* eng, int nd, const double* const* Ds, double*
         PQ_TRY(launch_eri_class(la, lb, lc, ld, nd == 2 ? MODE_JK2 : MODE_JK, prec, eng->view(),
                                 prec ? c.d_t32 : c.d_t64,
                                 (int)tl.size(), eng->d_sorted[c.Y], D_dev, eng->d_Jacc, eng->d_Kacc, nullptr,
-                                nullptr, ks, prec ? c.d_g32 : c.d_g64, (int)(prec ? c.g32 : c.g64).size()));
+                                nullptr, ks, prec ? c.d_g32 : c.d_g64, (int)(prec ? c.g32 : c.g64).size(),
+                                c.gflags(prec)));
         if (ms) {
           PQ_TRY(mark(ks));
           slot.push_back((int)(2 * ci + prec));

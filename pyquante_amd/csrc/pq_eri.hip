@@ -10,7 +10,7 @@ namespace pq {
   hipError_t launch_cls_##A##B##C_##D_##_m##M(const EngineView& ev, const Tile* tiles, int ntiles,         \
                                               const int* kets, const double* D, double* Jacc, double* Kacc, \
                                               double* ints, double* qout, hipStream_t st, const int2* groups, \
-                                              int ngroups);
+                                              int ngroups, int gflags);
 #define PQ_DECL(A, B, C_, D_) \
   PQ_DECL1(A, B, C_, D_, 0) PQ_DECL1(A, B, C_, D_, 1) PQ_DECL1(A, B, C_, D_, 2) PQ_DECL1(A, B, C_, D_, 5)
 PQ_DECL(0, 0, 0, 0)
@@ -195,7 +195,7 @@ hipError_t launch_shell_blocks(int nq, const double* ints, const int* first, con
 
 static hipError_t launch_diag_0000(const EngineView& ev, const Tile* tiles, int ntiles, const int* kets, const double* D,
                                    double* Jacc, double* Kacc, double* ints, double* qout, hipStream_t st) {
-  return launch_cls_0000_m3(ev, tiles, ntiles, kets, D, Jacc, Kacc, ints, qout, st, nullptr, 0);
+  return launch_cls_0000_m3(ev, tiles, ntiles, kets, D, Jacc, Kacc, ints, qout, st, nullptr, 0, 0);
 }
 static hipError_t launch_diag_1000(const EngineView&, const Tile*, int, const int*, const double*, double*, double*,
                                    double*, double*, hipStream_t) {
@@ -203,7 +203,7 @@ static hipError_t launch_diag_1000(const EngineView&, const Tile*, int, const in
 }
 static hipError_t launch_diag_1010(const EngineView& ev, const Tile* tiles, int ntiles, const int* kets, const double* D,
                                    double* Jacc, double* Kacc, double* ints, double* qout, hipStream_t st) {
-  return launch_cls_1010_m3(ev, tiles, ntiles, kets, D, Jacc, Kacc, ints, qout, st, nullptr, 0);
+  return launch_cls_1010_m3(ev, tiles, ntiles, kets, D, Jacc, Kacc, ints, qout, st, nullptr, 0, 0);
 }
 static hipError_t launch_diag_1100(const EngineView&, const Tile*, int, const int*, const double*, double*, double*,
                                    double*, double*, hipStream_t) {
@@ -215,7 +215,7 @@ static hipError_t launch_diag_1110(const EngineView&, const Tile*, int, const in
 }
 static hipError_t launch_diag_1111(const EngineView& ev, const Tile* tiles, int ntiles, const int* kets, const double* D,
                                    double* Jacc, double* Kacc, double* ints, double* qout, hipStream_t st) {
-  return launch_cls_1111_m3(ev, tiles, ntiles, kets, D, Jacc, Kacc, ints, qout, st, nullptr, 0);
+  return launch_cls_1111_m3(ev, tiles, ntiles, kets, D, Jacc, Kacc, ints, qout, st, nullptr, 0, 0);
 }
 static hipError_t launch_diag_2000(const EngineView&, const Tile*, int, const int*, const double*, double*, double*,
                                    double*, double*, hipStream_t) {
@@ -231,7 +231,7 @@ static hipError_t launch_diag_2011(const EngineView&, const Tile*, int, const in
 }
 static hipError_t launch_diag_2020(const EngineView& ev, const Tile* tiles, int ntiles, const int* kets, const double* D,
                                    double* Jacc, double* Kacc, double* ints, double* qout, hipStream_t st) {
-  return launch_cls_2020_m3(ev, tiles, ntiles, kets, D, Jacc, Kacc, ints, qout, st, nullptr, 0);
+  return launch_cls_2020_m3(ev, tiles, ntiles, kets, D, Jacc, Kacc, ints, qout, st, nullptr, 0, 0);
 }
 static hipError_t launch_diag_2100(const EngineView&, const Tile*, int, const int*, const double*, double*, double*,
                                    double*, double*, hipStream_t) {
@@ -251,7 +251,7 @@ static hipError_t launch_diag_2120(const EngineView&, const Tile*, int, const in
 }
 static hipError_t launch_diag_2121(const EngineView& ev, const Tile* tiles, int ntiles, const int* kets, const double* D,
                                    double* Jacc, double* Kacc, double* ints, double* qout, hipStream_t st) {
-  return launch_cls_2121_m3(ev, tiles, ntiles, kets, D, Jacc, Kacc, ints, qout, st, nullptr, 0);
+  return launch_cls_2121_m3(ev, tiles, ntiles, kets, D, Jacc, Kacc, ints, qout, st, nullptr, 0, 0);
 }
 static hipError_t launch_diag_2200(const EngineView&, const Tile*, int, const int*, const double*, double*, double*,
                                    double*, double*, hipStream_t) {
@@ -275,23 +275,23 @@ static hipError_t launch_diag_2221(const EngineView&, const Tile*, int, const in
 }
 static hipError_t launch_diag_2222(const EngineView& ev, const Tile* tiles, int ntiles, const int* kets, const double* D,
                                    double* Jacc, double* Kacc, double* ints, double* qout, hipStream_t st) {
-  return launch_cls_2222_m3(ev, tiles, ntiles, kets, D, Jacc, Kacc, ints, qout, st, nullptr, 0);
+  return launch_cls_2222_m3(ev, tiles, ntiles, kets, D, Jacc, Kacc, ints, qout, st, nullptr, 0, 0);
 }
 
 hipError_t launch_eri_class(int la, int lb, int lc, int ld, int mode, int fp32, const EngineView& ev,
                             const Tile* tiles, int ntiles, const int* kets, const double* D, double* Jacc,
                             double* Kacc, double* ints, double* qout, hipStream_t st, const int2* groups,
-                            int ngroups) {
+                            int ngroups, int gflags) {
 #define PQ_CASE(A, B, C_, D_)                                                                      \
   if (la == A && lb == B && lc == C_ && ld == D_) {                                                \
     if (mode == MODE_JK && !fp32)                                                                  \
-      return launch_cls_##A##B##C_##D_##_m0(ev, tiles, ntiles, kets, D, Jacc, Kacc, ints, qout, st, groups, ngroups); \
+      return launch_cls_##A##B##C_##D_##_m0(ev, tiles, ntiles, kets, D, Jacc, Kacc, ints, qout, st, groups, ngroups, gflags); \
     if (mode == MODE_JK)                                                                           \
-      return launch_cls_##A##B##C_##D_##_m1(ev, tiles, ntiles, kets, D, Jacc, Kacc, ints, qout, st, groups, ngroups); \
+      return launch_cls_##A##B##C_##D_##_m1(ev, tiles, ntiles, kets, D, Jacc, Kacc, ints, qout, st, groups, ngroups, gflags); \
     if (mode == MODE_STORE)                                                                        \
-      return launch_cls_##A##B##C_##D_##_m2(ev, tiles, ntiles, kets, D, Jacc, Kacc, ints, qout, st, nullptr, 0); \
+      return launch_cls_##A##B##C_##D_##_m2(ev, tiles, ntiles, kets, D, Jacc, Kacc, ints, qout, st, nullptr, 0, 0); \
     if (mode == MODE_JK2)                                                                          \
-      return launch_cls_##A##B##C_##D_##_m5(ev, tiles, ntiles, kets, D, Jacc, Kacc, ints, qout, st, groups, ngroups); \
+      return launch_cls_##A##B##C_##D_##_m5(ev, tiles, ntiles, kets, D, Jacc, Kacc, ints, qout, st, groups, ngroups, gflags); \
     return launch_diag_##A##B##C_##D_(ev, tiles, ntiles, kets, D, Jacc, Kacc, ints, qout, st);      \
   }
   PQ_CASE(0, 0, 0, 0)

@@ -20,22 +20,22 @@ hipError_t PQ_CAT(PQ_LA, PQ_LB, PQ_LC, PQ_LD, PQ_MODE)(const EngineView& ev, con
 hipError_t PQ_CAT(PQ_LA, PQ_LB, PQ_LC, PQ_LD, PQ_MODE)(const EngineView& ev, const Tile* tiles, int ntiles,
                                                        const int* kets, const double* D, double* Jacc,
                                                        double* Kacc, double* ints, double* qout, hipStream_t st,
-                                                       const int2* groups, int ngroups) {
+                                                       const int2* groups, int ngroups, int gflags) {
 #if PQ_MODE == 0
   return launch_one<PQ_LA, PQ_LB, PQ_LC, PQ_LD, double, MODE_JK>(ev, tiles, ntiles, kets, D, Jacc, Kacc, ints,
-                                                                 qout, st, groups, ngroups);
+                                                                 qout, st, groups, ngroups, gflags);
 #elif PQ_MODE == 1
   return launch_one<PQ_LA, PQ_LB, PQ_LC, PQ_LD, float, MODE_JK>(ev, tiles, ntiles, kets, D, Jacc, Kacc, ints,
-                                                                qout, st, groups, ngroups);
+                                                                qout, st, groups, ngroups, gflags);
 #elif PQ_MODE == 2
   return launch_one<PQ_LA, PQ_LB, PQ_LC, PQ_LD, double, MODE_STORE>(ev, tiles, ntiles, kets, D, Jacc, Kacc, ints,
-                                                                    qout, st, nullptr, 0);
+                                                                    qout, st, nullptr, 0, 0);
 #elif PQ_MODE == 5
   return launch_one<PQ_LA, PQ_LB, PQ_LC, PQ_LD, double, MODE_JK2>(ev, tiles, ntiles, kets, D, Jacc, Kacc, ints,
-                                                                  qout, st, groups, ngroups);
+                                                                  qout, st, groups, ngroups, gflags);
 #else
   return launch_one<PQ_LA, PQ_LB, PQ_LC, PQ_LD, double, MODE_DIAG>(ev, tiles, ntiles, kets, D, Jacc, Kacc, ints,
-                                                                   qout, st, nullptr, 0);
+                                                                   qout, st, nullptr, 0, 0);
 #endif
 }
 #endif

@@ -128,12 +128,27 @@ __device__ __forceinline__ void eri_block(const EngineView& ev, const PairInfo& 
     return UB ? ld_uniform(ev.prims + bp.pp_off + i) : ev.prims[bp.pp_off + i];
   };
   // ket primitives (per lane) outside, bra primitives (shared by the
-  // wavefront, scalar loads) inside
+  // wavefront, scalar loads) inside.  Light classes load the next ket
+  // primitive pair while the current one is processed (its latency is then
+  // hidden behind the bra loop); heavy ones keep the registers.
+  constexpr bool PF = NAS * NBS * C::NC * C::ND <= 27;
+  if constexpr (PF) {
+    if (k0 >= k1) return;
+    PrimPair q = ev.prims[kp.pp_off + k0];
 #pragma unroll 1
-  for (int k = k0; k < k1; ++k) {
-    const PrimPair q = ev.prims[kp.pp_off + k];
+    for (int k = k0; k < k1; ++k) {
+      const PrimPair qn = ev.prims[kp.pp_off + (k + 1 < k1 ? k + 1 : k)];
 #pragma unroll 1
-    for (int i = i0; i < i1; ++i) quartet(bra_prim(i), q);
+      for (int i = i0; i < i1; ++i) quartet(bra_prim(i), q);
+      q = qn;
+    }
+  } else {
+#pragma unroll 1
+    for (int k = k0; k < k1; ++k) {
+      const PrimPair q = ev.prims[kp.pp_off + k];
+#pragma unroll 1
+      for (int i = i0; i < i1; ++i) quartet(bra_prim(i), q);
+    }
   }
 }
 
@@ -152,12 +167,15 @@ __device__ __forceinline__ void eri_block(const EngineView& ev, const PairInfo& 
 // block as coalesced global atomics (flush_block), one cache line per 16
 // columns instead of one per lane.  jsum, when non-null, is the block's LDS
 // accumulator of the ket-side J block per lane (shared by the component
-// splits of one tile), flushed once per block.
+// splits of one tile), flushed once per block.  drow, when non-null, is the
+// block's LDS copy of the bra rows of D (same layout as krow): the D elements
+// of the bra rows then come from LDS and only D[c,d] is gathered from memory.
 template <int LA, int LB, int LC, int LD, int A0, int NAS, int B0, int NBS, typename T>
 __device__ __forceinline__ void digest_lane(const EngineView& ev, const PairInfo& bp, const PairInfo& kp,
                                             const T (&out)[NAS * NBS * ncart(LC) * ncart(LD)], double f,
                                             bool active, const double* __restrict__ D, double* Jacc,
-                                            double* Kacc, double* krow, double* jsum) {
+                                            double* Kacc, double* krow, double* jsum,
+                                            const double* drow = nullptr) {
   using C = Cls<LA, LB, LC, LD>;
   // the quartet's molecule block (bra and ket share it)
   const int n = bp.mstride;
@@ -167,6 +185,26 @@ __device__ __forceinline__ void digest_lane(const EngineView& ev, const PairInfo
   const int a0 = bp.bfA + A0, b0 = bp.bfB + B0, c0 = kp.bfA, d0 = kp.bfB;
   double jab[NAS][NBS], jcd[C::NC][C::ND], kac[NAS][C::NC], kad[NAS][C::ND], kbc[NBS][C::NC], kbd[NBS][C::ND];
   double Dab[NAS][NBS], Dcd[C::NC][C::ND], Dac[NAS][C::NC], Dad[NAS][C::ND], Dbc[NBS][C::NC], Dbd[NBS][C::ND];
+  if (drow) {
+#pragma unroll
+    for (int a = 0; a < NAS; ++a) {
+      const double* ra = drow + (A0 + a) * n;
+#pragma unroll
+      for (int b = 0; b < NBS; ++b) { Dab[a][b] = ra[b0 + b]; jab[a][b] = 0; }
+#pragma unroll
+      for (int c = 0; c < C::NC; ++c) { Dac[a][c] = ra[c0 + c]; kac[a][c] = 0; }
+#pragma unroll
+      for (int d = 0; d < C::ND; ++d) { Dad[a][d] = ra[d0 + d]; kad[a][d] = 0; }
+    }
+#pragma unroll
+    for (int b = 0; b < NBS; ++b) {
+      const double* rb = drow + (C::NA + B0 + b) * n;
+#pragma unroll
+      for (int c = 0; c < C::NC; ++c) { Dbc[b][c] = rb[c0 + c]; kbc[b][c] = 0; }
+#pragma unroll
+      for (int d = 0; d < C::ND; ++d) { Dbd[b][d] = rb[d0 + d]; kbd[b][d] = 0; }
+    }
+  } else {
 #pragma unroll
   for (int a = 0; a < NAS; ++a) {
 #pragma unroll
@@ -182,6 +220,7 @@ __device__ __forceinline__ void digest_lane(const EngineView& ev, const PairInfo
     for (int c = 0; c < C::NC; ++c) { Dbc[b][c] = D[(b0 + b) * n + c0 + c]; kbc[b][c] = 0; }
 #pragma unroll
     for (int d = 0; d < C::ND; ++d) { Dbd[b][d] = D[(b0 + b) * n + d0 + d]; kbd[b][d] = 0; }
+  }
   }
 #pragma unroll
   for (int c = 0; c < C::NC; ++c)
@@ -292,7 +331,8 @@ template <int LA, int LB, int LC, int LD, typename T, int MODE, int A0, int NAS,
 __device__ __forceinline__ void lane_body(const EngineView& ev, const Tile& tl, bool active, int bra, int ket,
                                           const double* __restrict__ D, double* Jacc, double* Kacc,
                                           double* ints, double* qout, double* krow, double* jsum,
-                                          const void* rtab = nullptr, int kc = 0, int ks = 1, int nk = 64) {
+                                          const void* rtab = nullptr, int kc = 0, int ks = 1, int nk = 64,
+                                          const double* drow = nullptr) {
   using C = Cls<LA, LB, LC, LD>;
   constexpr bool UB = MODE != MODE_DIAG;  // Schwarz diagonal: bra = ket per lane
   const PairInfo bp = UB ? ld_uniform(ev.pairs + bra) : ev.pairs[bra];
@@ -329,7 +369,8 @@ __device__ __forceinline__ void lane_body(const EngineView& ev, const Tile& tl, 
     for (int d = 0; d < mode_ndens(MODE); ++d)
       digest_lane<LA, LB, LC, LD, A0, NAS, B0, NBS, T>(
           ev, bp, kp, out, f, active, D + d * nn, Jacc + d * nn, Kacc + d * nn,
-          krow ? krow + d * (C::NA + C::NB) * ev.nbf : nullptr, jsum ? jsum + d * 64 * C::NC * C::ND : nullptr);
+          krow ? krow + d * (C::NA + C::NB) * ev.nbf : nullptr, jsum ? jsum + d * 64 * C::NC * C::ND : nullptr,
+          drow ? drow + d * (C::NA + C::NB) * ev.nbf : nullptr);
   } else if constexpr (MODE == MODE_STORE) {
     if (!active) return;
     const bool same = bra == ket;
@@ -415,12 +456,13 @@ __device__ __forceinline__ void stage_rys_table(double* lds) {
 
 // Unsplit classes with bra groups: one block of 4 wavefronts walks a run of
 // tiles that share the bra pair (wavefront w takes tiles w, w+4, ...), so the
-// LDS K rows are zeroed and flushed once per run.
+// LDS K rows are zeroed and flushed once per run.  stage_d: the bra rows of D
+// are copied to LDS once per run as well.
 template <int LA, int LB, int LC, int LD, typename T, int MODE>
 __global__ __launch_bounds__(256) void k_eri_group(EngineView ev, const Tile* __restrict__ tiles,
                                                    const int2* __restrict__ groups, int ngroups,
                                                    const int* __restrict__ kets, const double* __restrict__ D,
-                                                   double* Jacc, double* Kacc) {
+                                                   double* Jacc, double* Kacc, int stage_d) {
   using C = Cls<LA, LB, LC, LD>;
   static_assert(C::SPLIT == 0, "bra groups are for unsplit classes");
   extern __shared__ double pq_lds[];
@@ -436,6 +478,19 @@ __global__ __launch_bounds__(256) void k_eri_group(EngineView ev, const Tile* __
   constexpr int ND = mode_ndens(MODE);
   double* krow = pq_lds + NT;
   for (int i = threadIdx.x; i < ND * (C::NA + C::NB) * n; i += blockDim.x) krow[i] = 0.0;
+  const PairInfo bp = ld_uniform(ev.pairs + tiles[g.x].bra);
+  const int ms = bp.mstride;
+  const int rows = (C::NA + C::NB) * ms, region = (C::NA + C::NB) * n;
+  double* drow = nullptr;
+  if (stage_d) {
+    drow = krow + ND * region;
+    for (int i = threadIdx.x; i < ND * rows; i += blockDim.x) {
+      const int d = i / rows, ir = i - d * rows;
+      const int r = ir / ms, col = ir - r * ms;
+      const int grow = r < C::NA ? bp.bfA + r : bp.bfB + (r - C::NA);
+      drow[d * region + ir] = D[(size_t)d * ev.msize + bp.moff + grow * ms + col];
+    }
+  }
   __syncthreads();
   for (int tt = wv; tt < g.y; tt += 4) {
     Tile tl = ld_uniform(tiles + g.x + tt);
@@ -447,12 +502,9 @@ __global__ __launch_bounds__(256) void k_eri_group(EngineView ev, const Tile* __
     const int ket = kets[tl.ket0 + tn.slot];
     lane_body<LA, LB, LC, LD, T, MODE, 0, C::NA, 0, C::NB, (NT > 0)>(ev, tl, tn.active, tl.bra, ket, D, Jacc,
                                                                        Kacc, nullptr, nullptr, krow, nullptr, rtab,
-                                                                       tn.kc, tn.ks, tn.nk);
+                                                                       tn.kc, tn.ks, tn.nk, drow);
   }
   __syncthreads();
-  const PairInfo bp = ev.pairs[tiles[g.x].bra];
-  const int ms = bp.mstride;
-  const int rows = (C::NA + C::NB) * ms, region = (C::NA + C::NB) * n;
   for (int i = threadIdx.x; i < ND * rows; i += blockDim.x) {
     const int d = i / rows, ir = i - d * rows;
     const double v = krow[d * region + ir];
@@ -553,16 +605,18 @@ __global__ __launch_bounds__(256) void k_eri_lane(EngineView ev, const Tile* __r
 template <int LA, int LB, int LC, int LD, typename T, int MODE>
 static hipError_t launch_one(const EngineView& ev, const Tile* tiles, int ntiles, const int* kets,
                              const double* D, double* Jacc, double* Kacc, double* ints, double* qout,
-                             hipStream_t st, const int2* groups, int ngroups) {
+                             hipStream_t st, const int2* groups, int ngroups, int gflags) {
   if (ntiles <= 0) return hipSuccess;
   using C = Cls<LA, LB, LC, LD>;
   const size_t lds = mode_ndens(MODE) * ((size_t)(C::NA + C::NB) * ev.nbf + (C::SPLIT ? 64 * C::NC * C::ND : 0)) *
                      sizeof(double);
   if constexpr (C::SPLIT == 0 && mode_is_jk(MODE)) {
     const size_t glds = lds + kLdsTableDoubles<C::NR, T> * sizeof(double);
+    // the staged D rows take as much LDS as the K rows
+    const int stage_d = (gflags & kGroupStageD) && glds + lds <= 65536;
     if (groups && ngroups > 0 && glds <= 65536) {
-      hipLaunchKernelGGL((k_eri_group<LA, LB, LC, LD, T, MODE>), dim3(ngroups), dim3(256), glds, st, ev, tiles,
-                         groups, ngroups, kets, D, Jacc, Kacc);
+      hipLaunchKernelGGL((k_eri_group<LA, LB, LC, LD, T, MODE>), dim3(ngroups), dim3(256),
+                         stage_d ? glds + lds : glds, st, ev, tiles, groups, ngroups, kets, D, Jacc, Kacc, stage_d);
       return hipGetLastError();
     }
   }

@@ -20,11 +20,14 @@ constexpr int kLaneMaxComp = 81;
 constexpr int kGradCopies = 256;
 
 // groups (J/K mode, unsplit classes): ngroups entries of (first tile, tile
-// count), each a run of tiles with the same bra pair processed by one block
+// count), each a run of tiles with the same bra pair processed by one block;
+// gflags & kGroupStageD: those blocks also stage the bra rows of D in LDS
+// (when they fit)
+constexpr int kGroupStageD = 1;
 hipError_t launch_eri_class(int la, int lb, int lc, int ld, int mode, int fp32, const EngineView& ev,
                             const Tile* tiles, int ntiles, const int* kets, const double* D, double* Jacc,
                             double* Kacc, double* ints, double* qout, hipStream_t st,
-                            const int2* groups = nullptr, int ngroups = 0);
+                            const int2* groups = nullptr, int ngroups = 0, int gflags = 0);
 
 // two-electron nuclear gradient into kGradCopies private copies of the
 // natom x 3 gradient (gacc), summed by launch_grad_finish

@@ -124,7 +124,8 @@ class Engine(object):
         check(lib.pq_engine_set_streams(self.handle, int(n)))
 
     def set_group_cap(self, cap):
-        """Tiles per bra run for every class (0 = none, -1 = default)."""
+        """Tiles per bra run for every class (0 = none, -1 = default); a run
+        length may be or-ed with _abi.PQ_CAP_STAGE_D (bra rows of D in LDS)."""
         check(lib.pq_engine_set_group_cap(self.handle, int(cap)))
 
     def set_group_caps(self, caps):
@@ -135,7 +136,8 @@ class Engine(object):
 
     def autotune(self, D, reps=3):
         """Times each class's J/K kernel on density D for the candidate bra
-        run lengths and keeps the fastest (pq_engine_autotune); returns the
+        run lengths, with and without staged D rows, and keeps the fastest
+        (pq_engine_autotune); returns the
         chosen cap per (class, precision) as a 21 x 2 list."""
         if self.cut is None:
             self.build()

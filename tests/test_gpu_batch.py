@@ -30,6 +30,13 @@ def test_batch_fock_energies_forces_match_single_engines():
         Dblocks.append(C @ C.T)
     D = torch.as_tensor(np.concatenate([b.reshape(-1) for b in Dblocks]), device="cuda")
     J, K = be.fock_jk(D)
+    # bra runs with staged D rows over the packed molecule blocks
+    from pyquante_amd import _abi
+    be.set_group_cap(4 | _abi.PQ_CAP_STAGE_D)
+    Jr, Kr = be.fock_jk(D)
+    be.set_group_cap(-1)
+    assert torch.max(torch.abs(Jr - J)).item() < 1e-11
+    assert torch.max(torch.abs(Kr - K)).item() < 1e-11
     for m, mol in enumerate(mols):
         eng = Engine(BasisSet(mol, "6-31g**"), mol)
         eng.build(1e-12, 0.0)

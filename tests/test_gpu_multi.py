@@ -14,7 +14,7 @@ def _density(n, seed, k=12):
     return torch.as_tensor(C @ C.T, device="cuda")
 
 
-@pytest.mark.parametrize("cap", [0, 4])
+@pytest.mark.parametrize("cap", [0, 4, 4 | 4096])
 def test_two_density_pass_matches_single_builds(cap):
     from pyquante_amd.basis import BasisSet
     from pyquante_amd.engine import Engine

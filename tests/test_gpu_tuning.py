@@ -41,7 +41,8 @@ def test_bra_runs_match_plain_launch():
         if fp32 == 0.0:
             assert torch.max(torch.abs(J0 - Js)).item() < 1e-11
             assert torch.max(torch.abs(K0 - Ks)).item() < 1e-11
-        for cap in (1, 3, 4, 16, 1024):
+        S = _abi.PQ_CAP_STAGE_D
+        for cap in (1, 3, 4, 16, 1024, 1 | S, 4 | S, 1024 | S):
             eng.set_group_cap(cap)
             J, K = eng.fock_jk(D, precision=prec)
             print("bra runs: cut %g fp32 %g cap %d" % (cut, fp32, cap), flush=True)
@@ -63,7 +64,8 @@ def test_autotune_choices_and_results():
     J0, K0 = eng.fock_jk(D, precision=_abi.PQ_PREC_MIXED)
     caps = eng.autotune(D, reps=1)
     assert len(caps) == 21
-    assert all(c in (0, 4, 8, 16, 32) for pair in caps for c in pair)
+    S = _abi.PQ_CAP_STAGE_D
+    assert all(c in (0, 4, 8, 16, 32, 4 | S, 8 | S, 16 | S, 32 | S) for pair in caps for c in pair)
     J, K = eng.fock_jk(D, precision=_abi.PQ_PREC_MIXED)
     assert torch.max(torch.abs(J - J0)).item() < 1e-11
     assert torch.max(torch.abs(K - K0)).item() < 1e-11
