@@ -125,6 +125,17 @@ int pq_engine_set_geometry(pq_engine *eng, const double *atom_xyz_host);
  * builds, quartets with bound below fp32_cut run in FP32. */
 int pq_build_pairs(pq_engine *eng, double schwarz_cut, double fp32_cut);
 
+/* Primitive screening of the J/K and gradient work (takes effect at the
+ * next pq_build_pairs): with a Schwarz cut > 0, primitive pairs whose factor
+ * max sqrt|(ab|ab)| times the largest such factor is below rel * cut are
+ * skipped, so every skipped primitive quartet is bounded by rel * cut.  The
+ * default rel = 1e-4 keeps that bound below the shell quartets the cut
+ * itself drops; rel = 0 keeps every primitive.  Stored integrals
+ * (pq_get2ints) and one-electron integrals always use every primitive.  The
+ * reference contracts every primitive (Ints.py:146-167 over
+ * cints.contr_coulomb); this is the engine's own screening, like the cut. */
+int pq_engine_set_prim_screen(pq_engine *eng, double rel);
+
 /* Number of engine-owned streams (0..4, default 4) the class kernels of one
  * Fock build are spread over; 0 launches everything on the caller's stream
  * (one kernel at a time, e.g. for per-kernel profiling).  No reference

@@ -109,7 +109,19 @@ struct EngineView {
   int nbf;               // largest molecule's function count (LDS row length)
   int64_t msize;         // elements of one packed matrix (sum of n_m^2)
   int maxat;             // largest molecule's atom count
+  // ket-side J of the J/K kernels, indexed by ket position instead of matrix
+  // element so that the lanes of a tile add into consecutive addresses: ket
+  // pair class Y's block starts at jp_off[Y] and holds component cd of the
+  // pair at position j of that class's ket order at cd * jp_n[Y] + j; one
+  // density's array is jp_size long
+  double* jpair;
+  int64_t jp_size;
+  int jp_off[6];
+  int jp_n[6];
 };
+
+// pair class of shells with l1 >= l2 (ss, ps, pp, ds, dp, dd)
+constexpr int pair_class_of(int l1, int l2) { return l1 * (l1 + 1) / 2 + l2; }
 
 // ---------------------------------------------------------------------
 // Rys roots u_i = t_i^2 and weights w_i for order NR at argument X.
@@ -384,10 +396,30 @@ __device__ __forceinline__ T ld_uniform(const T* p) {
   return r;
 }
 
+// v moved across lanes by the DPP control CTRL; lanes of rows outside
+// ROW_MASK (and lanes without a source) receive 0
+template <int CTRL, int ROW_MASK = 0xf>
+__device__ __forceinline__ double dpp_move(double v) {
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_update_dpp(0, (int)b, CTRL, ROW_MASK, 0xf, false);
+  const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), CTRL, ROW_MASK, 0xf, false);
+  return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+
+// Sum over the 64 lanes of a wavefront (every lane active), returned to all
+// lanes: quad permutations, half-row and row mirrors, then the row
+// broadcasts 15 and 31 gather the total in lane 63 without leaving the VALU.
 __device__ __forceinline__ double wave_sum(double v) {
-#pragma unroll
-  for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off, 64);
-  return v;
+  v += dpp_move<0xb1>(v);        // quad_perm [1,0,3,2]
+  v += dpp_move<0x4e>(v);        // quad_perm [2,3,0,1]
+  v += dpp_move<0x141>(v);       // row_half_mirror: 8-lane sums
+  v += dpp_move<0x140>(v);       // row_mirror: 16-lane row sums
+  v += dpp_move<0x142, 0xa>(v);  // row_bcast:15 into rows 1 and 3
+  v += dpp_move<0x143, 0xc>(v);  // row_bcast:31 into rows 2 and 3
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_readlane((int)b, 63);
+  const int hi = __builtin_amdgcn_readlane((int)(b >> 32), 63);
+  return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
 }
 
 }  // namespace pq

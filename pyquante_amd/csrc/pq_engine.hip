@@ -6,6 +6,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
+#include <cstdint>
 #include <cstring>
 #include <numeric>
 #include <string>
@@ -108,6 +109,23 @@ struct pq_engine {
   double* d_prim_a = nullptr;
   int* d_members[6] = {};
   int* d_sorted[6] = {};
+  // primitive screening: Schwarz factor of every primitive pair (each pair's
+  // primitives are stored in decreasing order of it), and the J/K and
+  // gradient copy of the pairs whose pp_n counts only the primitives that
+  // survive the current cut, with its own per-class ket order
+  std::vector<double> qprim;
+  double qprim_max = 0;
+  std::vector<PairInfo> pairs_s;
+  std::vector<std::vector<int>> sorted_s;
+  // each dropped primitive quartet is bounded by prim_rel * cut, below the
+  // quartets the shell-level cut drops
+  double prim_rel = 1e-4;
+  PairInfo* d_pairs_s = nullptr;
+  int* d_sorted_s[6] = {};
+  // position-indexed ket J (EngineView::jpair), room for two densities
+  double* d_jpair = nullptr;
+  int64_t jp_size = 0;
+  int jp_off[6] = {}, jp_n[6] = {};
   double* d_atom_xyz = nullptr;
   double* d_atom_z = nullptr;
 
@@ -153,6 +171,18 @@ struct pq_engine {
 
   int maxat = 0;      // largest molecule's atom count
   EngineView view() const { return EngineView{d_pairs, d_prims, d_prim_a, maxn, msize, maxat}; }
+  // pairs with the screened primitive counts and the ket J (J/K and
+  // gradient kernels)
+  EngineView view_s() const {
+    EngineView v{d_pairs_s, d_prims, d_prim_a, maxn, msize, maxat};
+    v.jpair = d_jpair;
+    v.jp_size = jp_size;
+    for (int y = 0; y < 6; ++y) {
+      v.jp_off[y] = jp_off[y];
+      v.jp_n[y] = jp_n[y];
+    }
+    return v;
+  }
 
   // per-function primitive data for the HGP path (built on first use, dropped
   // with the geometry)
@@ -186,9 +216,13 @@ struct pq_engine {
     dev_free(d_pairs);
     dev_free(d_prims);
     dev_free(d_prim_a);
+    dev_free(d_pairs_s);
+    dev_free(d_jpair);
+    jp_size = 0;
     for (int i = 0; i < 6; ++i) {
       dev_free(d_members[i]);
       dev_free(d_sorted[i]);
+      dev_free(d_sorted_s[i]);
     }
     free_work(work);
     free_work(store_work);
@@ -218,6 +252,101 @@ struct pq_engine {
 };
 
 namespace {
+
+// Schwarz factors max over components sqrt|(ab|ab)| of the pairs ids[c] (per
+// pair class c, indices into the device pair array d_list of n entries)
+// into q (n entries, 0 for pairs not listed).
+int diag_factors(pq_engine* e, const PairInfo* d_list, size_t n, const std::vector<std::vector<int>>& ids,
+                 std::vector<double>& q) {
+  EngineView v = e->view();
+  v.pairs = d_list;
+  double* d_q = nullptr;
+  PQ_HIP(dev_alloc(&d_q, n));
+  PQ_HIP(launch_fill(d_q, (int64_t)n, 0.0, 0));
+  int rc = PQ_OK;
+  for (int c = 0; c < 6 && rc == PQ_OK; ++c) {
+    const int np = (int)ids[c].size();
+    if (!np) continue;
+    std::vector<Tile> tiles;
+    for (int k = 0; k < np; k += 64) tiles.push_back(Tile{-1, k, std::min(64, np - k), 0});
+    Tile* d_tiles = nullptr;
+    int* d_ids = nullptr;
+    hipError_t err = dev_alloc(&d_tiles, tiles.size());
+    if (err == hipSuccess) err = dev_alloc(&d_ids, (size_t)np);
+    if (err == hipSuccess)
+      err = hipMemcpy(d_tiles, tiles.data(), tiles.size() * sizeof(Tile), hipMemcpyHostToDevice);
+    if (err == hipSuccess) err = hipMemcpy(d_ids, ids[c].data(), np * sizeof(int), hipMemcpyHostToDevice);
+    if (err == hipSuccess)
+      err = launch_eri_class(kPairL[c][0], kPairL[c][1], kPairL[c][0], kPairL[c][1], MODE_DIAG, 0, v, d_tiles,
+                             (int)tiles.size(), d_ids, nullptr, nullptr, nullptr, nullptr, d_q, 0);
+    if (err == hipSuccess) err = hipDeviceSynchronize();
+    dev_free(d_tiles);
+    dev_free(d_ids);
+    if (err != hipSuccess) rc = fail(PQ_E_HIP, hipGetErrorString(err));
+  }
+  q.assign(n, 0.0);
+  if (rc == PQ_OK) {
+    hipError_t err = hipMemcpy(q.data(), d_q, n * sizeof(double), hipMemcpyDeviceToHost);
+    if (err != hipSuccess) rc = fail(PQ_E_HIP, hipGetErrorString(err));
+  }
+  dev_free(d_q);
+  return rc;
+}
+
+// Per pair class, the pair ids sorted by molecule (a bra's kets are its own
+// molecule's), then contraction length in P (uniform primitive loops within a
+// tile), then Q descending; uploaded to d_sorted.
+int sort_pairs(pq_engine* e, const std::vector<PairInfo>& P, std::vector<std::vector<int>>& sorted,
+               int* (&d_sorted)[6]) {
+  sorted.assign(6, {});
+  for (int c = 0; c < 6; ++c) {
+    auto v = e->members[c];
+    std::stable_sort(v.begin(), v.end(), [&](int a, int b) {
+      const int ma = e->sh_mol[P[a].shA], mb = e->sh_mol[P[b].shA];
+      if (ma != mb) return ma < mb;
+      if (P[a].pp_n != P[b].pp_n) return P[a].pp_n > P[b].pp_n;
+      return e->Q[a] > e->Q[b];
+    });
+    sorted[c] = v;
+    dev_free(d_sorted[c]);
+    PQ_HIP(dev_alloc(&d_sorted[c], v.size()));
+    if (!v.empty()) PQ_HIP(hipMemcpy(d_sorted[c], v.data(), v.size() * sizeof(int), hipMemcpyHostToDevice));
+  }
+  return PQ_OK;
+}
+
+// The screened pair copy for a shell-quartet cut: primitive pairs are
+// dropped from the J/K and gradient work when their Schwarz factor times the
+// largest one is below e->prim_rel * cut (cut = 0 or prim_rel = 0 keeps every
+// primitive); pp_n = the primitives that pass, at least one.
+int build_screened(pq_engine* e, double cut) {
+  const double tau = cut > 0.0 ? cut * e->prim_rel : 0.0;
+  e->pairs_s = e->pairs;
+  if (tau > 0.0)
+    for (auto& p : e->pairs_s) {
+      int n = 1;
+      while (n < p.pp_n && e->qprim[p.pp_off + n] * e->qprim_max >= tau) ++n;
+      p.pp_n = n;
+    }
+  dev_free(e->d_pairs_s);
+  PQ_HIP(dev_alloc(&e->d_pairs_s, e->pairs_s.size()));
+  PQ_HIP(hipMemcpy(e->d_pairs_s, e->pairs_s.data(), e->pairs_s.size() * sizeof(PairInfo), hipMemcpyHostToDevice));
+  int rc = sort_pairs(e, e->pairs_s, e->sorted_s, e->d_sorted_s);
+  if (rc) return rc;
+  // the ket J by ket position: class Y holds ncart(l1) * ncart(l2) components
+  // for each of its pairs
+  int64_t off = 0;
+  for (int y = 0; y < 6; ++y) {
+    e->jp_off[y] = (int)off;
+    e->jp_n[y] = (int)e->sorted_s[y].size();
+    off += (int64_t)e->jp_n[y] * ncart(kPairL[y][0]) * ncart(kPairL[y][1]);
+  }
+  if (off > INT32_MAX / 2) return fail(PQ_E_ARG, "too many shell pairs for the ket J array");
+  dev_free(e->d_jpair);
+  e->jp_size = off;
+  PQ_HIP(dev_alloc(&e->d_jpair, (size_t)std::max<int64_t>(2 * off, 1)));
+  return PQ_OK;
+}
 
 int build_pair_data(pq_engine* e) {
   e->pairs.clear();
@@ -288,45 +417,56 @@ int build_pair_data(pq_engine* e) {
       PQ_HIP(hipMemcpy(e->d_members[c], e->members[c].data(), e->members[c].size() * sizeof(int),
                        hipMemcpyHostToDevice));
   }
-  // Schwarz bounds from the diagonal quartets (ab|ab)
-  double* d_q = nullptr;
-  PQ_HIP(dev_alloc(&d_q, e->pairs.size()));
-  PQ_HIP(launch_fill(d_q, (int64_t)e->pairs.size(), 0.0, 0));
-  for (int c = 0; c < 6; ++c) {
-    const int np = (int)e->members[c].size();
-    if (!np) continue;
-    std::vector<Tile> tiles;
-    for (int k = 0; k < np; k += 64) tiles.push_back(Tile{-1, k, std::min(64, np - k), 0});
-    Tile* d_tiles = nullptr;
-    PQ_HIP(dev_alloc(&d_tiles, tiles.size()));
-    PQ_HIP(hipMemcpy(d_tiles, tiles.data(), tiles.size() * sizeof(Tile), hipMemcpyHostToDevice));
-    hipError_t err = launch_eri_class(kPairL[c][0], kPairL[c][1], kPairL[c][0], kPairL[c][1], MODE_DIAG, 0,
-                                      e->view(), d_tiles, (int)tiles.size(), e->d_members[c], nullptr,
-                                      nullptr, nullptr, nullptr, d_q, 0);
-    PQ_HIP(err);
-    PQ_HIP(hipDeviceSynchronize());
-    dev_free(d_tiles);
+  // Schwarz bounds from the diagonal quartets (ab|ab), of the contracted
+  // pairs and of every primitive pair
+  {
+    int rc = diag_factors(e, e->d_pairs, e->pairs.size(), e->members, e->Q);
+    if (rc) return rc;
+    std::vector<PairInfo> plist;
+    std::vector<std::vector<int>> pids(6);
+    plist.reserve(e->prims.size());
+    for (size_t i = 0; i < e->pairs.size(); ++i)
+      for (int k = 0; k < e->pairs[i].pp_n; ++k) {
+        PairInfo pk = e->pairs[i];
+        pk.pp_off += k;
+        pk.pp_n = 1;
+        pids[e->pcls[i]].push_back((int)plist.size());
+        plist.push_back(pk);
+      }
+    PairInfo* d_plist = nullptr;
+    PQ_HIP(dev_alloc(&d_plist, plist.size()));
+    PQ_HIP(hipMemcpy(d_plist, plist.data(), plist.size() * sizeof(PairInfo), hipMemcpyHostToDevice));
+    rc = diag_factors(e, d_plist, plist.size(), pids, e->qprim);
+    dev_free(d_plist);
+    if (rc) return rc;
   }
-  e->Q.assign(e->pairs.size(), 0.0);
-  PQ_HIP(hipMemcpy(e->Q.data(), d_q, e->pairs.size() * sizeof(double), hipMemcpyDeviceToHost));
-  dev_free(d_q);
-  // per class, sorted by Q descending (ties by pair id)
-  e->sorted.assign(6, {});
-  for (int c = 0; c < 6; ++c) {
-    auto v = e->members[c];
-    // molecule first (a bra's kets are its own molecule's), then contraction
-    // length (uniform primitive loops within a tile), then Q
-    std::stable_sort(v.begin(), v.end(), [&](int a, int b) {
-      const int ma = e->sh_mol[e->pairs[a].shA], mb = e->sh_mol[e->pairs[b].shA];
-      if (ma != mb) return ma < mb;
-      if (e->pairs[a].pp_n != e->pairs[b].pp_n) return e->pairs[a].pp_n > e->pairs[b].pp_n;
-      return e->Q[a] > e->Q[b];
-    });
-    e->sorted[c] = v;
-    dev_free(e->d_sorted[c]);
-    PQ_HIP(dev_alloc(&e->d_sorted[c], v.size()));
-    if (!v.empty())
-      PQ_HIP(hipMemcpy(e->d_sorted[c], v.data(), v.size() * sizeof(int), hipMemcpyHostToDevice));
+  // each pair's primitives in decreasing order of their Schwarz factor, so
+  // that the ones a cut keeps are a prefix
+  e->qprim_max = 0;
+  for (const auto& p : e->pairs) {
+    std::vector<int> perm(p.pp_n);
+    std::iota(perm.begin(), perm.end(), 0);
+    std::stable_sort(perm.begin(), perm.end(),
+                     [&](int a, int b) { return e->qprim[p.pp_off + a] > e->qprim[p.pp_off + b]; });
+    std::vector<PrimPair> pp(p.pp_n);
+    std::vector<double> pa(p.pp_n), pq(p.pp_n);
+    for (int k = 0; k < p.pp_n; ++k) {
+      pp[k] = e->prims[p.pp_off + perm[k]];
+      pa[k] = e->prim_a[p.pp_off + perm[k]];
+      pq[k] = e->qprim[p.pp_off + perm[k]];
+    }
+    for (int k = 0; k < p.pp_n; ++k) {
+      e->prims[p.pp_off + k] = pp[k];
+      e->prim_a[p.pp_off + k] = pa[k];
+      e->qprim[p.pp_off + k] = pq[k];
+    }
+    if (p.pp_n) e->qprim_max = std::max(e->qprim_max, pq[0]);
+  }
+  PQ_HIP(hipMemcpy(e->d_prim_a, e->prim_a.data(), e->prim_a.size() * sizeof(double), hipMemcpyHostToDevice));
+  PQ_HIP(hipMemcpy(e->d_prims, e->prims.data(), e->prims.size() * sizeof(PrimPair), hipMemcpyHostToDevice));
+  {
+    int rc = sort_pairs(e, e->pairs, e->sorted, e->d_sorted);
+    if (rc) return rc;
   }
   e->pairs_ok = true;
   return PQ_OK;
@@ -361,7 +501,17 @@ int set_groups(ClassWork& cw, int prec, int cap) {
   return PQ_OK;
 }
 
-int build_work(pq_engine* e, std::vector<ClassWork>& out, double cut, double fp32cut, bool split_prims) {
+// Rotates the tiles [first, end) of bra number i by a bra-dependent offset.
+void rotate_tiles(std::vector<Tile>& t, size_t first, size_t i) {
+  const size_t cnt = t.size() - first;
+  if (cnt < 2) return;
+  const size_t r = (i * 2654435761u) % cnt;
+  std::rotate(t.begin() + first, t.begin() + first + r, t.end());
+}
+
+// Work lists over the pair copy P (its pp_n) and its per-class order S.
+int build_work(pq_engine* e, std::vector<ClassWork>& out, double cut, double fp32cut, bool split_prims,
+               const std::vector<PairInfo>& P, const std::vector<std::vector<int>>& S) {
   for (auto& c : out) {
     dev_free(c.d_t64);
     dev_free(c.d_t32);
@@ -374,8 +524,8 @@ int build_work(pq_engine* e, std::vector<ClassWork>& out, double cut, double fp3
       ClassWork cw;
       cw.X = X;
       cw.Y = Y;
-      const auto& bras = e->sorted[X];
-      const auto& kets = e->sorted[Y];
+      const auto& bras = S[X];
+      const auto& kets = S[Y];
       if (bras.empty() || kets.empty()) {
         out.push_back(cw);
         continue;
@@ -385,20 +535,20 @@ int build_work(pq_engine* e, std::vector<ClassWork>& out, double cut, double fp3
       std::vector<double> kq(kets.size());
       std::vector<double> kpre(kets.size() + 1, 0.0), fpre(kets.size() + 1, 0.0);
       auto ufun = [&](int pid, int l1, int l2) -> double {  // unique function pairs of a shell pair
-        const PairInfo& p = e->pairs[pid];
+        const PairInfo& p = P[pid];
         return p.shA == p.shB ? ncart(l1) * (ncart(l1) + 1) / 2.0 : (double)ncart(l1) * ncart(l2);
       };
       for (size_t j = 0; j < kets.size(); ++j) {
         kq[j] = e->Q[kets[j]];
-        kpre[j + 1] = kpre[j] + e->pairs[kets[j]].pp_n;
+        kpre[j + 1] = kpre[j] + P[kets[j]].pp_n;
         fpre[j + 1] = fpre[j] + ufun(kets[j], lc, ld);
       }
       // ket groups: runs of one molecule and equal contraction length, each
       // sorted by Q descending; gmol[m] = first group of molecule m
-      auto kmol = [&](int j) { return e->sh_mol[e->pairs[kets[j]].shA]; };
+      auto kmol = [&](int j) { return e->sh_mol[P[kets[j]].shA]; };
       std::vector<int> gs;
       for (size_t j = 0; j < kets.size(); ++j)
-        if (j == 0 || e->pairs[kets[j]].pp_n != e->pairs[kets[j - 1]].pp_n || kmol(j) != kmol(j - 1))
+        if (j == 0 || P[kets[j]].pp_n != P[kets[j - 1]].pp_n || kmol(j) != kmol(j - 1))
           gs.push_back((int)j);
       gs.push_back((int)kets.size());
       std::vector<int> gmol(e->nmol + 1, (int)gs.size() - 1);
@@ -412,9 +562,10 @@ int build_work(pq_engine* e, std::vector<ClassWork>& out, double cut, double fp3
       };
       for (size_t i = 0; i < bras.size(); ++i) {
         const double qi = e->Q[bras[i]];
-        const int bpn = e->pairs[bras[i]].pp_n;
+        const int bpn = P[bras[i]].pp_n;
         const double ua = ufun(bras[i], la, lb);
-        const int bm = e->sh_mol[e->pairs[bras[i]].shA];
+        const int bm = e->sh_mol[P[bras[i]].shA];
+        const size_t t64_0 = cw.t64.size(), t32_0 = cw.t32.size();
         for (int g = gmol[bm]; g < gmol[bm + 1]; ++g) {
           const int g0 = gs[g], g1 = gs[g + 1];
           if (X == Y && g0 > (int)i) break;
@@ -427,7 +578,7 @@ int build_work(pq_engine* e, std::vector<ClassWork>& out, double cut, double fp3
           int e64 = end;
           if (fp32cut > 0.0) e64 = qi <= 0.0 ? g0 : std::min(end, g0 + count_ge(g0, g1, fp32cut / qi));
           // split the bra primitive sum when the quartets are heavily contracted
-          const int kpn = e->pairs[kets[g0]].pp_n;
+          const int kpn = P[kets[g0]].pp_n;
           int chunk = bpn;
           if (split_prims) chunk = std::max(1, std::min(bpn, kPrimBudget / std::max(1, kpn)));
           // a tile with fewer than 64 kets spreads each ket's primitive
@@ -462,6 +613,11 @@ int build_work(pq_engine* e, std::vector<ClassWork>& out, double cut, double fp3
           cw.nf64 += f64;
           cw.nf32 += f32;
         }
+        // start each bra's tiles at a different ket tile: blocks that run
+        // at the same time then add into different J[c,d] entries instead of
+        // all contending for the atomics of the first kets
+        rotate_tiles(cw.t64, t64_0, i);
+        rotate_tiles(cw.t32, t32_0, i);
       }
       for (int prec = 0; prec < 2 && split_prims; ++prec) {
         const int tuned = e->group_cap[X][Y][prec];
@@ -794,11 +950,20 @@ int pq_build_pairs(pq_engine* eng, double schwarz_cut, double fp32_cut) {
     int rc = build_pair_data(eng);
     if (rc) return rc;
   }
-  int rc = build_work(eng, eng->work, schwarz_cut, fp32_cut, true);
+  int rc = build_screened(eng, schwarz_cut);
+  if (rc) return rc;
+  rc = build_work(eng, eng->work, schwarz_cut, fp32_cut, true, eng->pairs_s, eng->sorted_s);
   if (rc) return rc;
   eng->cut = schwarz_cut;
   eng->fp32cut = fp32_cut;
   eng->work_ok = true;
+  return PQ_OK;
+}
+
+int pq_engine_set_prim_screen(pq_engine* eng, double rel) {
+  if (!eng) return fail(PQ_E_ARG, "null engine");
+  if (!(rel >= 0.0) || rel > 1.0) return fail(PQ_E_ARG, "rel must be in [0, 1]");
+  eng->prim_rel = rel;
   return PQ_OK;
 }
 
@@ -875,8 +1040,8 @@ int pq_engine_autotune(pq_engine* eng, const double* D_dev, int reps, void* stre
         for (int r = 0; r <= reps && err == hipSuccess; ++r) {
           if (r == 1) err = hipEventRecord(e0, st);
           if (err == hipSuccess)
-            err = launch_eri_class(la, lb, lc, ld, MODE_JK, prec, eng->view(), prec ? c.d_t32 : c.d_t64,
-                                   (int)tl.size(), eng->d_sorted[c.Y], D_dev, eng->d_Jacc, eng->d_Kacc, nullptr,
+            err = launch_eri_class(la, lb, lc, ld, MODE_JK, prec, eng->view_s(), prec ? c.d_t32 : c.d_t64,
+                                   (int)tl.size(), eng->d_sorted_s[c.Y], D_dev, eng->d_Jacc, eng->d_Kacc, nullptr,
                                    nullptr, st, prec ? c.d_g32 : c.d_g64, (int)(prec ? c.g32 : c.g64).size(),
                                    c.gflags(prec));
         }
@@ -926,7 +1091,7 @@ int pq_get2ints(pq_engine* eng, double* out_dev, uint64_t len, void* stream) {
     if (rc) return rc;
   }
   if (!eng->store_ok) {
-    int rc = build_work(eng, eng->store_work, 0.0, 0.0, false);
+    int rc = build_work(eng, eng->store_work, 0.0, 0.0, false, eng->pairs, eng->sorted);
     if (rc) return rc;
     eng->store_ok = true;
   }
@@ -1081,6 +1246,7 @@ static int fock_jk_impl(pq_engine* eng, int nd, const double* const* Ds, double*
     }
     PQ_TRY(launch_fill(eng->d_Jacc, nd * nn, 0.0, st));
     PQ_TRY(launch_fill(eng->d_Kacc, nd * nn, 0.0, st));
+    if (eng->jp_size > 0) PQ_TRY(launch_fill(eng->d_jpair, nd * eng->jp_size, 0.0, st));
     const int nside = eng->nside;
     if (nside > 0) {
       // fork: class kernels spread over the side streams, joined before the
@@ -1108,9 +1274,9 @@ static int fock_jk_impl(pq_engine* eng, int nd, const double* const* Ds, double*
         hipStream_t ks = nside > 0 ? eng->side[next] : st;
         if (nside > 0) next = (next + 1) % nside;
         if (ms) PQ_TRY(mark(ks));
-        PQ_TRY(launch_eri_class(la, lb, lc, ld, nd == 2 ? MODE_JK2 : MODE_JK, prec, eng->view(),
+        PQ_TRY(launch_eri_class(la, lb, lc, ld, nd == 2 ? MODE_JK2 : MODE_JK, prec, eng->view_s(),
                                 prec ? c.d_t32 : c.d_t64,
-                                (int)tl.size(), eng->d_sorted[c.Y], D_dev, eng->d_Jacc, eng->d_Kacc, nullptr,
+                                (int)tl.size(), eng->d_sorted_s[c.Y], D_dev, eng->d_Jacc, eng->d_Kacc, nullptr,
                                 nullptr, ks, prec ? c.d_g32 : c.d_g64, (int)(prec ? c.g32 : c.g64).size(),
                                 c.gflags(prec)));
         if (ms) {
@@ -1124,6 +1290,11 @@ static int fock_jk_impl(pq_engine* eng, int nd, const double* const* Ds, double*
       err = hipEventRecord(eng->join_ev[i], eng->side[i]);
       if (err == hipSuccess) err = hipStreamWaitEvent(st, eng->join_ev[i], 0);
     }
+    PQ_TRY(err);
+    // the ket-side J, accumulated by ket position, into Jacc
+    for (int y = 0; y < 6 && err == hipSuccess; ++y)
+      err = launch_jpair_scatter(eng->view_s(), y, ncart(kPairL[y][0]), ncart(kPairL[y][1]), eng->d_sorted_s[y], nd,
+                                 eng->d_Jacc, st);
     PQ_TRY(err);
     for (int d = 0; d < nd && rc == PQ_OK; ++d)
       PQ_TRY(eng->nmol == 1
@@ -1239,10 +1410,10 @@ int pq_forces(pq_engine* eng, const double* D_dev, const double* W_dev, double* 
   for (const auto& c : eng->work) {
     const int la = kPairL[c.X][0], lb = kPairL[c.X][1], lc = kPairL[c.Y][0], ld = kPairL[c.Y][1];
     if (!c.t64.empty())
-      PQ_HIP(launch_grad_class(la, lb, lc, ld, eng->view(), c.d_t64, (int)c.t64.size(), eng->d_sorted[c.Y], D_dev,
+      PQ_HIP(launch_grad_class(la, lb, lc, ld, eng->view_s(), c.d_t64, (int)c.t64.size(), eng->d_sorted_s[c.Y], D_dev,
                                eng->d_gacc, eng->natom, st));
     if (!c.t32.empty())
-      PQ_HIP(launch_grad_class(la, lb, lc, ld, eng->view(), c.d_t32, (int)c.t32.size(), eng->d_sorted[c.Y], D_dev,
+      PQ_HIP(launch_grad_class(la, lb, lc, ld, eng->view_s(), c.d_t32, (int)c.t32.size(), eng->d_sorted_s[c.Y], D_dev,
                                eng->d_gacc, eng->natom, st));
   }
   for (int c = 0; c < 6; ++c) {

@@ -170,12 +170,14 @@ __device__ __forceinline__ void eri_block(const EngineView& ev, const PairInfo& 
 // splits of one tile), flushed once per block.  drow, when non-null, is the
 // block's LDS copy of the bra rows of D (same layout as krow): the D elements
 // of the bra rows then come from LDS and only D[c,d] is gathered from memory.
+// jrow, when non-null, is the block's LDS accumulator of the bra J block
+// (NA x NB), flushed once per block instead of one global atomic per tile.
 template <int LA, int LB, int LC, int LD, int A0, int NAS, int B0, int NBS, typename T>
 __device__ __forceinline__ void digest_lane(const EngineView& ev, const PairInfo& bp, const PairInfo& kp,
                                             const T (&out)[NAS * NBS * ncart(LC) * ncart(LD)], double f,
                                             bool active, const double* __restrict__ D, double* Jacc,
                                             double* Kacc, double* krow, double* jsum,
-                                            const double* drow = nullptr) {
+                                            const double* drow, double* jrow, double* jpair, int jn, int kpos) {
   using C = Cls<LA, LB, LC, LD>;
   // the quartet's molecule block (bra and ket share it)
   const int n = bp.mstride;
@@ -185,10 +187,15 @@ __device__ __forceinline__ void digest_lane(const EngineView& ev, const PairInfo
   const int a0 = bp.bfA + A0, b0 = bp.bfB + B0, c0 = kp.bfA, d0 = kp.bfB;
   double jab[NAS][NBS], jcd[C::NC][C::ND], kac[NAS][C::NC], kad[NAS][C::ND], kbc[NBS][C::NC], kbd[NBS][C::ND];
   double Dab[NAS][NBS], Dcd[C::NC][C::ND], Dac[NAS][C::NC], Dad[NAS][C::ND], Dbc[NBS][C::NC], Dbd[NBS][C::ND];
+  // explicit address spaces keep the two paths apart (a merged load through
+  // a generic pointer would be a FLAT load on both)
+  using lds_cd = const __attribute__((address_space(3))) double;
+  using glb_cd = const __attribute__((address_space(1))) double;
+  glb_cd* Dg = (glb_cd*)D;
   if (drow) {
 #pragma unroll
     for (int a = 0; a < NAS; ++a) {
-      const double* ra = drow + (A0 + a) * n;
+      lds_cd* ra = (lds_cd*)drow + (A0 + a) * n;
 #pragma unroll
       for (int b = 0; b < NBS; ++b) { Dab[a][b] = ra[b0 + b]; jab[a][b] = 0; }
 #pragma unroll
@@ -198,7 +205,7 @@ __device__ __forceinline__ void digest_lane(const EngineView& ev, const PairInfo
     }
 #pragma unroll
     for (int b = 0; b < NBS; ++b) {
-      const double* rb = drow + (C::NA + B0 + b) * n;
+      lds_cd* rb = (lds_cd*)drow + (C::NA + B0 + b) * n;
 #pragma unroll
       for (int c = 0; c < C::NC; ++c) { Dbc[b][c] = rb[c0 + c]; kbc[b][c] = 0; }
 #pragma unroll
@@ -208,24 +215,24 @@ __device__ __forceinline__ void digest_lane(const EngineView& ev, const PairInfo
 #pragma unroll
   for (int a = 0; a < NAS; ++a) {
 #pragma unroll
-    for (int b = 0; b < NBS; ++b) { Dab[a][b] = D[(a0 + a) * n + b0 + b]; jab[a][b] = 0; }
+    for (int b = 0; b < NBS; ++b) { Dab[a][b] = Dg[(a0 + a) * n + b0 + b]; jab[a][b] = 0; }
 #pragma unroll
-    for (int c = 0; c < C::NC; ++c) { Dac[a][c] = D[(a0 + a) * n + c0 + c]; kac[a][c] = 0; }
+    for (int c = 0; c < C::NC; ++c) { Dac[a][c] = Dg[(a0 + a) * n + c0 + c]; kac[a][c] = 0; }
 #pragma unroll
-    for (int d = 0; d < C::ND; ++d) { Dad[a][d] = D[(a0 + a) * n + d0 + d]; kad[a][d] = 0; }
+    for (int d = 0; d < C::ND; ++d) { Dad[a][d] = Dg[(a0 + a) * n + d0 + d]; kad[a][d] = 0; }
   }
 #pragma unroll
   for (int b = 0; b < NBS; ++b) {
 #pragma unroll
-    for (int c = 0; c < C::NC; ++c) { Dbc[b][c] = D[(b0 + b) * n + c0 + c]; kbc[b][c] = 0; }
+    for (int c = 0; c < C::NC; ++c) { Dbc[b][c] = Dg[(b0 + b) * n + c0 + c]; kbc[b][c] = 0; }
 #pragma unroll
-    for (int d = 0; d < C::ND; ++d) { Dbd[b][d] = D[(b0 + b) * n + d0 + d]; kbd[b][d] = 0; }
+    for (int d = 0; d < C::ND; ++d) { Dbd[b][d] = Dg[(b0 + b) * n + d0 + d]; kbd[b][d] = 0; }
   }
   }
 #pragma unroll
   for (int c = 0; c < C::NC; ++c)
 #pragma unroll
-    for (int d = 0; d < C::ND; ++d) { Dcd[c][d] = D[(c0 + c) * n + d0 + d]; jcd[c][d] = 0; }
+    for (int d = 0; d < C::ND; ++d) { Dcd[c][d] = Dg[(c0 + c) * n + d0 + d]; jcd[c][d] = 0; }
 
 #pragma unroll
   for (int a = 0; a < NAS; ++a)
@@ -251,7 +258,10 @@ __device__ __forceinline__ void digest_lane(const EngineView& ev, const PairInfo
 #pragma unroll
     for (int b = 0; b < NBS; ++b) {
       const double s = wave_sum(jab[a][b]);
-      if (lane == 0) unsafeAtomicAdd(&Jacc[(a0 + a) * n + b0 + b], 2.0 * s);
+      if (lane == 0) {
+        if (jrow) atomicAdd(&jrow[(A0 + a) * C::NB + B0 + b], 2.0 * s);
+        else unsafeAtomicAdd(&Jacc[(a0 + a) * n + b0 + b], 2.0 * s);
+      }
     }
   if (!active) return;
   if (jsum) {
@@ -260,10 +270,12 @@ __device__ __forceinline__ void digest_lane(const EngineView& ev, const PairInfo
 #pragma unroll
       for (int d = 0; d < C::ND; ++d) atomicAdd(&jsum[lane * C::NC * C::ND + c * C::ND + d], 2.0 * jcd[c][d]);
   } else {
+    // ket-side J by ket position: the lanes of a tile hit consecutive
+    // addresses (coalesced atomics), scattered into J after the build
 #pragma unroll
     for (int c = 0; c < C::NC; ++c)
 #pragma unroll
-      for (int d = 0; d < C::ND; ++d) unsafeAtomicAdd(&Jacc[(c0 + c) * n + d0 + d], 2.0 * jcd[c][d]);
+      for (int d = 0; d < C::ND; ++d) unsafeAtomicAdd(&jpair[(c * C::ND + d) * jn + kpos], 2.0 * jcd[c][d]);
   }
   if (krow) {
 #pragma unroll
@@ -332,7 +344,7 @@ __device__ __forceinline__ void lane_body(const EngineView& ev, const Tile& tl, 
                                           const double* __restrict__ D, double* Jacc, double* Kacc,
                                           double* ints, double* qout, double* krow, double* jsum,
                                           const void* rtab = nullptr, int kc = 0, int ks = 1, int nk = 64,
-                                          const double* drow = nullptr) {
+                                          const double* drow = nullptr, double* jrow = nullptr, int kpos = 0) {
   using C = Cls<LA, LB, LC, LD>;
   constexpr bool UB = MODE != MODE_DIAG;  // Schwarz diagonal: bra = ket per lane
   const PairInfo bp = UB ? ld_uniform(ev.pairs + bra) : ev.pairs[bra];
@@ -365,12 +377,14 @@ __device__ __forceinline__ void lane_body(const EngineView& ev, const Tile& tl, 
     // one block of integrals, digested once per density (stacked n x n
     // matrices; LDS K rows and J sums stacked the same way)
     const size_t nn = (size_t)ev.msize;
+    constexpr int Y = pair_class_of(LC, LD);
 #pragma unroll
     for (int d = 0; d < mode_ndens(MODE); ++d)
       digest_lane<LA, LB, LC, LD, A0, NAS, B0, NBS, T>(
           ev, bp, kp, out, f, active, D + d * nn, Jacc + d * nn, Kacc + d * nn,
           krow ? krow + d * (C::NA + C::NB) * ev.nbf : nullptr, jsum ? jsum + d * 64 * C::NC * C::ND : nullptr,
-          drow ? drow + d * (C::NA + C::NB) * ev.nbf : nullptr);
+          drow ? drow + d * (C::NA + C::NB) * ev.nbf : nullptr, jrow ? jrow + d * C::NA * C::NB : nullptr,
+          ev.jpair + d * ev.jp_size + ev.jp_off[Y], ev.jp_n[Y], kpos);
   } else if constexpr (MODE == MODE_STORE) {
     if (!active) return;
     const bool same = bra == ket;
@@ -481,6 +495,9 @@ __global__ __launch_bounds__(256) void k_eri_group(EngineView ev, const Tile* __
   const PairInfo bp = ld_uniform(ev.pairs + tiles[g.x].bra);
   const int ms = bp.mstride;
   const int rows = (C::NA + C::NB) * ms, region = (C::NA + C::NB) * n;
+  // J bra block accumulator after the K rows (and the staged D rows)
+  double* jrow = krow + ND * region * (stage_d ? 2 : 1);
+  for (int i = threadIdx.x; i < ND * C::NA * C::NB; i += blockDim.x) jrow[i] = 0.0;
   double* drow = nullptr;
   if (stage_d) {
     drow = krow + ND * region;
@@ -502,9 +519,16 @@ __global__ __launch_bounds__(256) void k_eri_group(EngineView ev, const Tile* __
     const int ket = kets[tl.ket0 + tn.slot];
     lane_body<LA, LB, LC, LD, T, MODE, 0, C::NA, 0, C::NB, (NT > 0)>(ev, tl, tn.active, tl.bra, ket, D, Jacc,
                                                                        Kacc, nullptr, nullptr, krow, nullptr, rtab,
-                                                                       tn.kc, tn.ks, tn.nk, drow);
+                                                                       tn.kc, tn.ks, tn.nk, drow, jrow,
+                                                                       tl.ket0 + tn.slot);
   }
   __syncthreads();
+  for (int i = threadIdx.x; i < ND * C::NA * C::NB; i += blockDim.x) {
+    const int d = i / (C::NA * C::NB), ab = i - d * (C::NA * C::NB);
+    const double v = jrow[i];
+    if (v != 0.0)
+      unsafeAtomicAdd(&Jacc[(size_t)d * ev.msize + bp.moff + (bp.bfA + ab / C::NB) * ms + bp.bfB + ab % C::NB], v);
+  }
   for (int i = threadIdx.x; i < ND * rows; i += blockDim.x) {
     const int d = i / rows, ir = i - d * rows;
     const double v = krow[d * region + ir];
@@ -550,20 +574,23 @@ __global__ __launch_bounds__(256) void k_eri_lane(EngineView ev, const Tile* __r
   const int bra = (MODE == MODE_DIAG) ? ket : tl.bra;
   if constexpr (C::SPLIT == 0) {
     lane_body<LA, LB, LC, LD, T, MODE, 0, C::NA, 0, C::NB>(ev, tl, active, bra, ket, D, Jacc, Kacc, ints, qout,
-                                                          krow, jsum, nullptr, tn.kc, tn.ks, tn.nk);
+                                                          krow, jsum, nullptr, tn.kc, tn.ks, tn.nk, nullptr,
+                                                          nullptr, tl.ket0 + tn.slot);
   } else {
     const int split = lds_mode ? (int)blockIdx.y * kWaveSplit + wv : (int)blockIdx.y;
     if constexpr (C::SPLIT == 1) {
       static_switch<0, C::NA>(split, [&](auto ia) {
         lane_body<LA, LB, LC, LD, T, MODE, decltype(ia)::value, 1, 0, C::NB>(
-            ev, tl, active, bra, ket, D, Jacc, Kacc, ints, qout, krow, jsum, nullptr, tn.kc, tn.ks, tn.nk);
+            ev, tl, active, bra, ket, D, Jacc, Kacc, ints, qout, krow, jsum, nullptr, tn.kc, tn.ks, tn.nk, nullptr,
+            nullptr, tl.ket0 + tn.slot);
       });
     } else {
       static_switch<0, C::NA * C::NB>(split, [&](auto iab) {
         constexpr int v = decltype(iab)::value;
         lane_body<LA, LB, LC, LD, T, MODE, v / C::NB, 1, v % C::NB, 1>(ev, tl, active, bra, ket, D, Jacc, Kacc,
                                                                         ints, qout, krow, jsum, nullptr, tn.kc,
-                                                                        tn.ks, tn.nk);
+                                                                        tn.ks, tn.nk, nullptr, nullptr,
+                                                                        tl.ket0 + tn.slot);
       });
     }
   }
@@ -584,17 +611,17 @@ __global__ __launch_bounds__(256) void k_eri_lane(EngineView ev, const Tile* __r
   }
   if constexpr (C::SPLIT != 0) {
     constexpr int NCD = C::NC * C::ND;
+    constexpr int Y = pair_class_of(LC, LD);
+    // component-major over the lanes: consecutive threads add into
+    // consecutive ket positions of the position-indexed ket J
     for (int d = 0; d < ND; ++d)
       for (int i = threadIdx.x; i < 64 * NCD; i += blockDim.x) {
-        const int l = i / NCD, cd = i - l * NCD;
+        const int cd = i >> 6, l = i & 63;
         const TileLane tl_l = tile_lane(tl.nket, l);
-        if (!tl_l.active) break;
-        const double v = jsum[d * 64 * NCD + i];
-        if (v != 0.0) {
-          const PairInfo kq = ev.pairs[kets[tl.ket0 + tl_l.slot]];
-          unsafeAtomicAdd(&Jacc[(size_t)d * ev.msize + bp.moff + (kq.bfA + cd / C::ND) * ms + kq.bfB + cd % C::ND],
-                          v);
-        }
+        if (!tl_l.active) continue;
+        const double v = jsum[d * 64 * NCD + l * NCD + cd];
+        if (v != 0.0)
+          unsafeAtomicAdd(&ev.jpair[d * ev.jp_size + ev.jp_off[Y] + cd * ev.jp_n[Y] + tl.ket0 + tl_l.slot], v);
       }
   }
 }
@@ -612,11 +639,13 @@ static hipError_t launch_one(const EngineView& ev, const Tile* tiles, int ntiles
                      sizeof(double);
   if constexpr (C::SPLIT == 0 && mode_is_jk(MODE)) {
     const size_t glds = lds + kLdsTableDoubles<C::NR, T> * sizeof(double);
-    // the staged D rows take as much LDS as the K rows
-    const int stage_d = (gflags & kGroupStageD) && glds + lds <= 65536;
-    if (groups && ngroups > 0 && glds <= 65536) {
+    // plus the bra J block; the staged D rows take as much LDS as the K rows
+    const size_t jlds = mode_ndens(MODE) * C::NA * C::NB * sizeof(double);
+    const int stage_d = (gflags & kGroupStageD) && glds + jlds + lds <= 65536;
+    if (groups && ngroups > 0 && glds + jlds <= 65536) {
       hipLaunchKernelGGL((k_eri_group<LA, LB, LC, LD, T, MODE>), dim3(ngroups), dim3(256),
-                         stage_d ? glds + lds : glds, st, ev, tiles, groups, ngroups, kets, D, Jacc, Kacc, stage_d);
+                         glds + jlds + (stage_d ? lds : 0), st, ev, tiles, groups, ngroups, kets, D, Jacc, Kacc,
+                         stage_d);
       return hipGetLastError();
     }
   }

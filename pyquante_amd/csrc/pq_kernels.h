@@ -29,6 +29,11 @@ hipError_t launch_eri_class(int la, int lb, int lc, int ld, int mode, int fp32, 
                             double* Kacc, double* ints, double* qout, hipStream_t st,
                             const int2* groups = nullptr, int ngroups = 0, int gflags = 0);
 
+// adds the position-indexed ket J of pair class Y (ev.jpair; nc x ndf
+// functions per pair, ket order pair_ids) into nd stacked Jacc matrices
+hipError_t launch_jpair_scatter(const EngineView& ev, int Y, int nc, int ndf, const int* pair_ids, int nd,
+                                double* Jacc, hipStream_t st);
+
 // two-electron nuclear gradient into kGradCopies private copies of the
 // natom x 3 gradient (gacc), summed by launch_grad_finish
 hipError_t launch_grad_class(int la, int lb, int lc, int ld, const EngineView& ev, const Tile* tiles, int ntiles,

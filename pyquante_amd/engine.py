@@ -123,6 +123,12 @@ class Engine(object):
         """Spread class kernels over n engine streams (0 = caller's stream only)."""
         check(lib.pq_engine_set_streams(self.handle, int(n)))
 
+    def set_prim_screen(self, rel):
+        """Primitive screening of the J/K and gradient work relative to the
+        Schwarz cut (pq_engine_set_prim_screen; 0 = every primitive); applies
+        from the next build()."""
+        check(lib.pq_engine_set_prim_screen(self.handle, float(rel)))
+
     def set_group_cap(self, cap):
         """Tiles per bra run for every class (0 = none, -1 = default); a run
         length may be or-ed with _abi.PQ_CAP_STAGE_D (bra rows of D in LDS)."""

@@ -229,6 +229,43 @@ def test_mixed_precision_close_to_fp64():
     assert torch.max(torch.abs(J64 - J32)).item() < 1e-6
 
 
+def test_primitive_screening_bound():
+    """Skipping primitive pairs (factor product below 1e-4 * cut) moves J/K
+    by far less than the shell-level cut itself and leaves the quartet count
+    alone; the unscreened build is pinned against the oracle elsewhere."""
+    from pyquante_amd.basis import BasisSet
+    from pyquante_amd.engine import Engine
+    from pyquante_amd.molecule import water_cluster
+    mol = water_cluster((2, 2, 1), seed=5)
+    eng = Engine(BasisSet(mol, "6-31g**"), mol)
+    n = eng.nbf
+    rng = np.random.default_rng(9)
+    C = rng.standard_normal((n, 20)) * 0.1
+    D = torch.as_tensor(C @ C.T, device=eng.device)
+    eng.build(0.0)
+    Je, Ke = eng.fock_jk(D)
+    eng.set_prim_screen(0.0)
+    eng.build(1e-12)
+    Ja, Ka = eng.fock_jk(D)
+    sa = eng.stats()
+    eng.set_prim_screen(1e-4)
+    eng.build(1e-12)
+    Jb, Kb = eng.fock_jk(D)
+    sb = eng.stats()
+    print("prim quartets %d -> %d" % (sa["prim_quartets_fp64"], sb["prim_quartets_fp64"]), flush=True)
+    assert sb["quartets_fp64"] == sa["quartets_fp64"]
+    assert sb["prim_quartets_fp64"] < 0.9 * sa["prim_quartets_fp64"]
+    dprim = max(torch.max(torch.abs(Ja - Jb)).item(), torch.max(torch.abs(Ka - Kb)).item())
+    dshell = max(torch.max(torch.abs(Ja - Je)).item(), torch.max(torch.abs(Ka - Ke)).item())
+    print("max |d| from primitive screening %.2e, from the shell cut %.2e" % (dprim, dshell), flush=True)
+    assert dprim < 1e-11
+    assert dshell < 1e-8
+    # every primitive again once switched off
+    eng.set_prim_screen(0.0)
+    eng.build(1e-12)
+    assert eng.stats()["prim_quartets_fp64"] == sa["prim_quartets_fp64"]
+
+
 def test_function_quartet_count_unscreened():
     from pyquante_amd.basis import BasisSet
     from pyquante_amd.engine import Engine
