@@ -115,7 +115,7 @@ def apply_tuning(eng, D, name, args, precision=0):
         return None
     caps = tuning.load(name) if args.tune == "table" else None
     if caps is None:
-        caps = eng.autotune(D)
+        caps = eng.autotune(D, reps=10)
         if args.write_tuning:
             tuning.store(name, caps, nbf=eng.nbf, precision="mixed" if precision else "fp64")
     else:
