@@ -331,7 +331,10 @@ int build_screened(pq_engine* e, double cut) {
   dev_free(e->d_pairs_s);
   PQ_HIP(dev_alloc(&e->d_pairs_s, e->pairs_s.size()));
   PQ_HIP(hipMemcpy(e->d_pairs_s, e->pairs_s.data(), e->pairs_s.size() * sizeof(PairInfo), hipMemcpyHostToDevice));
-  int rc = sort_pairs(e, e->pairs_s, e->sorted_s, e->d_sorted_s);
+  // ket groups keep the unscreened contraction lengths: grouping by the
+  // screened ones splits them into many short groups, and the extra partly
+  // filled tiles cost more than the skipped primitives save (measured)
+  int rc = sort_pairs(e, e->pairs, e->sorted_s, e->d_sorted_s);
   if (rc) return rc;
   // the ket J by ket position: class Y holds ncart(l1) * ncart(l2) components
   // for each of its pairs
@@ -543,12 +546,13 @@ int build_work(pq_engine* e, std::vector<ClassWork>& out, double cut, double fp3
         kpre[j + 1] = kpre[j] + P[kets[j]].pp_n;
         fpre[j + 1] = fpre[j] + ufun(kets[j], lc, ld);
       }
-      // ket groups: runs of one molecule and equal contraction length, each
-      // sorted by Q descending; gmol[m] = first group of molecule m
+      // ket groups: runs of one molecule and equal (unscreened) contraction
+      // length, each sorted by Q descending; gmol[m] = first group of
+      // molecule m
       auto kmol = [&](int j) { return e->sh_mol[P[kets[j]].shA]; };
       std::vector<int> gs;
       for (size_t j = 0; j < kets.size(); ++j)
-        if (j == 0 || P[kets[j]].pp_n != P[kets[j - 1]].pp_n || kmol(j) != kmol(j - 1))
+        if (j == 0 || e->pairs[kets[j]].pp_n != e->pairs[kets[j - 1]].pp_n || kmol(j) != kmol(j - 1))
           gs.push_back((int)j);
       gs.push_back((int)kets.size());
       std::vector<int> gmol(e->nmol + 1, (int)gs.size() - 1);
