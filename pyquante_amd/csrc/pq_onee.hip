@@ -182,8 +182,10 @@ hipError_t launch_onee(int la, int lb, const EngineView& ev, const int* pair_ids
 // AnalyticDerivatives.py:27-455): centre derivatives of S, T and V by the
 // shift rule d/dA_x g(l) = 2 alpha g(l+1) - l g(l-1) on the primitives; the
 // nucleus-C derivative of V from translational invariance of the
-// three-centre integral; d/dB of S and T = -d/dA.  One lane per shell pair,
-// contracted at once with D and W; results go to kGradCopies private copies.
+// three-centre integral; d/dB of S and T = -d/dA.  One lane per (shell pair,
+// nucleus of its molecule): the lane of the molecule's first nucleus also does
+// the S and T part; contracted at once with D and W; results go to
+// kGradCopies private copies.
 // ---------------------------------------------------------------------
 template <int LA, int LB>
 __global__ __launch_bounds__(128) void k_onee_grad(EngineView ev, const int* __restrict__ pair_ids, int npairs,
@@ -193,13 +195,15 @@ __global__ __launch_bounds__(128) void k_onee_grad(EngineView ev, const int* __r
                                                    double* gacc) {
   constexpr int NA = ncart(LA), NB = ncart(LB);
   constexpr int NRV = (LA + LB + 1) / 2 + 1;
-  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  const int tt = blockIdx.x * blockDim.x + threadIdx.x;
+  const int t = tt / ev.maxat, slot = tt - t * ev.maxat;
   if (t >= npairs) return;
   const PairInfo bp = ev.pairs[pair_ids[t]];
+  if (slot >= bp.mnat) return;
   const int n = bp.mstride;
   D += bp.moff;
   W += bp.moff;
-  double* gcopy = gacc + (size_t)(t % kGradCopies) * 3 * natom;
+  double* gcopy = gacc + (size_t)(tt % kGradCopies) * 3 * natom;
   const double A[3] = {bp.Ax, bp.Ay, bp.Az};
   const double B[3] = {bp.Ax - bp.ABx, bp.Ay - bp.ABy, bp.Az - bp.ABz};
   const double sym = bp.shA == bp.shB ? 1.0 : 2.0;
@@ -213,7 +217,7 @@ __global__ __launch_bounds__(128) void k_onee_grad(EngineView ev, const int* __r
       ww[a][b] = 2.0 * sym * nf * W[(bp.bfA + a) * n + bp.bfB + b];
     }
   double gA[3] = {0, 0, 0}, gB[3] = {0, 0, 0};
-  for (int ip = 0; ip < bp.pp_n; ++ip) {
+  for (int ip = 0; ip < (slot == 0 ? bp.pp_n : 0); ++ip) {
     const PrimPair pp = ev.prims[bp.pp_off + ip];
     const double p = pp.p, alp = ev.prim_a[bp.pp_off + ip], beta = p - alp;
     const double P[3] = {pp.Px, pp.Py, pp.Pz};
@@ -271,9 +275,17 @@ __global__ __launch_bounds__(128) void k_onee_grad(EngineView ev, const int* __r
           gB[x] -= g;
         }
       }
-    // nuclear attraction: derivatives on A, B and the nucleus (own molecule)
+  }
+  // nuclear attraction of this lane's nucleus: derivatives on A, B and C
+  const int c = bp.mat0 + slot;
+  double gC[3] = {0, 0, 0};
+  for (int ip = 0; ip < bp.pp_n; ++ip) {
+    const PrimPair pp = ev.prims[bp.pp_off + ip];
+    const double p = pp.p, alp = ev.prim_a[bp.pp_off + ip], beta = p - alp;
+    const double P[3] = {pp.Px, pp.Py, pp.Pz};
+    const double i2p = 0.5 / p;
     const double vpre = -2.0 * M_PI * pp.Kp;
-    for (int c = bp.mat0; c < bp.mat0 + bp.mnat; ++c) {
+    {
       const double Cc[3] = {atom_xyz[3 * c], atom_xyz[3 * c + 1], atom_xyz[3 * c + 2]};
       const double PC[3] = {P[0] - Cc[0], P[1] - Cc[1], P[2] - Cc[2]};
       const double X = p * (PC[0] * PC[0] + PC[1] * PC[1] + PC[2] * PC[2]);
@@ -331,12 +343,13 @@ __global__ __launch_bounds__(128) void k_onee_grad(EngineView ev, const int* __r
       for (int x = 0; x < 3; ++x) {
         gA[x] += vA[x];
         gB[x] += vB[x];
-        unsafeAtomicAdd(&gcopy[3 * c + x], -(vA[x] + vB[x]));
+        gC[x] -= vA[x] + vB[x];
       }
     }
   }
 #pragma unroll
   for (int x = 0; x < 3; ++x) {
+    unsafeAtomicAdd(&gcopy[3 * c + x], gC[x]);
     unsafeAtomicAdd(&gcopy[3 * bp.atA + x], gA[x]);
     unsafeAtomicAdd(&gcopy[3 * bp.atB + x], gB[x]);
   }
@@ -346,7 +359,10 @@ hipError_t launch_onee_grad(int la, int lb, const EngineView& ev, const int* pai
                             const double* atom_xyz, const double* atom_z, int natom, const double* D,
                             const double* W, double* gacc, hipStream_t st) {
   if (npairs <= 0) return hipSuccess;
-  dim3 g((npairs + 127) / 128), b(128);
+  if (ev.maxat <= 0) return hipErrorInvalidValue;
+  const int64_t lanes = (int64_t)npairs * ev.maxat;  // one lane per (pair, nucleus slot)
+  if (lanes > INT32_MAX - 128) return hipErrorInvalidValue;
+  dim3 g((unsigned)((lanes + 127) / 128)), b(128);
 #define PQ_OGCASE(A, B_)                                                                                    \
   if (la == A && lb == B_) {                                                                                \
     hipLaunchKernelGGL((k_onee_grad<A, B_>), g, b, 0, st, ev, pair_ids, npairs, atom_xyz, atom_z, natom, D, W, \
