@@ -137,7 +137,7 @@ int pq_build_pairs(pq_engine *eng, double schwarz_cut, double fp32_cut);
 int pq_engine_set_prim_screen(pq_engine *eng, double rel);
 
 /* Number of engine-owned streams (0..4, default 4) the class kernels of one
- * Fock build are spread over; 0 launches everything on the caller's stream
+ * Fock build or gradient are spread over; 0 launches everything on the caller's stream
  * (one kernel at a time, e.g. for per-kernel profiling).  No reference
  * counterpart (a launch knob). */
 int pq_engine_set_streams(pq_engine *eng, int nstreams);

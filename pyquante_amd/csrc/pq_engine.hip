@@ -1191,6 +1191,35 @@ int pq_engine_class_stats(const pq_engine* eng, int c, int l[4], uint64_t counts
 static int fock_jk_impl(pq_engine* eng, int nd, const double* const* Ds, double* const* Js, double* const* Ks,
                         int precision, void* stream, float* ms);
 
+// Makes the engine's side streams (default priority) wait for st; creates
+// them and their events on first use.  No-op with nside = 0.
+static hipError_t fork_side(pq_engine* eng, hipStream_t st) {
+  if (eng->nside <= 0) return hipSuccess;
+  hipError_t err = hipSuccess;
+  if (!eng->streams_ok) {
+    for (int i = 0; i < pq_engine::kSide && err == hipSuccess; ++i) {
+      err = hipStreamCreateWithFlags(&eng->side[i], hipStreamNonBlocking);
+      if (err == hipSuccess) err = hipEventCreateWithFlags(&eng->join_ev[i], hipEventDisableTiming);
+    }
+    if (err == hipSuccess) err = hipEventCreateWithFlags(&eng->fork_ev, hipEventDisableTiming);
+    if (err != hipSuccess) return err;
+    eng->streams_ok = true;
+  }
+  err = hipEventRecord(eng->fork_ev, st);
+  for (int i = 0; i < eng->nside && err == hipSuccess; ++i) err = hipStreamWaitEvent(eng->side[i], eng->fork_ev, 0);
+  return err;
+}
+
+// Makes st wait for everything enqueued on the side streams.
+static hipError_t join_side(pq_engine* eng, hipStream_t st) {
+  hipError_t err = hipSuccess;
+  for (int i = 0; i < eng->nside && err == hipSuccess; ++i) {
+    err = hipEventRecord(eng->join_ev[i], eng->side[i]);
+    if (err == hipSuccess) err = hipStreamWaitEvent(st, eng->join_ev[i], 0);
+  }
+  return err;
+}
+
 int pq_fock_jk(pq_engine* eng, const double* D_dev, double* J_dev, double* K_dev, int precision, void* stream) {
   return fock_jk_impl(eng, 1, &D_dev, &J_dev, &K_dev, precision, stream, nullptr);
 }
@@ -1251,23 +1280,10 @@ static int fock_jk_impl(pq_engine* eng, int nd, const double* const* Ds, double*
     PQ_TRY(launch_fill(eng->d_Jacc, nd * nn, 0.0, st));
     PQ_TRY(launch_fill(eng->d_Kacc, nd * nn, 0.0, st));
     if (eng->jp_size > 0) PQ_TRY(launch_fill(eng->d_jpair, nd * eng->jp_size, 0.0, st));
+    // fork: class kernels spread over the side streams, joined before the
+    // symmetrisation (all of them only add into Jacc/Kacc)
     const int nside = eng->nside;
-    if (nside > 0) {
-      // fork: class kernels spread over the side streams, joined before the
-      // symmetrisation (all of them only add into Jacc/Kacc)
-      if (!eng->streams_ok) {
-        for (int i = 0; i < pq_engine::kSide && err == hipSuccess; ++i) {
-          err = hipStreamCreateWithFlags(&eng->side[i], hipStreamNonBlocking);
-          if (err == hipSuccess) err = hipEventCreateWithFlags(&eng->join_ev[i], hipEventDisableTiming);
-        }
-        if (err == hipSuccess) err = hipEventCreateWithFlags(&eng->fork_ev, hipEventDisableTiming);
-        PQ_TRY(err);
-        eng->streams_ok = true;
-      }
-      PQ_TRY(hipEventRecord(eng->fork_ev, st));
-      for (int i = 0; i < nside && err == hipSuccess; ++i) err = hipStreamWaitEvent(eng->side[i], eng->fork_ev, 0);
-      PQ_TRY(err);
-    }
+    PQ_TRY(fork_side(eng, st));
     int next = 0;
     for (size_t ci = 0; ci < eng->work.size() && rc == PQ_OK; ++ci) {
       const auto& c = eng->work[ci];
@@ -1290,11 +1306,7 @@ static int fock_jk_impl(pq_engine* eng, int nd, const double* const* Ds, double*
       }
     }
     if (rc != PQ_OK) break;
-    for (int i = 0; i < nside && err == hipSuccess; ++i) {
-      err = hipEventRecord(eng->join_ev[i], eng->side[i]);
-      if (err == hipSuccess) err = hipStreamWaitEvent(st, eng->join_ev[i], 0);
-    }
-    PQ_TRY(err);
+    PQ_TRY(join_side(eng, st));
     // the ket-side J, accumulated by ket position, into Jacc
     for (int y = 0; y < 6 && err == hipSuccess; ++y)
       err = launch_jpair_scatter(eng->view_s(), y, ncart(kPairL[y][0]), ncart(kPairL[y][1]), eng->d_sorted_s[y], nd,
@@ -1411,20 +1423,30 @@ int pq_forces(pq_engine* eng, const double* D_dev, const double* W_dev, double* 
     eng->gacc_n = gsize;
   }
   PQ_HIP(launch_fill(eng->d_gacc, (int64_t)gsize, 0.0, st));
+  // class kernels round-robin over the side streams, as in the Fock build
+  // (they only add into the gradient copies)
+  PQ_HIP(fork_side(eng, st));
+  int next = 0;
+  auto ks = [&]() {
+    hipStream_t s = eng->nside > 0 ? eng->side[next] : st;
+    if (eng->nside > 0) next = (next + 1) % eng->nside;
+    return s;
+  };
   for (const auto& c : eng->work) {
     const int la = kPairL[c.X][0], lb = kPairL[c.X][1], lc = kPairL[c.Y][0], ld = kPairL[c.Y][1];
     if (!c.t64.empty())
       PQ_HIP(launch_grad_class(la, lb, lc, ld, eng->view_s(), c.d_t64, (int)c.t64.size(), eng->d_sorted_s[c.Y], D_dev,
-                               eng->d_gacc, eng->natom, st));
+                               eng->d_gacc, eng->natom, ks()));
     if (!c.t32.empty())
       PQ_HIP(launch_grad_class(la, lb, lc, ld, eng->view_s(), c.d_t32, (int)c.t32.size(), eng->d_sorted_s[c.Y], D_dev,
-                               eng->d_gacc, eng->natom, st));
+                               eng->d_gacc, eng->natom, ks()));
   }
   for (int c = 0; c < 6; ++c) {
     if (eng->members[c].empty()) continue;
     PQ_HIP(launch_onee_grad(kPairL[c][0], kPairL[c][1], eng->view(), eng->d_members[c], (int)eng->members[c].size(),
-                            eng->d_atom_xyz, eng->d_atom_z, eng->natom, D_dev, W_dev, eng->d_gacc, st));
+                            eng->d_atom_xyz, eng->d_atom_z, eng->natom, D_dev, W_dev, eng->d_gacc, ks()));
   }
+  PQ_HIP(join_side(eng, st));
   PQ_HIP(launch_grad_finish(eng->d_gacc, ncopy, eng->natom, eng->d_atom_xyz, eng->d_atom_z,
                             eng->nmol > 1 ? eng->d_atom_mol : nullptr, grad_dev, st));
   return PQ_OK;
