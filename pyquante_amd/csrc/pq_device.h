@@ -183,8 +183,9 @@ __device__ __forceinline__ RysPoint<NR> rys_prepare(double X, const double* tab 
   const double* base = LT ? tab : rys_coef + rys_off[NR - 1];
   p.asym = X >= rys_xasym[NR];
   if (p.asym) {
-    p.rx = 1.0 / X;
+    // 1/X as the square of 1/sqrt(X): a few ulp, without a full division
     p.rs = rsqrt(X);
+    p.rx = p.rs * p.rs;
     p.c = base;
     p.sv = 0.0;
   } else {
@@ -233,8 +234,8 @@ __device__ __forceinline__ RysPoint32<NR> rys_prepare32(double X, const float* t
   const float* base = LT ? tab : rys_coef32 + rys_off32[NR - 1];
   p.asym = X >= rys_xasym32[NR];
   if (p.asym) {
-    p.rx = 1.0 / X;
     p.rs = rsqrt(X);
+    p.rx = p.rs * p.rs;
     p.c = base;
     p.sv = 0.f;
   } else {
