@@ -14,8 +14,10 @@ primitive and contracted quartets) over its average launch duration, timed
 with HIP events on the stream the kernel is launched on, in the same
 concurrent configuration as the timed builds, against the MI355X FP64 vector
 peak.  `traffic` is the HBM bytes per launch of that kernel from rocprofv3
-FETCH_SIZE/WRITE_SIZE runs (profiles/traffic.json), when recorded.  The
-cpu_baseline block times the compiled reference crys.c (oracle/_ref) on a
+FETCH_SIZE/WRITE_SIZE runs (profiles/traffic.json), when recorded;
+`counted_fp64_tflops` is its FP64 VALU work counted by rocprofv3 (64 lanes x
+(2 FMA + ADD + MUL) per launch, profiles/valu.json) over the same launch
+duration.  The cpu_baseline block times the compiled reference crys.c (oracle/_ref) on a
 seeded sample of the same workload's unique function quartets,
 single-threaded.
 
@@ -85,15 +87,20 @@ def cpu_baseline(bfs, seconds=10.0, seed=0, method="crys"):
                       "%s compiled -O2, %.1f s, 1 thread" % (done, fn, t_used)}
 
 
-def recorded_traffic(key):
-    """HBM bytes per launch recorded for `key` (kernel and workload) in
-    profiles/traffic.json from rocprofv3 counter runs, or None."""
-    path = os.path.join(ROOT, "profiles", "traffic.json")
+def recorded(name, key, field):
+    """`field` of the entry recorded for `key` (kernel and workload) in
+    profiles/<name> from rocprofv3 counter runs, or None."""
+    path = os.path.join(ROOT, "profiles", name)
     if not os.path.exists(path):
         return None
     with open(path) as f:
         entry = json.load(f).get(key)
-    return None if entry is None else entry["bytes_per_launch"]
+    return None if entry is None else entry[field]
+
+
+def recorded_traffic(key):
+    """HBM bytes per launch (profiles/traffic.json), or None."""
+    return recorded("traffic.json", key, "bytes_per_launch")
 
 
 def kernel_name(cls, cap, nbf):
@@ -409,6 +416,9 @@ def main():
     k = int(np.argmax(acc[:42:2]))
     kname = kernel_name(cls[k]["class"], caps[k][0] if caps else None, eng.nbf)
     traffic = recorded_traffic("%s h2o8_631ss_fp64" % kname)
+    # FP64 VALU flops per launch counted by rocprofv3 (64 lanes x (2 FMA +
+    # ADD + MUL)), over the same event-timed launch duration
+    hw_flops = recorded("valu.json", "%s h2o8_631ss_fp64" % kname, "fp64_flops_per_launch")
     c = cls[k]
     ktime = acc[2 * k] * 1e-3
     achieved = c["flops_fp64"] / ktime / 1e12
@@ -454,7 +464,8 @@ def main():
                      "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved / FP64_PEAK_TFLOPS, "traffic": traffic,
                      "kernel": "%s, %s FP64 J/K" % (kname, lname), "launch_ms": ktime * 1e3,
-                     "all_kernels_tflops": total_flops / (acc[42] * 1e-3) / 1e12},
+                     "all_kernels_tflops": total_flops / (acc[42] * 1e-3) / 1e12,
+                     "counted_fp64_tflops": hw_flops / ktime / 1e12 if hw_flops else None},
         "cpu_baseline": cpu,
     }
     print(json.dumps(line))
