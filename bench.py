@@ -123,6 +123,9 @@ def apply_tuning(eng, D, name, args, precision=0):
     caps = tuning.load(name) if args.tune == "table" else None
     if caps is None:
         caps = eng.autotune(D, reps=10)
+        if not args.no_build_tune:
+            caps, ms = eng.autotune_build(D, caps, precision=precision)
+            print("bench: whole-build tuning of %s: %.3f ms/build" % (name, ms), file=sys.stderr, flush=True)
         if args.write_tuning:
             tuning.store(name, caps, nbf=eng.nbf, precision="mixed" if precision else "fp64")
     else:
@@ -334,6 +337,8 @@ def main():
                     help="bra-run caps: the stored autotuner table for the workload (falling back to a live "
                          "autotune when none is stored), a live autotune, or the defaults")
     ap.add_argument("--write-tuning", action="store_true", help="store live autotune results in the tuning table")
+    ap.add_argument("--no-build-tune", action="store_true",
+                    help="with --tune live: keep the per-class autotune result (no whole-build refinement)")
     ap.add_argument("--eager", action="store_true", help="time eager pq_fock_jk calls instead of graph replays")
     args = ap.parse_args()
     if args.gpus != 1:

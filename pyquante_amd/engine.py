@@ -153,6 +153,59 @@ class Engine(object):
         torch.cuda.synchronize(self.device)
         return [[caps[2 * c], caps[2 * c + 1]] for c in range(21)]
 
+    def autotune_build(self, D, caps=None, precision=_abi.PQ_PREC_FP64, reps=5, steps=10, min_gain=0.003):
+        """Whole-build refinement of a cap table (per-class autotune result if
+        caps is None): for each class without a component split, in order of
+        primitive work, tries every candidate cap while the other classes keep
+        theirs, timing graph-replayed builds (median of `reps` runs of `steps`
+        builds), and keeps a change when the build gets at least `min_gain`
+        faster.  The class kernels of a build run concurrently, so the best cap
+        of a class alone is not always the best inside the build.  Returns
+        (caps, ms per build)."""
+        import statistics
+        import time
+        if self.cut is None:
+            self.build()
+        prec = 1 if precision == _abi.PQ_PREC_MIXED else 0
+        caps = [list(c) for c in (caps if caps is not None else self.autotune(D, reps=10))]
+        S = _abi.PQ_CAP_STAGE_D
+        cands = [0, 4, 8, 16, 32, 4 | S, 8 | S, 16 | S, 32 | S]
+        ncart = [1, 3, 6]
+
+        def timed(table):
+            self.set_group_caps(table)
+            g = self.capture_fock_jk(D, precision)
+            for _ in range(3):
+                g(D)
+            torch.cuda.synchronize(self.device)
+            ts = []
+            for _ in range(reps):
+                t0 = time.perf_counter()
+                for _ in range(steps):
+                    g(D)
+                torch.cuda.synchronize(self.device)
+                ts.append((time.perf_counter() - t0) / steps)
+            return statistics.median(ts)
+
+        best = timed(caps)
+        cls = self.class_stats()
+        key = "prim_quartets_fp32" if prec else "prim_quartets_fp64"
+        order = sorted(range(len(cls)), key=lambda c: -cls[c][key])
+        for c in order:
+            la, lb, lc, ld = cls[c]["class"]
+            if cls[c][key] == 0 or ncart[la] * ncart[lb] * ncart[lc] * ncart[ld] > 81:
+                continue
+            for cap in cands:
+                if cap == caps[c][prec]:
+                    continue
+                trial = [list(x) for x in caps]
+                trial[c][prec] = cap
+                t = timed(trial)
+                if t < best * (1.0 - min_gain):
+                    best, caps = t, trial
+        self.set_group_caps(caps)
+        return caps, best * 1e3
+
     def stats(self):
         counts = (ctypes.c_uint64 * 6)()
         flops = (ctypes.c_double * 2)()

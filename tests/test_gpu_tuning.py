@@ -71,6 +71,24 @@ def test_autotune_choices_and_results():
     assert torch.max(torch.abs(K - K0)).item() < 1e-11
 
 
+def test_whole_build_tuning():
+    """The whole-build refinement returns a valid table that the engine keeps,
+    and builds with it give the plain launch's J/K."""
+    from pyquante_amd import _abi
+    eng = _engine((2, 1, 1), 5)
+    D = _density(eng.nbf, 15)
+    eng.build(1e-12, 0.0)
+    eng.set_group_cap(0)
+    J0, K0 = eng.fock_jk(D)
+    caps, ms = eng.autotune_build(D, reps=1, steps=2)
+    assert ms > 0 and len(caps) == 21
+    S = _abi.PQ_CAP_STAGE_D
+    assert all(c in (-1, 0, 4, 8, 16, 32, 4 | S, 8 | S, 16 | S, 32 | S) for pair in caps for c in pair)
+    J, K = eng.fock_jk(D)
+    assert torch.max(torch.abs(J - J0)).item() < 1e-11
+    assert torch.max(torch.abs(K - K0)).item() < 1e-11
+
+
 @pytest.mark.parametrize("streams", [0, 4])
 def test_graph_replay_matches_eager(streams):
     eng = _engine((2, 1, 1), 3)
