@@ -114,16 +114,18 @@ def kernel_name(cls, cap, nbf):
     return "%s<%d, %d, %d, %d, double, 0>" % (tmpl, *cls)
 
 
-def apply_tuning(eng, D, name, args, precision=0):
+def apply_tuning(eng, D, name, args, precision=0, build_tune=True):
     """Sets the engine's bra-run caps for workload `name` (see --tune);
-    returns the 21 x 2 cap table in use, or None for the defaults."""
+    returns the 21 x 2 cap table in use, or None for the defaults.
+    build_tune=False skips the whole-build refinement (per-class autotune
+    only), for workloads whose builds are too long to time many times."""
     from pyquante_amd import tuning
     if args.tune == "none":
         return None
     caps = tuning.load(name) if args.tune == "table" else None
     if caps is None:
         caps = eng.autotune(D, reps=10)
-        if not args.no_build_tune:
+        if build_tune and not args.no_build_tune:
             caps, ms = eng.autotune_build(D, caps, precision=precision)
             print("bench: whole-build tuning of %s: %.3f ms/build" % (name, ms), file=sys.stderr, flush=True)
         if args.write_tuning:
@@ -280,6 +282,11 @@ def batch_workload(args):
     out = {"batch_molecules": be.nmol, "batch_functions": be.nbf}
     for label, fp32, prec in (("fp64", 0.0, _abi.PQ_PREC_FP64), ("mixed", 1e-4, _abi.PQ_PREC_MIXED)):
         be.build(1e-12, fp32)
+        # bra-run caps: the kernels' work does not depend on the density's
+        # values, so a random packed density serves for live tuning
+        Dt = torch.rand(be.mshape, dtype=torch.float64, device="cuda")
+        apply_tuning(be, Dt, "batch%d_631ss_%s" % (be.nmol, label), args, precision=prec, build_tune=False)
+        del Dt
         st = be.stats()
         torch.cuda.synchronize()
         t0 = time.perf_counter()
