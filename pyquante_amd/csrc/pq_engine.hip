@@ -1277,9 +1277,7 @@ static int fock_jk_impl(pq_engine* eng, int nd, const double* const* Ds, double*
       PQ_TRY(hipMemcpyAsync(eng->d_D2 + nn, Ds[1], nn * sizeof(double), hipMemcpyDeviceToDevice, st));
       D_dev = eng->d_D2;
     }
-    PQ_TRY(launch_fill(eng->d_Jacc, nd * nn, 0.0, st));
-    PQ_TRY(launch_fill(eng->d_Kacc, nd * nn, 0.0, st));
-    if (eng->jp_size > 0) PQ_TRY(launch_fill(eng->d_jpair, nd * eng->jp_size, 0.0, st));
+    PQ_TRY(launch_zero3(eng->d_Jacc, nd * nn, eng->d_Kacc, nd * nn, eng->d_jpair, nd * eng->jp_size, st));
     // fork: class kernels spread over the side streams, joined before the
     // symmetrisation (all of them only add into Jacc/Kacc)
     const int nside = eng->nside;
@@ -1308,10 +1306,12 @@ static int fock_jk_impl(pq_engine* eng, int nd, const double* const* Ds, double*
     if (rc != PQ_OK) break;
     PQ_TRY(join_side(eng, st));
     // the ket-side J, accumulated by ket position, into Jacc
-    for (int y = 0; y < 6 && err == hipSuccess; ++y)
-      err = launch_jpair_scatter(eng->view_s(), y, ncart(kPairL[y][0]), ncart(kPairL[y][1]), eng->d_sorted_s[y], nd,
-                                 eng->d_Jacc, st);
-    PQ_TRY(err);
+    JpairMap jmap;
+    for (int y = 0; y < 6; ++y) {
+      jmap.ids[y] = eng->d_sorted_s[y];
+      jmap.ndf[y] = ncart(kPairL[y][1]);
+    }
+    PQ_TRY(launch_jpair_scatter(eng->view_s(), jmap, nd, eng->d_Jacc, st));
     for (int d = 0; d < nd && rc == PQ_OK; ++d)
       PQ_TRY(eng->nmol == 1
                  ? launch_symmetrize(eng->nbf, eng->d_Jacc + d * nn, eng->d_Kacc + d * nn, Js[d], Ks[d], 1.0, 1.0,

@@ -278,30 +278,34 @@ static hipError_t launch_diag_2222(const EngineView& ev, const Tile* tiles, int 
   return launch_cls_2222_m3(ev, tiles, ntiles, kets, D, Jacc, Kacc, ints, qout, st, nullptr, 0, 0);
 }
 
-// Adds the position-indexed ket J of pair class Y (npair pairs in ket order
-// pair_ids, ndf functions on the second shell, ncd components) into Jacc for
-// nd stacked densities.  Every (pair, component) owns one matrix element, so
-// plain read-modify-writes suffice once the J/K kernels have finished.
-__global__ void k_jpair_scatter(EngineView ev, int Y, const int* __restrict__ pair_ids, int npair, int ncd, int ndf,
-                                int nd, double* Jacc) {
+// Adds the position-indexed ket J of every pair class into Jacc for nd
+// stacked densities, one thread per (class, component, ket position).  Every
+// (pair, component) owns one matrix element, so plain read-modify-writes
+// suffice once the J/K kernels have finished.
+__global__ void k_jpair_scatter(EngineView ev, JpairMap map, int nd, double* Jacc) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= npair * ncd) return;
-  const int cd = i / npair, j = i - cd * npair;
-  const PairInfo p = ev.pairs[pair_ids[j]];
+  if (i >= ev.jp_size) return;
+  // class blocks are consecutive; an empty class shares its offset with the
+  // next one, so the last class starting at or before i owns it
+  int Y = 0;
+#pragma unroll
+  for (int y = 1; y < 6; ++y)
+    if (i >= ev.jp_off[y]) Y = y;
+  const int np = ev.jp_n[Y], ndf = map.ndf[Y];
+  const int local = i - ev.jp_off[Y];
+  const int cd = local / np, j = local - cd * np;
+  const PairInfo p = ev.pairs[map.ids[Y][j]];
   const int64_t e = p.moff + (int64_t)(p.bfA + cd / ndf) * p.mstride + p.bfB + cd % ndf;
   for (int d = 0; d < nd; ++d) {
-    const double v = ev.jpair[d * ev.jp_size + ev.jp_off[Y] + i];
+    const double v = ev.jpair[d * ev.jp_size + i];
     if (v != 0.0) Jacc[d * ev.msize + e] += v;
   }
 }
 
-hipError_t launch_jpair_scatter(const EngineView& ev, int Y, int nc, int ndf, const int* pair_ids, int nd,
-                                double* Jacc, hipStream_t st) {
-  const int np = ev.jp_n[Y];
-  if (np <= 0) return hipSuccess;
-  const int tot = np * nc * ndf;
-  hipLaunchKernelGGL(k_jpair_scatter, dim3((tot + 255) / 256), dim3(256), 0, st, ev, Y, pair_ids, np, nc * ndf, ndf,
-                     nd, Jacc);
+hipError_t launch_jpair_scatter(const EngineView& ev, const JpairMap& map, int nd, double* Jacc, hipStream_t st) {
+  if (ev.jp_size <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_jpair_scatter, dim3((unsigned)((ev.jp_size + 255) / 256)), dim3(256), 0, st, ev, map, nd,
+                     Jacc);
   return hipGetLastError();
 }
 

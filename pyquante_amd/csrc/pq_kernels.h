@@ -29,10 +29,18 @@ hipError_t launch_eri_class(int la, int lb, int lc, int ld, int mode, int fp32, 
                             double* Kacc, double* ints, double* qout, hipStream_t st,
                             const int2* groups = nullptr, int ngroups = 0, int gflags = 0);
 
-// adds the position-indexed ket J of pair class Y (ev.jpair; nc x ndf
-// functions per pair, ket order pair_ids) into nd stacked Jacc matrices
-hipError_t launch_jpair_scatter(const EngineView& ev, int Y, int nc, int ndf, const int* pair_ids, int nd,
-                                double* Jacc, hipStream_t st);
+// ket order of each pair class (ids[Y][j] = pair at position j) and the
+// function count of its second shell
+struct JpairMap {
+  const int* ids[6];
+  int ndf[6];
+};
+// adds the position-indexed ket J of every pair class (ev.jpair) into nd
+// stacked Jacc matrices, one launch
+hipError_t launch_jpair_scatter(const EngineView& ev, const JpairMap& map, int nd, double* Jacc, hipStream_t st);
+
+// zeroes three arrays in one launch (fill kernel, safe under graph capture)
+hipError_t launch_zero3(double* a, int64_t na, double* b, int64_t nb, double* c, int64_t nc, hipStream_t st);
 
 // two-electron nuclear gradient into kGradCopies private copies of the
 // natom x 3 gradient (gacc), summed by launch_grad_finish

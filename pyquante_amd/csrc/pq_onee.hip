@@ -18,6 +18,24 @@ __global__ void k_fill(double* p, int64_t n, double v) {
   for (; i < n; i += stride) p[i] = v;
 }
 
+__global__ void k_zero3(double* a, int64_t na, double* b, int64_t nb, double* c, int64_t nc) {
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < na + nb + nc; i += stride) {
+    if (i < na) a[i] = 0.0;
+    else if (i < na + nb) b[i - na] = 0.0;
+    else c[i - na - nb] = 0.0;
+  }
+}
+
+hipError_t launch_zero3(double* a, int64_t na, double* b, int64_t nb, double* c, int64_t nc, hipStream_t st) {
+  const int64_t n = na + nb + nc;
+  if (n <= 0) return hipSuccess;
+  int64_t blocks = (n + 255) / 256;
+  if (blocks > 65536) blocks = 65536;
+  hipLaunchKernelGGL(k_zero3, dim3((unsigned)blocks), dim3(256), 0, st, a, na, b, nb, c, nc);
+  return hipGetLastError();
+}
+
 hipError_t launch_fill(double* p, int64_t n, double v, hipStream_t st) {
   if (n <= 0) return hipSuccess;
   int64_t blocks = (n + 255) / 256;
