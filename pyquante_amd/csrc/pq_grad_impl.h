@@ -16,11 +16,14 @@
 
 namespace pq {
 
+// Component split of the gradient kernels: up to 54 Cartesian integrals per
+// lane (each split recomputes the roots and 1-D integrals, so fewer splits
+// beat the occupancy their registers cost; measured against 27 and 81).
 template <int LA, int LB, int LC, int LD>
 struct GCls {
   using C = Cls<LA, LB, LC, LD>;
   static constexpr int NCOMP = C::NCOMP;
-  static constexpr int SPLIT = NCOMP <= 27 ? 0 : (NCOMP / C::NA <= 27 ? 1 : 2);
+  static constexpr int SPLIT = NCOMP <= 54 ? 0 : (NCOMP / C::NA <= 54 ? 1 : 2);
   static constexpr int NSPLIT = SPLIT == 0 ? 1 : (SPLIT == 1 ? C::NA : C::NA * C::NB);
   static constexpr int NR = (C::L + 1) / 2 + 1;
 };
