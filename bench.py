@@ -105,12 +105,12 @@ def recorded_traffic(key):
 
 def kernel_name(cls, cap, nbf):
     """Kernel template a class's FP64 J/K tiles run in: bra runs (k_eri_group)
-    for classes of at most 81 components when the run cap is nonzero."""
+    for classes of at most 108 components when the run cap is nonzero."""
     ncart = [1, 3, 6]
     ncomp = ncart[cls[0]] * ncart[cls[1]] * ncart[cls[2]] * ncart[cls[3]]
     if cap is None:
         cap = 16 if nbf >= 300 else 0
-    tmpl = "k_eri_group" if ncomp <= 81 and cap > 0 else "k_eri_lane"
+    tmpl = "k_eri_group" if ncomp <= 108 and cap > 0 else "k_eri_lane"
     return "%s<%d, %d, %d, %d, double, 0>" % (tmpl, *cls)
 
 

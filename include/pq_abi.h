@@ -142,7 +142,7 @@ int pq_engine_set_prim_screen(pq_engine *eng, double rel);
  * counterpart (a launch knob). */
 int pq_engine_set_streams(pq_engine *eng, int nstreams);
 
-/* Bra runs: for the J/K kernels of classes with at most 81 Cartesian
+/* Bra runs: for the J/K kernels of classes with at most 108 Cartesian
  * components, one 256-thread block walks up to `cap` consecutive tiles that
  * share a bra shell pair and flushes its LDS copy of the bra K rows once.
  * cap = 0 disables runs, -1 restores the default (16 tiles when nbf >= 300,
@@ -165,7 +165,7 @@ int pq_engine_set_group_caps(pq_engine *eng, const int *caps);
  * on `stream` and keeps the fastest.  The choice
  * persists across pq_build_pairs calls of this engine.  caps_out (optional,
  * 42 ints) receives the chosen cap per (class, precision); classes with more
- * than 81 components report their unused cap.  Overwrites the engine's J/K
+ * than 108 components report their unused cap.  Overwrites the engine's J/K
  * scratch only.  No reference counterpart (a launch knob). */
 int pq_engine_autotune(pq_engine *eng, const double *D_dev, int reps, void *stream, int *caps_out);
 

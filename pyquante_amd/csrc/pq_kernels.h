@@ -13,7 +13,7 @@ constexpr bool mode_is_jk(int m) { return m == MODE_JK || m == MODE_JK2; }
 constexpr int mode_ndens(int m) { return m == MODE_JK2 ? 2 : 1; }
 
 // classes with at most this many Cartesian integrals run one lane per quartet
-constexpr int kLaneMaxComp = 81;
+constexpr int kLaneMaxComp = 108;
 
 // private copies of the natom x 3 gradient that the gradient kernels spread
 // their atomics over (summed by launch_grad_finish)
