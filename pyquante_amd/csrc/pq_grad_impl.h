@@ -67,10 +67,22 @@ __device__ __forceinline__ void grad_lane(const EngineView& ev, const Tile& tl, 
     const int i1 = ni ? i0 + ni : bp.pp_n;
     const double AB[3] = {bp.ABx, bp.ABy, bp.ABz};
     const double CD[3] = {kp.ABx, kp.ABy, kp.ABz};
+    // the next ket primitive pair is loaded while the current one is
+    // processed (its latency hides behind the bra loop)
+    const int k0 = kc * kp.pp_n / ks, k1 = (kc + 1) * kp.pp_n / ks;
+    PrimPair qn{};
+    double gn = 0.0;
+    if (k0 < k1) {
+      qn = ev.prims[kp.pp_off + k0];
+      gn = ev.prim_a[kp.pp_off + k0];
+    }
 #pragma unroll 1
-    for (int k = kc * kp.pp_n / ks; k < (kc + 1) * kp.pp_n / ks; ++k) {
-      const PrimPair q = ev.prims[kp.pp_off + k];
-      const double gam = ev.prim_a[kp.pp_off + k];
+    for (int k = k0; k < k1; ++k) {
+      const PrimPair q = qn;
+      const double gam = gn;
+      const int kn = k + 1 < k1 ? k + 1 : k;
+      qn = ev.prims[kp.pp_off + kn];
+      gn = ev.prim_a[kp.pp_off + kn];
 #pragma unroll 1
       for (int i = i0; i < i1; ++i) {
         const PrimPair b = ld_uniform(ev.prims + bp.pp_off + i);
