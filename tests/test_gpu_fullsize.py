@@ -1,0 +1,69 @@
+"""GPU: the benchmark configurations at full size, checked through properties
+that do not need the oracle at that size.
+
+- (H2O)_8 6-31G** unscreened: 4,656 shell pairs, so 4656*4657/2 = 10,841,496
+  unique shell quartets and 20100*20101/2 = 202,015,050 unique function
+  quartets (closed forms of the enumeration in Ints.py:146-167);
+- J and K are linear in D: J(D1 + 2 D2) = J(D1) + 2 J(D2) to 1e-10 relative,
+  with screening and primitive screening on (they depend on the pairs, not on
+  D);
+- the two-density pass equals two single builds to 1e-11;
+- (H2O)_16 6-31G** (400 functions): the mixed-precision energy functional
+  tr(D(2J - K)) is within 1e-6 Eh of the FP64 one for a seeded density (the
+  accuracy bound of the mixed path).
+"""
+import numpy as np
+import pytest
+import torch
+
+pytestmark = pytest.mark.gpu
+
+
+def _engine(lattice, seed=0):
+    from pyquante_amd.basis import BasisSet
+    from pyquante_amd.engine import Engine
+    from pyquante_amd.molecule import water_cluster
+    mol = water_cluster(lattice, seed=seed)
+    return Engine(BasisSet(mol, "6-31g**"), mol)
+
+
+def _density(n, k, seed):
+    rng = np.random.default_rng(seed)
+    C = rng.standard_normal((n, k)) * 0.1
+    return torch.as_tensor(C @ C.T, device="cuda")
+
+
+def test_h2o8_counts_and_linearity():
+    eng = _engine((2, 2, 2))
+    assert eng.nbf == 200
+    eng.build(0.0)
+    st = eng.stats()
+    assert st["quartets_fp64"] == 4656 * 4657 // 2
+    assert st["fquartets_fp64"] == 20100 * 20101 // 2
+    eng.build(1e-12)
+    D1 = _density(eng.nbf, 20, 1)
+    D2 = _density(eng.nbf, 20, 2)
+    J1, K1 = eng.fock_jk(D1)
+    J2, K2 = eng.fock_jk(D2)
+    J3, K3 = eng.fock_jk(D1 + 2.0 * D2)
+    for a, b in ((J3, J1 + 2.0 * J2), (K3, K1 + 2.0 * K2)):
+        assert torch.max(torch.abs(a - b)).item() < 1e-10 * torch.max(torch.abs(b)).item()
+    (Ja, Ka), (Jb, Kb) = eng.fock_jk_multi([D1, D2])
+    for a, b in ((Ja, J1), (Ka, K1), (Jb, J2), (Kb, K2)):
+        assert torch.max(torch.abs(a - b)).item() < 1e-11
+
+
+def test_h2o16_mixed_precision_energy_bound():
+    from pyquante_amd import _abi
+    eng = _engine((2, 2, 4))
+    assert eng.nbf == 400
+    D = _density(eng.nbf, 40, 3)
+    eng.build(1e-12, 0.0)
+    J64, K64 = eng.fock_jk(D)
+    eng.build(1e-12, 1e-4)
+    assert eng.stats()["fquartets_fp32"] > eng.stats()["fquartets_fp64"]
+    J32, K32 = eng.fock_jk(D, precision=_abi.PQ_PREC_MIXED)
+    e64 = float(torch.sum(D * (2.0 * J64 - K64)))
+    e32 = float(torch.sum(D * (2.0 * J32 - K32)))
+    print("(H2O)_16 mixed - FP64 energy functional: %.2e" % (e32 - e64), flush=True)
+    assert abs(e32 - e64) < 1e-6
