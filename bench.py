@@ -19,7 +19,8 @@ FETCH_SIZE/WRITE_SIZE runs (profiles/traffic.json), when recorded;
 (2 FMA + ADD + MUL) per launch, profiles/valu.json) over the same launch
 duration.  The cpu_baseline block times the compiled reference crys.c (oracle/_ref) on a
 seeded sample of the same workload's unique function quartets,
-single-threaded.
+single-threaded; cpu_baseline_all_cores runs the same sample in one process
+per host core (up to 16) and sums their rates.
 
 Outside the timed region the line also carries: (H2O)_16 FP64 against mixed
 precision (times, energy error), forces per second on (H2O)_8, J/K of two
@@ -43,48 +44,84 @@ sys.path.insert(0, ROOT)
 FP64_PEAK_TFLOPS = 78.6
 
 
-def cpu_baseline(bfs, seconds=10.0, seed=0, method="crys"):
-    """Time the reference's crys.c contr_coulomb (or chgp.c contr_hrr),
-    compiled from the reference sources into oracle/_ref, on random unique
-    function quartets."""
+def _basis_arrays(bfs):
+    """Flat primitive arrays of a basis in the reference batch harness's layout."""
+    import numpy as np
+    nprim = np.array([len(f.pexps) for f in bfs], dtype=np.int32)
+    return {"nprim": nprim, "poff": np.concatenate([[0], np.cumsum(nprim)[:-1]]).astype(np.int32),
+            "e": np.array([x for f in bfs for x in f.pexps]), "c": np.array([x for f in bfs for x in f.pcoefs]),
+            "nr": np.array([x for f in bfs for x in f.pnorms]), "org": np.array([f.origin for f in bfs]),
+            "pw": np.array([f.powers for f in bfs], dtype=np.int32)}
+
+
+def _reference_batches(job):
+    """Runs the compiled reference contraction on random unique function
+    quartets for `seconds` of compute; returns (quartets done, seconds).
+    Module-level so that spawned worker processes can run it."""
     import ctypes
 
     import numpy as np
-    path = os.path.join(ROOT, "oracle", "_ref", "libref_%s.so" % method)
-    if not os.path.exists(path):
-        return None
+    path, method, arrays, seconds, seed = job
     lib = ctypes.CDLL(path)
     PD, PI = ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_int)
     batch = getattr(lib, "pqref_%s_contr_batch" % method)
     batch.argtypes = [ctypes.c_int, PI, PI, PD, PD, PD, PD, PI, PD]
     rng = np.random.default_rng(seed)
-    n = len(bfs)
-    nprim = np.array([len(f.pexps) for f in bfs], dtype=np.int32)
-    poff = np.concatenate([[0], np.cumsum(nprim)[:-1]]).astype(np.int32)
-    e = np.array([x for f in bfs for x in f.pexps])
-    c = np.array([x for f in bfs for x in f.pcoefs])
-    nr = np.array([x for f in bfs for x in f.pnorms])
-    org = np.array([f.origin for f in bfs])
-    pw = np.array([f.powers for f in bfs], dtype=np.int32)
-
-    def sample(m):
-        q = rng.integers(0, n, size=(m, 4))
-        return (np.ascontiguousarray(nprim[q]), np.ascontiguousarray(poff[q]),
-                np.ascontiguousarray(org[q].reshape(m, 12)), np.ascontiguousarray(pw[q].reshape(m, 12)))
-
+    a_ = arrays
+    n = len(a_["nprim"])
     done, t_used, m = 0, 0.0, 2000
     while t_used < seconds:
-        a, b, o, p = sample(m)
+        q = rng.integers(0, n, size=(m, 4))
+        a = np.ascontiguousarray(a_["nprim"][q])
+        b = np.ascontiguousarray(a_["poff"][q])
+        o = np.ascontiguousarray(a_["org"][q].reshape(m, 12))
+        p = np.ascontiguousarray(a_["pw"][q].reshape(m, 12))
         out = np.zeros(m)
         t0 = time.perf_counter()
-        batch(m, a.ctypes.data_as(PI), b.ctypes.data_as(PI), e.ctypes.data_as(PD), c.ctypes.data_as(PD),
-              nr.ctypes.data_as(PD), o.ctypes.data_as(PD), p.ctypes.data_as(PI), out.ctypes.data_as(PD))
+        batch(m, a.ctypes.data_as(PI), b.ctypes.data_as(PI), a_["e"].ctypes.data_as(PD),
+              a_["c"].ctypes.data_as(PD), a_["nr"].ctypes.data_as(PD), o.ctypes.data_as(PD), p.ctypes.data_as(PI),
+              out.ctypes.data_as(PD))
         t_used += time.perf_counter() - t0
         done += m
+    return done, t_used
+
+
+def cpu_baseline(bfs, seconds=10.0, seed=0, method="crys"):
+    """Time the reference's crys.c contr_coulomb (or chgp.c contr_hrr),
+    compiled from the reference sources into oracle/_ref, on random unique
+    function quartets, on one thread (the reference is single-threaded)."""
+    path = os.path.join(ROOT, "oracle", "_ref", "libref_%s.so" % method)
+    if not os.path.exists(path):
+        return None
+    done, t_used = _reference_batches((path, method, _basis_arrays(bfs), seconds, seed))
     fn = "crys.c contr_coulomb" if method == "crys" else "chgp.c contr_hrr"
     return {"value": done / t_used, "unit": "quartets/s", "cores": 1, "kind": "reference",
             "sample": "%d random unique function quartets of the (H2O)_8 6-31G** basis, reference "
                       "%s compiled -O2, %.1f s, 1 thread" % (done, fn, t_used)}
+
+
+def cpu_baseline_all_cores(bfs, seconds=5.0, seed=0, method="crys", max_workers=16):
+    """The same reference contraction in one process per host core (the
+    reference keeps its Rys roots in static arrays, so threads of one process
+    would race): the summed rate of up to max_workers spawned workers."""
+    import multiprocessing as mp
+    path = os.path.join(ROOT, "oracle", "_ref", "libref_%s.so" % method)
+    if not os.path.exists(path):
+        return None
+    try:
+        cores = len(os.sched_getaffinity(0))
+    except AttributeError:
+        cores = os.cpu_count() or 1
+    workers = max(1, min(max_workers, cores))
+    arrays = _basis_arrays(bfs)
+    jobs = [(path, method, arrays, seconds, seed + 1 + i) for i in range(workers)]
+    with mp.get_context("spawn").Pool(workers) as pool:
+        res = pool.map(_reference_batches, jobs)
+    rate = sum(d / t for d, t in res)
+    fn = "crys.c contr_coulomb" if method == "crys" else "chgp.c contr_hrr"
+    return {"value": rate, "unit": "quartets/s", "cores": workers, "kind": "reference",
+            "sample": "%d random unique function quartets of the (H2O)_8 6-31G** basis, reference %s compiled "
+                      "-O2, %d processes x %.1f s" % (sum(d for d, _ in res), fn, workers, seconds)}
 
 
 def recorded(name, key, field):
@@ -449,9 +486,10 @@ def main():
     extra = {}
     if not args.no_extra:
         extra = extra_workloads(args)
-    cpu = None
+    cpu = cpu_all = None
     if not args.no_cpu:
         cpu = cpu_baseline(bs.bfs, seconds=args.cpu_seconds)
+        cpu_all = cpu_baseline_all_cores(bs.bfs)
     fq = st["fquartets_fp64"]
     line = {
         "metric": "ERI quartets/s + s per Fock build, mixed vs FP64 at <=1e-6 Eh; forces/s",
@@ -479,6 +517,7 @@ def main():
                      "all_kernels_tflops": total_flops / (acc[42] * 1e-3) / 1e12,
                      "counted_fp64_tflops": hw_flops / ktime / 1e12 if hw_flops else None},
         "cpu_baseline": cpu,
+        "cpu_baseline_all_cores": cpu_all,
     }
     print(json.dumps(line))
     return 0
