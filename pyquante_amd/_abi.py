@@ -17,6 +17,12 @@ PQ_CAP_STAGE_D = 4096  # bra-run cap flag: stage the bra rows of D in LDS
 if not os.path.exists(LIB_PATH):
     raise ImportError("libpqhip.so is not built (run __graft_entry__.build() or make -C pyquante_amd/csrc)")
 
+# torch first: the process then has one HIP runtime, the one torch's device
+# memory and streams live in, and libpqhip.so binds to it.  Loaded first,
+# libpqhip.so would bring in the system runtime, and torch would then find no
+# device.
+import torch  # noqa: E402,F401
+
 lib = ctypes.CDLL(LIB_PATH)
 
 _c = ctypes

@@ -1159,10 +1159,30 @@ int pq_get2ints_hgp(pq_engine* eng, double* out_dev, uint64_t len, int compat_nr
   if (eng->nmol > 1) return fail(PQ_E_STATE, "stored integrals need a single-molecule engine");
   const uint64_t n = eng->nbf, npair = n * (n + 1) / 2;
   if (len != npair * (npair + 1) / 2) return fail(PQ_E_ARG, "output length must be npair*(npair+1)/2");
+  hipStream_t st = (hipStream_t)stream;
+  if (!compat_nr) {
+    // one shell quartet per lane over the store work lists
+    if (!eng->pairs_ok) {
+      int rc = build_pair_data(eng);
+      if (rc) return rc;
+    }
+    if (!eng->store_ok) {
+      int rc = build_work(eng, eng->store_work, 0.0, 0.0, false, eng->pairs, eng->sorted);
+      if (rc) return rc;
+      eng->store_ok = true;
+    }
+    PQ_HIP(launch_fill(out_dev, (int64_t)len, 0.0, st));
+    for (const auto& c : eng->store_work) {
+      if (c.t64.empty()) continue;
+      PQ_HIP(launch_hgp_class(kPairL[c.X][0], kPairL[c.X][1], kPairL[c.Y][0], kPairL[c.Y][1], eng->view(),
+                              c.d_t64, (int)c.t64.size(), eng->d_sorted[c.Y], out_dev, st));
+    }
+    return PQ_OK;
+  }
   int rc = build_functions(eng);
   if (rc) return rc;
   PQ_HIP(launch_hgp_store(len, eng->d_fnp, eng->d_foff, eng->d_forg, eng->d_fpw, eng->d_fexps, eng->d_fcoefs,
-                          eng->d_fnorms, compat_nr, out_dev, (hipStream_t)stream));
+                          eng->d_fnorms, compat_nr, out_dev, st));
   return PQ_OK;
 }
 

@@ -351,6 +351,61 @@ hipError_t launch_eri_class(int la, int lb, int lc, int ld, int mode, int fp32, 
 }
 
 
+#define PQ_HDECL(A, B, C_, D_)                                                                          \
+  hipError_t launch_hgp_##A##B##C_##D_(const EngineView& ev, const Tile* tiles, int ntiles, const int* kets, \
+                                       double* ints, hipStream_t st);
+PQ_HDECL(0, 0, 0, 0)
+PQ_HDECL(1, 0, 0, 0)
+PQ_HDECL(1, 0, 1, 0)
+PQ_HDECL(1, 1, 0, 0)
+PQ_HDECL(1, 1, 1, 0)
+PQ_HDECL(1, 1, 1, 1)
+PQ_HDECL(2, 0, 0, 0)
+PQ_HDECL(2, 0, 1, 0)
+PQ_HDECL(2, 0, 1, 1)
+PQ_HDECL(2, 0, 2, 0)
+PQ_HDECL(2, 1, 0, 0)
+PQ_HDECL(2, 1, 1, 0)
+PQ_HDECL(2, 1, 1, 1)
+PQ_HDECL(2, 1, 2, 0)
+PQ_HDECL(2, 1, 2, 1)
+PQ_HDECL(2, 2, 0, 0)
+PQ_HDECL(2, 2, 1, 0)
+PQ_HDECL(2, 2, 1, 1)
+PQ_HDECL(2, 2, 2, 0)
+PQ_HDECL(2, 2, 2, 1)
+PQ_HDECL(2, 2, 2, 2)
+#undef PQ_HDECL
+
+hipError_t launch_hgp_class(int la, int lb, int lc, int ld, const EngineView& ev, const Tile* tiles, int ntiles,
+                            const int* kets, double* ints, hipStream_t st) {
+#define PQ_HCASE(A, B, C_, D_) \
+  if (la == A && lb == B && lc == C_ && ld == D_) return launch_hgp_##A##B##C_##D_(ev, tiles, ntiles, kets, ints, st);
+  PQ_HCASE(0, 0, 0, 0)
+  PQ_HCASE(1, 0, 0, 0)
+  PQ_HCASE(1, 0, 1, 0)
+  PQ_HCASE(1, 1, 0, 0)
+  PQ_HCASE(1, 1, 1, 0)
+  PQ_HCASE(1, 1, 1, 1)
+  PQ_HCASE(2, 0, 0, 0)
+  PQ_HCASE(2, 0, 1, 0)
+  PQ_HCASE(2, 0, 1, 1)
+  PQ_HCASE(2, 0, 2, 0)
+  PQ_HCASE(2, 1, 0, 0)
+  PQ_HCASE(2, 1, 1, 0)
+  PQ_HCASE(2, 1, 1, 1)
+  PQ_HCASE(2, 1, 2, 0)
+  PQ_HCASE(2, 1, 2, 1)
+  PQ_HCASE(2, 2, 0, 0)
+  PQ_HCASE(2, 2, 1, 0)
+  PQ_HCASE(2, 2, 1, 1)
+  PQ_HCASE(2, 2, 2, 0)
+  PQ_HCASE(2, 2, 2, 1)
+  PQ_HCASE(2, 2, 2, 2)
+#undef PQ_HCASE
+  return hipErrorInvalidValue;
+}
+
 hipError_t launch_grad_class(int la, int lb, int lc, int ld, const EngineView& ev, const Tile* tiles, int ntiles,
                              const int* kets, const double* D, double* gacc, int natom, hipStream_t st) {
 #define PQ_GCASE(A, B, C_, D_)                            \

@@ -28,6 +28,10 @@ hipError_t launch_eri_class(int la, int lb, int lc, int ld, int mode, int fp32, 
                             const Tile* tiles, int ntiles, const int* kets, const double* D, double* Jacc,
                             double* Kacc, double* ints, double* qout, hipStream_t st,
                             const int2* groups = nullptr, int ngroups = 0, int gflags = 0);
+// Shell-quartet Head-Gordon-Pople kernel of one class over store tiles
+// (the tiles and kets of MODE_STORE), writing the flat unique-ERI array.
+hipError_t launch_hgp_class(int la, int lb, int lc, int ld, const EngineView& ev, const Tile* tiles, int ntiles,
+                            const int* kets, double* ints, hipStream_t st);
 
 // ket order of each pair class (ids[Y][j] = pair at position j) and the
 // function count of its second shell

@@ -6,7 +6,10 @@ reference chgp.c and against the Rys engine.
 - full-precision Boys against the Rys engine and the compiled crys.c values:
   1e-12 absolute (the methods agree to rounding once both use exact Boys
   values);
-- vrr leaves against the (ss|ss) closed form pref * F_m(T).
+- vrr leaves against the (ss|ss) closed form pref * F_m(T);
+- the shell-quartet kernel (VRR per primitive quartet, HRR once per
+  contracted block, Boys values from Rys moments) against the Rys store
+  kernels and the per-function kernel on every class.
 """
 import math
 
@@ -142,6 +145,51 @@ def test_get2ints_hgp_h2o_631ss(golden):
     assert np.max(np.abs(exact - g["crys"])) < 1e-12
     compat = eng.get2ints("hgp", 1).cpu().numpy()
     assert np.max(np.abs(compat[g["index5"]] - g["chgp5"])) < 1e-12
+
+
+def test_get2ints_hgp_shell_quartets_every_class():
+    """The shell-quartet HGP kernel (get2ints "hgp", full-precision Boys) over
+    a water dimer in 6-31G**, which holds all 21 classes up to (dd|dd):
+    against the Rys store kernels over the whole array, and against the
+    per-function HGP kernel (independent VRR per leaf, series Boys values) on
+    a seeded sample of quartets from every class."""
+    import torch
+
+    from pyquante_amd.basis import BasisSet
+    from pyquante_amd.engine import Engine
+    from pyquante_amd.molecule import water_cluster
+    mol = water_cluster((2, 1, 1), seed=4)
+    bs = BasisSet(mol, "6-31g**")
+    eng = Engine(bs, mol)
+    hgp = eng.get2ints("hgp", 0)
+    rys = eng.get2ints("rys")
+    assert hgp.shape == rys.shape
+    assert torch.isfinite(hgp).all()
+    assert float((hgp - rys).abs().max()) < 1e-12
+    bfs = bs.bfs
+    L = [sum(f.powers) for f in bfs]
+    rng = np.random.default_rng(11)
+    by_class = {}
+    n = len(bfs)
+    for _ in range(60000):
+        i, j, k, l = (int(x) for x in rng.integers(0, n, 4))
+        if i < j:
+            i, j = j, i
+        if k < l:
+            k, l = l, k
+        if i * (i + 1) // 2 + j < k * (k + 1) // 2 + l:
+            i, j, k, l = k, l, i, j
+        key = tuple(sorted((L[i], L[j]), reverse=True)) + tuple(sorted((L[k], L[l]), reverse=True))
+        by_class.setdefault(key, [])
+        if len(by_class[key]) < 40:
+            by_class[key].append((i, j, k, l))
+    qs = [q for v in by_class.values() for q in v]
+    assert max(L) == 2 and (2, 2, 2, 2) in by_class
+    per_function = _batch(bfs, qs, 0)
+    idx = [i * (i + 1) // 2 + j for i, j, _, _ in qs], [k * (k + 1) // 2 + l for _, _, k, l in qs]
+    flat = [ij * (ij + 1) // 2 + kl for ij, kl in zip(*idx)]
+    shell = hgp.cpu().numpy()[flat]
+    assert np.max(np.abs(shell - np.asarray(per_function))) < 1e-12
 
 
 def test_rhf_hgp_backend_matches_rys():
