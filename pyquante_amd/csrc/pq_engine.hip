@@ -1172,6 +1172,11 @@ int pq_get2ints_hgp(pq_engine* eng, double* out_dev, uint64_t len, int compat_nr
       eng->store_ok = true;
     }
     PQ_HIP(launch_fill(out_dev, (int64_t)len, 0.0, st));
+    // all classes on the caller's stream: the d-shell classes keep their
+    // recurrence tables in per-lane private memory, whose backing store the
+    // runtime allocates per hardware queue; spread over the side streams'
+    // queues as well, it outgrew the runtime's scratch limit and the next
+    // large-scratch dispatch failed (HSA_STATUS_ERROR_OUT_OF_RESOURCES)
     for (const auto& c : eng->store_work) {
       if (c.t64.empty()) continue;
       PQ_HIP(launch_hgp_class(kPairL[c.X][0], kPairL[c.X][1], kPairL[c.Y][0], kPairL[c.Y][1], eng->view(),

@@ -18,12 +18,14 @@
 // F_m(X) = sum_r w_r t_r^(2m) holds exactly for m < 2 (L/2+1), which covers
 // every m <= L, and every term is positive.
 //
-// Up to kHgpUnrollMax table entries every index of the recurrences is a
-// compile-time constant (static_for over the Cartesian triples), so the light
-// classes keep their tables in registers.  The two largest classes, (dd|dp)
-// and (dd|dd), walk step tables in constant memory instead (uniform loads),
-// with the recurrence table in per-lane private memory (at most 3,366
-// doubles): unrolled, they take minutes to compile and gain little.
+// Up to kHgpUnrollMax table entries (the s/p classes up to (pp|ps) and
+// (ds|ps)) every index of the recurrences is a compile-time constant
+// (static_for over the Cartesian triples), and the tables stay in registers.
+// The larger classes walk step tables in constant memory instead (uniform
+// loads), with the recurrence table in per-lane private memory (at most 3,366
+// doubles): unrolled, their straight-line code spills and outgrows the
+// instruction cache, and ran 10-40x slower per tile (measured), and (dd|dd)
+// took 12 minutes to compile.
 #pragma once
 #include <type_traits>
 
@@ -94,7 +96,7 @@ struct HgpCls {
   static constexpr int SIZE = off(NE, 0);
 };
 
-constexpr int kHgpUnrollMax = 700;
+constexpr int kHgpUnrollMax = 60;
 
 // Step tables of the rolled variant.  A VRR step writes nm orders at o from
 // the entries at o1 (and o2, oe); dir is the recurrence direction, c1 the
