@@ -191,9 +191,13 @@ int pq_engine_class_stats(const pq_engine *eng, int c, int l[4], uint64_t counts
 int pq_get2ints(pq_engine *eng, double *out_dev, uint64_t len, void *stream);
 
 /* pq_get2ints by Head-Gordon-Pople recurrences: the reference's get2ints with
- * CGBF.coulomb bound to chgp (CGBF.py:23-25, Ints.py:146-167), one function
- * quartet per lane as chgp evaluates them (chgp.c:78-419).  compat_nr = 1
- * uses chgp's EPS = 3e-7 Boys function, 0 the full-precision one. */
+ * CGBF.coulomb bound to chgp (CGBF.py:23-25, Ints.py:146-167; recurrences
+ * chgp.c:78-419).  compat_nr = 0: one shell quartet per lane, the vertical
+ * recurrence per primitive quartet and the horizontal one once per
+ * contracted block, full-precision Boys values (from the Rys rule of the
+ * class), on the given stream.  compat_nr = 1: one function quartet per lane
+ * in chgp's own order, with chgp's EPS = 3e-7 Boys function started at each
+ * leaf's order, which reproduces chgp's values to 1e-12. */
 int pq_get2ints_hgp(pq_engine *eng, double *out_dev, uint64_t len, int compat_nr, void *stream);
 
 /* Shell-quartet blocks in the libint layout, the interface of the

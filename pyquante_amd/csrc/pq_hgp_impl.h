@@ -96,7 +96,7 @@ struct HgpCls {
   static constexpr int SIZE = off(NE, 0);
 };
 
-constexpr int kHgpUnrollMax = 60;
+constexpr int kHgpUnrollMax = 40;
 
 // Step tables of the rolled variant.  A VRR step writes nm orders at o from
 // the entries at o1 (and o2, oe); dir is the recurrence direction, c1 the
