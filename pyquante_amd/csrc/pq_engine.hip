@@ -135,6 +135,8 @@ struct pq_engine {
   std::vector<ClassWork> work;
   std::vector<ClassWork> store_work;  // unscreened, for get2ints
   bool store_ok = false;
+  std::vector<ClassWork> hgp_work;  // unscreened, primitive splits, for the HGP get2ints
+  bool hgp_ok = false;
   double* d_Jacc = nullptr;  // 2 n^2: room for two densities
   double* d_Kacc = nullptr;
   double* d_D2 = nullptr;    // two stacked densities (allocated on first use)
@@ -226,7 +228,8 @@ struct pq_engine {
     }
     free_work(work);
     free_work(store_work);
-    pairs_ok = work_ok = store_ok = false;
+    free_work(hgp_work);
+    pairs_ok = work_ok = store_ok = hgp_ok = false;
   }
   ~pq_engine() {
     if (streams_ok) {
@@ -1166,10 +1169,12 @@ int pq_get2ints_hgp(pq_engine* eng, double* out_dev, uint64_t len, int compat_nr
       int rc = build_pair_data(eng);
       if (rc) return rc;
     }
-    if (!eng->store_ok) {
-      int rc = build_work(eng, eng->store_work, 0.0, 0.0, false, eng->pairs, eng->sorted);
+    if (!eng->hgp_ok) {
+      // tiles split over bra primitives and ket-primitive lanes, as for
+      // J/K: the split tiles add their partial blocks into the zeroed array
+      int rc = build_work(eng, eng->hgp_work, 0.0, 0.0, true, eng->pairs, eng->sorted);
       if (rc) return rc;
-      eng->store_ok = true;
+      eng->hgp_ok = true;
     }
     PQ_HIP(launch_fill(out_dev, (int64_t)len, 0.0, st));
     // all classes on the caller's stream: the d-shell classes keep their
@@ -1177,7 +1182,7 @@ int pq_get2ints_hgp(pq_engine* eng, double* out_dev, uint64_t len, int compat_nr
     // runtime allocates per hardware queue; spread over the side streams'
     // queues as well, it outgrew the runtime's scratch limit and the next
     // large-scratch dispatch failed (HSA_STATUS_ERROR_OUT_OF_RESOURCES)
-    for (const auto& c : eng->store_work) {
+    for (const auto& c : eng->hgp_work) {
       if (c.t64.empty()) continue;
       PQ_HIP(launch_hgp_class(kPairL[c.X][0], kPairL[c.X][1], kPairL[c.Y][0], kPairL[c.Y][1], eng->view(),
                               c.d_t64, (int)c.t64.size(), eng->d_sorted[c.Y], out_dev, st));

@@ -3,8 +3,9 @@
 // primitive quartet at a time).
 //
 // One lane computes one contracted shell quartet (ab|cd) of class
-// (LA LB|LC LD), taken from the same tiles as the Rys store kernels (one bra
-// pair, up to 64 kets):
+// (LA LB|LC LD), or part of its primitive sum, from tiles of one bra pair and
+// up to 64 kets (split over bra primitives and ket-primitive lanes as the J/K
+// tiles are; a partial block is added into the zeroed array with atomics):
 //   1. per primitive quartet, the Obara-Saika vertical recurrence builds
 //      [e0|f0]^(m) for |e| <= LA+LB, |f| <= LC+LD from the Boys values
 //      F_m(X), m <= L (chgp.c:244-419, vrr);
@@ -362,17 +363,21 @@ __global__ __launch_bounds__(64) void k_hgp_shell(EngineView ev, const Tile* __r
                                                   const int* __restrict__ kets, double* __restrict__ ints) {
   using H = HgpCls<LA, LB, LC, LD>;
   const Tile tl = tiles[blockIdx.x];
-  const int nk = tl.nket & 0xff;
-  if ((int)threadIdx.x >= nk) return;
-  const int bra = tl.bra, ket = kets[tl.ket0 + threadIdx.x];
+  const TileLane tn = tile_lane(tl.nket, threadIdx.x);
+  if (!tn.active) return;
+  const int bra = tl.bra, ket = kets[tl.ket0 + tn.slot];
   const PairInfo bp = ev.pairs[bra], kp = ev.pairs[ket];
   const int i0 = tl.prim & 0xffff, ni = tl.prim >> 16;
   const int i1 = ni ? i0 + ni : bp.pp_n;
+  const int k0 = tn.kc * kp.pp_n / tn.ks, k1 = (tn.kc + 1) * kp.pp_n / tn.ks;
+  // a lane holding part of the primitive sum adds its block into the zeroed
+  // array; a whole quartet is stored
+  const bool part = tl.prim != 0 || tn.ks > 1;
   double acc[H::NEA * H::NFC];
 #pragma unroll
   for (int c = 0; c < H::NEA * H::NFC; ++c) acc[c] = 0.0;
 #pragma unroll 1
-  for (int k = 0; k < kp.pp_n; ++k) {
+  for (int k = k0; k < k1; ++k) {
     const PrimPair q = ev.prims[kp.pp_off + k];
 #pragma unroll 1
     for (int i = i0; i < i1; ++i) {
@@ -417,8 +422,10 @@ __global__ __launch_bounds__(64) void k_hgp_shell(EngineView ev, const Tile* __r
           }
           v += tb.cb * ABp[0][tb.p0] * ABp[1][tb.p1] * ABp[2][tb.p2] * r;
         }
-        ints[ijkl2intindex(fi, fj, fk, fl)] =
-            v * cart_norm(LA, a) * cart_norm(LB, bb) * cart_norm(LC, c) * cart_norm(LD, d);
+        v *= cart_norm(LA, a) * cart_norm(LB, bb) * cart_norm(LC, c) * cart_norm(LD, d);
+        double* dst = ints + ijkl2intindex(fi, fj, fk, fl);
+        if (part) unsafeAtomicAdd(dst, v);
+        else *dst = v;
       }
     }
     return;
@@ -449,7 +456,9 @@ __global__ __launch_bounds__(64) void k_hgp_shell(EngineView ev, const Tile* __r
             });
           });
           constexpr double nf = cart_norm(LA, a) * cart_norm(LB, bb) * cart_norm(LC, c) * cart_norm(LD, d);
-          ints[ijkl2intindex(fi, fj, fk, fl)] = v * nf;
+          double* dst = ints + ijkl2intindex(fi, fj, fk, fl);
+          if (part) unsafeAtomicAdd(dst, v * nf);
+          else *dst = v * nf;
         });
       });
     });
