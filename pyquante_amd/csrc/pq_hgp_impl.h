@@ -358,6 +358,18 @@ __device__ __forceinline__ double hrr_ket(const double* row, const double (&CDp)
   return v;
 }
 
+// Output split of (dd|dd): blockIdx.y picks the bra function a, and each of
+// the 6 waves of a tile redoes the vertical recurrence and contraction but
+// runs the horizontal one (1,296 outputs, most of the lane's work with
+// single-primitive d shells) for its own a only.  Measured: (dd|dd) 2.4x
+// faster; the same split slowed (dp|dp), (dd|pp) and (dd|ds), whose vertical
+// recurrence over several primitives outweighs their horizontal one.
+template <int LA, int LB, int LC, int LD>
+constexpr int kHgpSplit =
+    (HgpCls<LA, LB, LC, LD>::SIZE > kHgpUnrollMax && ncart(LA) * ncart(LB) * ncart(LC) * ncart(LD) >= 1296)
+        ? ncart(LA)
+        : 1;
+
 template <int LA, int LB, int LC, int LD>
 __global__ __launch_bounds__(64) void k_hgp_shell(EngineView ev, const Tile* __restrict__ tiles,
                                                   const int* __restrict__ kets, double* __restrict__ ints) {
@@ -402,8 +414,10 @@ __global__ __launch_bounds__(64) void k_hgp_shell(EngineView ev, const Tile* __r
   if constexpr (H::SIZE > kHgpUnrollMax) {
     using S = HgpSteps<LA, LB, LC, LD>;
     const S& T = kHgpSteps<LA, LB, LC, LD>;
+    constexpr int NABS = S::NAB / kHgpSplit<LA, LB, LC, LD>;
+    const int ab0 = (int)blockIdx.y * NABS;
 #pragma unroll 1
-    for (int ab = 0; ab < S::NAB; ++ab) {
+    for (int ab = ab0; ab < ab0 + NABS; ++ab) {
       const int a = ab / H::NB, bb = ab % H::NB;
 #pragma unroll 1
       for (int cd = 0; cd < S::NCD; ++cd) {
@@ -469,8 +483,8 @@ template <int LA, int LB, int LC, int LD>
 hipError_t launch_hgp_one(const EngineView& ev, const Tile* tiles, int ntiles, const int* kets, double* ints,
                           hipStream_t st) {
   if (ntiles <= 0) return hipSuccess;
-  hipLaunchKernelGGL((k_hgp_shell<LA, LB, LC, LD>), dim3((unsigned)ntiles), dim3(64), 0, st, ev, tiles, kets,
-                     ints);
+  hipLaunchKernelGGL((k_hgp_shell<LA, LB, LC, LD>), dim3((unsigned)ntiles, kHgpSplit<LA, LB, LC, LD>), dim3(64), 0,
+                     st, ev, tiles, kets, ints);
   return hipGetLastError();
 }
 
