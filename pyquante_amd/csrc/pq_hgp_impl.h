@@ -49,12 +49,9 @@ __host__ __device__ constexpr int tri_index(int lx, int ly, int lz) {
   const int l = lx + ly + lz;
   return ncum(l - 1) + (l - lx) * (l - lx + 1) / 2 + (l - lx - ly);
 }
-// t with its power along d lowered (raised) by one
+// t with its power along d lowered by one
 __host__ __device__ constexpr int tri_lower(int t, int d) {
   return tri_index(tri_pow(t, 0) - (d == 0), tri_pow(t, 1) - (d == 1), tri_pow(t, 2) - (d == 2));
-}
-__host__ __device__ constexpr int tri_raise(int t, int d, int k) {
-  return tri_index(tri_pow(t, 0) + (d == 0) * k, tri_pow(t, 1) + (d == 1) * k, tri_pow(t, 2) + (d == 2) * k);
 }
 // the direction the recurrences step along: the first with a nonzero power
 __host__ __device__ constexpr int tri_dir(int t) { return tri_pow(t, 0) > 0 ? 0 : (tri_pow(t, 1) > 0 ? 1 : 2); }
