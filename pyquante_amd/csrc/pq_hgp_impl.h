@@ -367,11 +367,11 @@ constexpr int kHgpSplit =
         ? ncart(LA)
         : 1;
 
+// One tile (and output part `split` of kHgpSplit) of class (LA LB|LC LD).
 template <int LA, int LB, int LC, int LD>
-__global__ __launch_bounds__(64) void k_hgp_shell(EngineView ev, const Tile* __restrict__ tiles,
-                                                  const int* __restrict__ kets, double* __restrict__ ints) {
+__device__ __forceinline__ void hgp_tile(const EngineView& ev, const Tile& tl, const int* __restrict__ kets,
+                                         double* __restrict__ ints, int split) {
   using H = HgpCls<LA, LB, LC, LD>;
-  const Tile tl = tiles[blockIdx.x];
   const TileLane tn = tile_lane(tl.nket, threadIdx.x);
   if (!tn.active) return;
   const int bra = tl.bra, ket = kets[tl.ket0 + tn.slot];
@@ -412,7 +412,7 @@ __global__ __launch_bounds__(64) void k_hgp_shell(EngineView ev, const Tile* __r
     using S = HgpSteps<LA, LB, LC, LD>;
     const S& T = kHgpSteps<LA, LB, LC, LD>;
     constexpr int NABS = S::NAB / kHgpSplit<LA, LB, LC, LD>;
-    const int ab0 = (int)blockIdx.y * NABS;
+    const int ab0 = split * NABS;
 #pragma unroll 1
     for (int ab = ab0; ab < ab0 + NABS; ++ab) {
       const int a = ab / H::NB, bb = ab % H::NB;
@@ -474,6 +474,12 @@ __global__ __launch_bounds__(64) void k_hgp_shell(EngineView ev, const Tile* __r
       });
     });
   });
+}
+
+template <int LA, int LB, int LC, int LD>
+__global__ __launch_bounds__(64) void k_hgp_shell(EngineView ev, const Tile* __restrict__ tiles,
+                                                  const int* __restrict__ kets, double* __restrict__ ints) {
+  hgp_tile<LA, LB, LC, LD>(ev, tiles[blockIdx.x], kets, ints, (int)blockIdx.y);
 }
 
 template <int LA, int LB, int LC, int LD>
