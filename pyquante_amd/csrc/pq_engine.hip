@@ -1182,8 +1182,9 @@ int pq_get2ints_hgp(pq_engine* eng, double* out_dev, uint64_t len, int compat_nr
     // runtime allocates per hardware queue; spread over the side streams'
     // queues as well, it outgrew the runtime's scratch limit and the next
     // large-scratch dispatch failed (HSA_STATUS_ERROR_OUT_OF_RESOURCES)
-    // the classes with step-table kernels share one launch (heaviest class
-    // first, so its long-running waves start first); the others run alone
+    // all classes share one launch (heaviest class first, so that its
+    // long-running waves start first); the per-class launches remain for a
+    // list beyond the launch's segment table
     HgpMulti mm{};
     int64_t nb = 0;
     for (auto it = eng->hgp_work.rbegin(); it != eng->hgp_work.rend(); ++it) {
@@ -1192,7 +1193,7 @@ int pq_get2ints_hgp(pq_engine* eng, double* out_dev, uint64_t len, int compat_nr
       const int la = kPairL[c.X][0], lb = kPairL[c.X][1], lc = kPairL[c.Y][0], ld = kPairL[c.Y][1];
       int split = 1;
       const int64_t blocks = (int64_t)c.t64.size();
-      if (hgp_rolled_class(la, lb, lc, ld, &split) && mm.nseg < 16 && nb + blocks * split < (1ll << 31)) {
+      if (hgp_multi_class(la, lb, lc, ld, &split) && mm.nseg < kHgpMaxSeg && nb + blocks * split < (1ll << 31)) {
         mm.seg[mm.nseg++] = HgpSeg{c.d_t64, eng->d_sorted[c.Y], hgp_code(la, lb, lc, ld), split, (int)nb, 0};
         nb += blocks * split;
       } else {

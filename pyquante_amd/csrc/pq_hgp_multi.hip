@@ -1,24 +1,20 @@
-// pq_hgp_multi.hip -- the rolled shell-quartet HGP classes (more than
-// kHgpUnrollMax table entries: every class with a d shell beyond (ds|ss) and
-// (dp|ss), and (pp|ps), (pp|pp), (ds|ps)) in one launch.  Their grids are
-// small (144 to 4,000 waves, against 1,024 SIMDs) and their waves run long
-// serial recurrences, so one class at a time leaves most of the chip idle;
-// one launch runs them together on one queue (side streams would need a
-// private-memory backing per hardware queue).  Block b belongs to the
-// segment with the largest first block <= b; within it, block
-// (tile * split + part) runs output part `part` of tile `tile`.
+// pq_hgp_multi.hip -- every shell-quartet HGP class in one launch.  The
+// grids of single classes are small (144 to 15,000 waves, the d classes
+// below 1,024, against 1,024 SIMDs) and the waves of the step-table classes
+// run long serial recurrences, so one class at a time leaves most of the chip
+// idle; one launch runs them together on one queue (side streams would each
+// need their own private-memory backing).  Block b belongs to the segment
+// with the largest first block <= b; within it, block (tile * split + part)
+// runs output part `part` of tile `tile`.
 #include "pq_hgp_impl.h"
 
 namespace pq {
 
-#define PQ_HGP_ROLLED(X) \
+#define PQ_HGP_CLASSES(X) \
+  X(0, 0, 0, 0) X(1, 0, 0, 0) X(1, 0, 1, 0) X(1, 1, 0, 0) X(2, 0, 0, 0) X(2, 1, 0, 0) \
   X(1, 1, 1, 0) X(1, 1, 1, 1) X(2, 0, 1, 0) X(2, 0, 1, 1) X(2, 0, 2, 0) X(2, 1, 1, 0) X(2, 1, 1, 1) X(2, 1, 2, 0) \
   X(2, 1, 2, 1) X(2, 2, 0, 0) X(2, 2, 1, 0) X(2, 2, 1, 1) X(2, 2, 2, 0) X(2, 2, 2, 1) X(2, 2, 2, 2)
 
-#define PQ_CHECK_ROLLED(A, B, C_, D_) \
-  static_assert(HgpCls<A, B, C_, D_>::SIZE > kHgpUnrollMax, "unrolled class in the multi-class launch");
-PQ_HGP_ROLLED(PQ_CHECK_ROLLED)
-#undef PQ_CHECK_ROLLED
 
 __global__ __launch_bounds__(64) void k_hgp_multi(EngineView ev, HgpMulti m, double* __restrict__ ints) {
   const int b = (int)blockIdx.x;
@@ -32,20 +28,20 @@ __global__ __launch_bounds__(64) void k_hgp_multi(EngineView ev, HgpMulti m, dou
   case hgp_code(A, B, C_, D_):                                      \
     hgp_tile<A, B, C_, D_>(ev, tl, g.kets, ints, part);             \
     break;
-    PQ_HGP_ROLLED(PQ_HGP_CASE)
+    PQ_HGP_CLASSES(PQ_HGP_CASE)
 #undef PQ_HGP_CASE
     default:
       break;
   }
 }
 
-bool hgp_rolled_class(int la, int lb, int lc, int ld, int* split) {
+bool hgp_multi_class(int la, int lb, int lc, int ld, int* split) {
 #define PQ_HGP_Q(A, B, C_, D_)                             \
   if (la == A && lb == B && lc == C_ && ld == D_) {        \
     *split = kHgpSplit<A, B, C_, D_>;                      \
     return true;                                           \
   }
-  PQ_HGP_ROLLED(PQ_HGP_Q)
+  PQ_HGP_CLASSES(PQ_HGP_Q)
 #undef PQ_HGP_Q
   return false;
 }

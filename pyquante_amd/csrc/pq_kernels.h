@@ -32,21 +32,21 @@ hipError_t launch_eri_class(int la, int lb, int lc, int ld, int mode, int fp32, 
 // (the tiles and kets of MODE_STORE), writing the flat unique-ERI array.
 hipError_t launch_hgp_class(int la, int lb, int lc, int ld, const EngineView& ev, const Tile* tiles, int ntiles,
                             const int* kets, double* ints, hipStream_t st);
-// The classes whose HGP kernel walks step tables (hgp_rolled_class; split =
-// its output parts per tile) run in one launch: segment i covers blocks
-// [b0, next b0) of class code = hgp_code(la, lb, lc, ld), block
-// b0 + tile * split + part.
+// The HGP classes run in one launch (hgp_multi_class: the class is in it;
+// split = its output parts per tile): segment i covers blocks [b0, next b0)
+// of class code = hgp_code(la, lb, lc, ld), block b0 + tile * split + part.
 constexpr int hgp_code(int a, int b, int c, int d) { return ((a * 3 + b) * 3 + c) * 3 + d; }
 struct HgpSeg {
   const Tile* tiles;
   const int* kets;
   int code, split, b0, pad;
 };
+constexpr int kHgpMaxSeg = 24;
 struct HgpMulti {
-  HgpSeg seg[16];
+  HgpSeg seg[kHgpMaxSeg];
   int nseg;
 };
-bool hgp_rolled_class(int la, int lb, int lc, int ld, int* split);
+bool hgp_multi_class(int la, int lb, int lc, int ld, int* split);
 hipError_t launch_hgp_multi(const EngineView& ev, const HgpMulti& m, int nblocks, double* ints, hipStream_t st);
 
 // ket order of each pair class (ids[Y][j] = pair at position j) and the
