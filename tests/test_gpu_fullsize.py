@@ -8,6 +8,10 @@ that do not need the oracle at that size.
   with screening and primitive screening on (they depend on the pairs, not on
   D);
 - the two-density pass equals two single builds to 1e-11;
+- (H2O)_8: every one of the 202,015,050 unique ERIs by the shell-quartet HGP
+  kernels (primitive-split tiles adding partial blocks with atomics, all
+  classes in one launch) equals the Rys store array to 1e-12, and the J/K
+  digested from that array equals the direct Fock build to 1e-10;
 - (H2O)_16 6-31G** (400 functions): the mixed-precision energy functional
   tr(D(2J - K)) is within 1e-6 Eh of the FP64 one for a seeded density (the
   accuracy bound of the mixed path).
@@ -67,3 +71,25 @@ def test_h2o16_mixed_precision_energy_bound():
     e32 = float(torch.sum(D * (2.0 * J32 - K32)))
     print("(H2O)_16 mixed - FP64 energy functional: %.2e" % (e32 - e64), flush=True)
     assert abs(e32 - e64) < 1e-6
+
+
+def test_h2o8_hgp_full_array_matches_rys():
+    eng = _engine((2, 2, 2))
+    hgp = eng.get2ints("hgp", 0)
+    assert hgp.numel() == 202015050
+    assert bool(torch.isfinite(hgp).all())
+    rys = eng.get2ints("rys")
+    d = float((hgp - rys).abs().max())
+    del rys
+    print("(H2O)_8 HGP vs Rys, all unique ERIs: max |d| = %.2e" % d, flush=True)
+    assert d < 1e-12
+    # J/K digested from the HGP array against the direct build
+    eng.build(0.0, 0.0)
+    D = _density(eng.nbf, 20, 5)
+    from pyquante_amd.engine import digest_stored
+    Js, Ks = digest_stored(hgp, D)
+    del hgp
+    J, K = eng.fock_jk(D)
+    scale = float(J.abs().max())
+    assert float((Js - J).abs().max()) < 1e-10 * scale
+    assert float((Ks - K).abs().max()) < 1e-10 * scale
