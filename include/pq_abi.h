@@ -138,8 +138,10 @@ int pq_engine_set_prim_screen(pq_engine *eng, double rel);
 
 /* Number of engine-owned streams (0..4, default 4) the class kernels of one
  * Fock build or gradient are spread over; 0 launches everything on the caller's stream
- * (one kernel at a time, e.g. for per-kernel profiling).  No reference
- * counterpart (a launch knob). */
+ * (one kernel at a time, e.g. for per-kernel profiling).  A Fock build
+ * outside a graph capture uses at most 3 of them (with the caller's stream,
+ * four streams for the process's four hardware queues); a captured one uses
+ * all.  No reference counterpart (a launch knob). */
 int pq_engine_set_streams(pq_engine *eng, int nstreams);
 
 /* Bra runs: for the J/K kernels of classes with at most 108 Cartesian
