@@ -521,8 +521,8 @@ def main():
                    "function_quartets": fq, "unscreened_function_quartets": unscreened,
                    "s_per_fock_build": dt, "timing": "eager" if args.eager else "HIP graph replay",
                    "s_per_fock_build_eager": dt_eager, "bra_run_caps": args.tune, **extra},
-        "roofline": {"bound": "mfma", "compute": "FP64 vector ALU (Rys recurrences are not matrix products; "
-                                                 "78.6 TFLOP/s is also the MI355X FP64 vector peak)",
+        "roofline": {"bound": "valu", "compute": "FP64 vector ALU (Rys recurrences are not matrix products, so "
+                                                 "no MFMA; 78.6 TFLOP/s is the MI355X FP64 vector peak)",
                      "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved / FP64_PEAK_TFLOPS, "traffic": traffic,
                      "kernel": "%s, %s FP64 J/K" % (kname, lname), "launch_ms": ktime * 1e3,

@@ -69,6 +69,20 @@ int pq_contr_coulomb_batch(int n, const int *nprim_host, const int *poff_host, i
                            const double *exps_host, const double *coefs_host, const double *norms_host,
                            const double *origins_host, const int *powers_host, double *out_host);
 
+/* cints.contr_coulomb / coulomb_repulsion (Src/cints.c:80-153 contr_coulomb,
+ * coulomb_repulsion, B_array; wrappers cints.c:572-803): the
+ * Taketa-Huzinaga-O-ohata path, same batch layout as pq_contr_coulomb_batch.
+ * Each primitive quartet's Rys integrand is expanded in u = t^2 on the device
+ * and its coefficients are contracted with Boys values F_0..F_L (the sum the
+ * reference's THO expansion forms).  compat_nr = 1 takes those values from
+ * the reference's EPS = 3e-7 Fgamma (cints.c:340-411) and so reproduces its
+ * numbers; 0 uses the full-precision Boys function (then the value equals
+ * pq_contr_coulomb_batch's).  L_total <= 9 (PQ_E_LTOTAL beyond). */
+int pq_tho_contr_coulomb_batch(int n, const int *nprim_host, const int *poff_host, int nprim_total,
+                               const double *exps_host, const double *coefs_host, const double *norms_host,
+                               const double *origins_host, const int *powers_host, int compat_nr,
+                               double *out_host);
+
 /* Head-Gordon-Pople contracted ERIs, replacing chgp.contr_coulomb /
  * coulomb_repulsion (Src/chgp.c:78-169 contr_hrr/contr_vrr, 172-242 hrr,
  * 244-419 vrr) and, with mext, chgp.vrr (chgp.c:857-890).  Same batch layout
@@ -116,6 +130,13 @@ int pq_engine_layout(const pq_engine *eng, int *nmol, int64_t *msize, int *mol_n
 int pq_engine_destroy(pq_engine *eng);
 /* Number of basis functions. */
 int pq_engine_nbf(const pq_engine *eng, int *nbf);
+/* Work-list generation: changes whenever the engine replaces or frees device
+ * arrays that a Fock build reads (pair data, tiles, bra runs, scratch):
+ * pq_engine_set_geometry, pq_build_pairs, pq_engine_set_group_cap(s),
+ * pq_engine_autotune.  A HIP graph captured from pq_fock_jk holds those
+ * addresses and must not be replayed once the generation has changed.  No
+ * reference counterpart (graph-capture bookkeeping). */
+int pq_engine_generation(const pq_engine *eng, uint64_t *gen);
 /* Moves the nuclei (and the shells on them); invalidates pair data. */
 int pq_engine_set_geometry(pq_engine *eng, const double *atom_xyz_host);
 
@@ -251,8 +272,12 @@ int pq_digest_stored(int nbf, const double *ints_dev, uint64_t len, const double
 int pq_unpack_eri(int nbf, const double *ints_dev, uint64_t len, int mu0, int nmu, double *out_dev, void *stream);
 
 /* Ints.get1ints (PyQuante/Ints.py:73-87): S, T and V = sum_A Z_A V_A
- * (h = T + V), nbf x nbf row-major FP64, on the device. */
-int pq_get1ints(pq_engine *eng, double *S_dev, double *T_dev, double *V_dev, void *stream);
+ * (h = T + V), nbf x nbf row-major FP64, on the device.  compat_nr = 1
+ * evaluates V with the reference's EPS = 3e-7 Boys function, as its
+ * cints.nuclear_attraction does (Src/cints.c:241-275, 340-411; reached through
+ * CGBF.py:108-114 and PGBF.py:90-96), and reproduces its values; 0 uses
+ * full-precision Boys values (the V whose derivative pq_forces contracts). */
+int pq_get1ints(pq_engine *eng, double *S_dev, double *T_dev, double *V_dev, int compat_nr, void *stream);
 
 /* RHF nuclear gradient dE/dR (force.py:31-70 rhf_force; der2JmK
  * force.py:318-357; AnalyticDerivatives.py:27-675), natom x 3 row-major into

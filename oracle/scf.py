@@ -11,9 +11,12 @@ def geigh(H, S):
     return e, X @ v
 
 
-def rhf(bfs, mol, ConvCriteria=1e-8, MaxIter=50, ints=None, S=None, h=None):
+def rhf(bfs, mol, ConvCriteria=1e-8, MaxIter=50, ints=None, S=None, h=None, compat_nr=1):
+    """compat_nr = 1 (default): V with the reference's EPS = 3e-7 Boys
+    function, as its getints computes it (Ints.py:73-87 over cints.c:241-275);
+    0: V with the accurate Boys function (the V of the gradient kernels)."""
     if S is None:
-        S, h = O.get1ints(bfs, mol.atoms)
+        S, h = O.get1ints(bfs, mol.atoms, compat_nr=compat_nr)
     if ints is None:
         ints = O.get2ints(bfs)
     nocc = mol.get_closedopen()[0]

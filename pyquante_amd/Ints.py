@@ -30,6 +30,15 @@ schwarz_cut = 1e-12
 method = "rys"
 # the HGP backend keeps chgp's Boys tolerance (EPS = 3e-7) and so its values
 hgp_compat_boys = True
+# V (get1ints, getV, getints) with the Boys function of the reference's
+# cints.nuclear_attraction (EPS = 3e-7, cints.c:241-275,340-411), so that S, h
+# and the SCF energies built on them are the reference's; False evaluates V
+# with full-precision Boys values
+reference_boys = True
+
+
+def _v_compat():
+    return 1 if reference_boys else 0
 
 
 class _Nucleus(object):
@@ -121,7 +130,7 @@ def _as_out(x, like):
 def get1ints(bfs, atoms):
     """S and h = T + sum_A Z_A V_A (Ints.py:73-87), computed on the GPU."""
     eng = Engine(bfs, atoms)
-    S, T, V = eng.get1ints()
+    S, T, V = eng.get1ints(_v_compat())
     return S.cpu().numpy(), (T + V).cpu().numpy()
 
 
@@ -137,7 +146,7 @@ def getT(bfs):
 
 def getV(bfs, atoms):
     eng = Engine(bfs, atoms)
-    return eng.get1ints()[2].cpu().numpy()
+    return eng.get1ints(_v_compat())[2].cpu().numpy()
 
 
 def get2ints(bfs, atoms=None, method=None):
@@ -148,7 +157,7 @@ def get2ints(bfs, atoms=None, method=None):
 def getints(bfs, atoms):
     """Ints.getints (Ints.py:66-71): (S, h, Ints) sharing one GPU engine."""
     eng = Engine(bfs, atoms)
-    S, T, V = eng.get1ints()
+    S, T, V = eng.get1ints(_v_compat())
     return S.cpu().numpy(), (T + V).cpu().numpy(), Integrals(bfs, atoms, engine=eng)
 
 

@@ -52,12 +52,14 @@ _proto("pq_ijkl2intindex", [_I64, _I64, _I64, _I64, _PU64])
 _proto("pq_boys_batch", [_I, _PD, _PD, _I, _PD])
 _proto("pq_rys_roots_batch", [_I, _I, _PD, _PD, _PD])
 _proto("pq_contr_coulomb_batch", [_I, _PI, _PI, _I, _PD, _PD, _PD, _PD, _PI, _PD])
+_proto("pq_tho_contr_coulomb_batch", [_I, _PI, _PI, _I, _PD, _PD, _PD, _PD, _PI, _I, _PD])
 _proto("pq_hgp_contr_coulomb_batch", [_I, _PI, _PI, _I, _PD, _PD, _PD, _PD, _PI, _PI, _I, _PD])
 _proto("pq_engine_create", [_I, _PI, _PI, _PI, _PI, _PD, _PD, _I, _PD, _PD, _c.POINTER(_VP)])
 _proto("pq_engine_create_batch", [_I, _PI, _PI, _PI, _PI, _PD, _PD, _I, _PD, _PD, _PI, _c.POINTER(_VP)])
 _proto("pq_engine_layout", [_VP, _PI, _c.POINTER(_I64), _PI, _c.POINTER(_I64), _PI, _PI])
 _proto("pq_engine_destroy", [_VP])
 _proto("pq_engine_nbf", [_VP, _PI])
+_proto("pq_engine_generation", [_VP, _PU64])
 _proto("pq_engine_set_geometry", [_VP, _PD])
 _proto("pq_build_pairs", [_VP, _D, _D])
 _proto("pq_engine_set_streams", [_VP, _I])
@@ -75,7 +77,7 @@ _proto("pq_engine_class_stats", [_VP, _I, _PI, _PU64, _PD])
 _proto("pq_fock_jk_timed", [_VP, _VP, _VP, _VP, _I, _VP, _c.POINTER(_c.c_float)])
 _proto("pq_digest_stored", [_I, _VP, _U64, _VP, _VP, _VP, _VP])
 _proto("pq_unpack_eri", [_I, _VP, _U64, _I, _I, _VP, _VP])
-_proto("pq_get1ints", [_VP, _VP, _VP, _VP, _VP])
+_proto("pq_get1ints", [_VP, _VP, _VP, _VP, _I, _VP])
 _proto("pq_forces", [_VP, _VP, _VP, _VP, _VP])
 
 

@@ -28,16 +28,18 @@ def _seq(x, n=None):
     return v
 
 
-RYS, HGP = 1, 2
+THO, RYS, HGP = 0, 1, 2
 
 
 def contr_coulomb_batch(quartets, method=RYS, compat_nr=1, mext=None):
     """quartets: list of 4-tuples of (exps, coefs, norms, origin, powers).
 
-    method RYS evaluates with the Rys engine (full precision); HGP with the
-    Head-Gordon-Pople kernel, whose compat_nr = 1 reproduces chgp.c's values
-    (its EPS = 3e-7 Boys function).  mext: per-quartet auxiliary order for
-    primitive VRR leaves (HGP only)."""
+    method RYS evaluates with the Rys engine (full precision); THO contracts
+    the integrand's coefficients in u with Boys values, and with compat_nr = 1
+    (the EPS = 3e-7 Boys function of cints.c) reproduces cints.c's values; HGP
+    runs the Head-Gordon-Pople kernel, whose compat_nr = 1 reproduces chgp.c's
+    values.  mext: per-quartet auxiliary order for primitive VRR leaves (HGP
+    only)."""
     _abi.require_gpu()
     nprim, poff, exps, coefs, norms, org, pw = [], [], [], [], [], [], []
     for q in quartets:
@@ -69,6 +71,9 @@ def contr_coulomb_batch(quartets, method=RYS, compat_nr=1, mext=None):
             check(lib.pq_hgp_contr_coulomb_batch(n, pi(a[0]), pi(a[1]), len(exps), pd(a[2]), pd(a[3]), pd(a[4]),
                                                  pd(a[5]), pi(a[6]), None if m is None else pi(m), int(compat_nr),
                                                  pd(out)))
+        elif method == THO:
+            check(lib.pq_tho_contr_coulomb_batch(n, pi(a[0]), pi(a[1]), len(exps), pd(a[2]), pd(a[3]), pd(a[4]),
+                                                 pd(a[5]), pi(a[6]), int(compat_nr), pd(out)))
         else:
             check(lib.pq_contr_coulomb_batch(n, pi(a[0]), pi(a[1]), len(exps), pd(a[2]), pd(a[3]), pd(a[4]),
                                              pd(a[5]), pi(a[6]), pd(out)))
@@ -79,21 +84,21 @@ def contr_coulomb_batch(quartets, method=RYS, compat_nr=1, mext=None):
     return out
 
 
-def contr_coulomb(*args, method=RYS):
+def contr_coulomb(*args, method=RYS, compat_nr=1):
     if len(args) != 20:
         raise SystemError("error return without exception set")
     q = tuple(tuple(args[5 * f:5 * f + 5]) for f in range(4))
-    return float(contr_coulomb_batch([q], method)[0])
+    return float(contr_coulomb_batch([q], method, compat_nr)[0])
 
 
-def coulomb_repulsion(*args, method=RYS):
+def coulomb_repulsion(*args, method=RYS, compat_nr=1):
     if len(args) != 16:
         raise SystemError("error return without exception set")
     q = []
     for f in range(4):
         o, nrm, p, a = args[4 * f:4 * f + 4]
         q.append(([float(a)], [1.0], [float(nrm)], o, p))
-    return float(contr_coulomb_batch([tuple(q)], method)[0])
+    return float(contr_coulomb_batch([tuple(q)], method, compat_nr)[0])
 
 
 def hgp_vrr(*args):

@@ -154,6 +154,9 @@ struct pq_engine {
   // tiles per bra run for each (bra class, ket class, precision): -1 until
   // set by pq_engine_autotune or pq_engine_set_group_cap
   int group_cap[6][6][2];
+  // bumped whenever device arrays that a captured build reads (pair data,
+  // tiles, bra runs, the ket J) are replaced or freed
+  uint64_t gen = 1;
   pq_engine() {
     for (auto& a : group_cap)
       for (auto& b : a) b[0] = b[1] = -1;
@@ -215,6 +218,7 @@ struct pq_engine {
     w.clear();
   }
   void free_pairs() {
+    ++gen;
     dev_free(d_pairs);
     dev_free(d_prims);
     dev_free(d_prim_a);
@@ -323,6 +327,7 @@ int sort_pairs(pq_engine* e, const std::vector<PairInfo>& P, std::vector<std::ve
 // largest one is below e->prim_rel * cut (cut = 0 or prim_rel = 0 keeps every
 // primitive); pp_n = the primitives that pass, at least one.
 int build_screened(pq_engine* e, double cut) {
+  ++e->gen;
   const double tau = cut > 0.0 ? cut * e->prim_rel : 0.0;
   e->pairs_s = e->pairs;
   if (tau > 0.0)
@@ -518,6 +523,7 @@ void rotate_tiles(std::vector<Tile>& t, size_t first, size_t i) {
 // Work lists over the pair copy P (its pp_n) and its per-class order S.
 int build_work(pq_engine* e, std::vector<ClassWork>& out, double cut, double fp32cut, bool split_prims,
                const std::vector<PairInfo>& P, const std::vector<std::vector<int>>& S) {
+  if (&out == &e->work) ++e->gen;  // the J/K lists (the stored-integral lists are not captured)
   for (auto& c : out) {
     dev_free(c.d_t64);
     dev_free(c.d_t32);
@@ -702,9 +708,15 @@ int pq_rys_roots_batch(int n, int nr, const double* X_host, double* u_host, doub
   return PQ_OK;
 }
 
-int pq_contr_coulomb_batch(int n, const int* nprim_host, const int* poff_host, int nprim_total,
-                           const double* exps_host, const double* coefs_host, const double* norms_host,
-                           const double* origins_host, const int* powers_host, double* out_host) {
+}  // extern "C"
+
+namespace {
+// Checks and uploads one batch of contracted quartets (layout of
+// pq_contr_coulomb_batch) and runs `launch` on it: the Rys and THO batches.
+template <typename Launch>
+int contr_batch(int n, const int* nprim_host, const int* poff_host, int nprim_total, const double* exps_host,
+                const double* coefs_host, const double* norms_host, const double* origins_host,
+                const int* powers_host, double* out_host, Launch launch) {
   if (n < 0 || nprim_total < 0) return fail(PQ_E_ARG, "negative size");
   if (n == 0) return PQ_OK;
   if (!nprim_host || !poff_host || !exps_host || !coefs_host || !norms_host || !origins_host || !powers_host ||
@@ -739,10 +751,35 @@ int pq_contr_coulomb_batch(int n, const int* nprim_host, const int* poff_host, i
   PQ_HIP(hipMemcpy(b_n.p, norms_host, nprim_total * sizeof(double), hipMemcpyHostToDevice));
   PQ_HIP(hipMemcpy(b_o.p, origins_host, 12 * n * sizeof(double), hipMemcpyHostToDevice));
   PQ_HIP(hipMemcpy(b_p.p, powers_host, 12 * n * sizeof(int), hipMemcpyHostToDevice));
-  PQ_HIP(launch_contr_eri(n, (int*)b_np.p, (int*)b_po.p, (double*)b_e.p, (double*)b_c.p, (double*)b_n.p,
-                          (double*)b_o.p, (int*)b_p.p, (double*)b_out.p, 0));
+  PQ_HIP(launch(n, (int*)b_np.p, (int*)b_po.p, (double*)b_e.p, (double*)b_c.p, (double*)b_n.p, (double*)b_o.p,
+                (int*)b_p.p, (double*)b_out.p));
   PQ_HIP(hipMemcpy(out_host, b_out.p, n * sizeof(double), hipMemcpyDeviceToHost));
   return PQ_OK;
+}
+}  // namespace
+
+extern "C" {
+
+int pq_contr_coulomb_batch(int n, const int* nprim_host, const int* poff_host, int nprim_total,
+                           const double* exps_host, const double* coefs_host, const double* norms_host,
+                           const double* origins_host, const int* powers_host, double* out_host) {
+  return contr_batch(n, nprim_host, poff_host, nprim_total, exps_host, coefs_host, norms_host, origins_host,
+                     powers_host, out_host,
+                     [](int n_, const int* np, const int* po, const double* e, const double* c, const double* nr,
+                        const double* o, const int* pw, double* out) {
+                       return launch_contr_eri(n_, np, po, e, c, nr, o, pw, out, 0);
+                     });
+}
+
+int pq_tho_contr_coulomb_batch(int n, const int* nprim_host, const int* poff_host, int nprim_total,
+                               const double* exps_host, const double* coefs_host, const double* norms_host,
+                               const double* origins_host, const int* powers_host, int compat_nr, double* out_host) {
+  return contr_batch(n, nprim_host, poff_host, nprim_total, exps_host, coefs_host, norms_host, origins_host,
+                     powers_host, out_host,
+                     [compat_nr](int n_, const int* np, const int* po, const double* e, const double* c,
+                                 const double* nr, const double* o, const int* pw, double* out) {
+                       return launch_tho_contr(n_, np, po, e, c, nr, o, pw, compat_nr, out, 0);
+                     });
 }
 
 int pq_hgp_contr_coulomb_batch(int n, const int* nprim_host, const int* poff_host, int nprim_total,
@@ -934,6 +971,12 @@ int pq_engine_layout(const pq_engine* eng, int* nmol, int64_t* msize, int* mol_n
   return PQ_OK;
 }
 
+int pq_engine_generation(const pq_engine* eng, uint64_t* gen) {
+  if (!eng || !gen) return fail(PQ_E_ARG, "null argument");
+  *gen = eng->gen;
+  return PQ_OK;
+}
+
 int pq_engine_nbf(const pq_engine* eng, int* nbf) {
   if (!eng || !nbf) return fail(PQ_E_ARG, "null argument");
   *nbf = eng->nbf;
@@ -991,6 +1034,7 @@ static bool valid_cap(int cap) {
 int pq_engine_set_group_cap(pq_engine* eng, int cap) {
   if (!eng) return fail(PQ_E_ARG, "null engine");
   if (!valid_cap(cap)) return fail(PQ_E_ARG, "cap must be -1, 0 or 1..1024 (optionally | PQ_CAP_STAGE_D)");
+  ++eng->gen;
   for (auto& a : eng->group_cap)
     for (auto& b : a) b[0] = b[1] = cap;
   if (eng->work_ok)
@@ -1006,6 +1050,7 @@ int pq_engine_set_group_caps(pq_engine* eng, const int* caps) {
   if (!eng || !caps) return fail(PQ_E_ARG, "null argument");
   for (int i = 0; i < 42; ++i)
     if (!valid_cap(caps[i])) return fail(PQ_E_ARG, "caps must be -1, 0 or 1..1024 (optionally | PQ_CAP_STAGE_D)");
+  ++eng->gen;
   int ci = 0;
   for (int X = 0; X < 6; ++X)
     for (int Y = 0; Y <= X; ++Y, ++ci)
@@ -1027,6 +1072,7 @@ int pq_engine_autotune(pq_engine* eng, const double* D_dev, int reps, void* stre
   static const int kCand[] = {0,  4,  8,  16,  32, 4 | PQ_CAP_STAGE_D, 8 | PQ_CAP_STAGE_D, 16 | PQ_CAP_STAGE_D,
                               32 | PQ_CAP_STAGE_D};
   hipStream_t st = (hipStream_t)stream;
+  ++eng->gen;
   hipEvent_t e0 = nullptr, e1 = nullptr;
   PQ_HIP(hipEventCreate(&e0));
   PQ_HIP(hipEventCreate(&e1));
@@ -1329,7 +1375,10 @@ static int fock_jk_impl(pq_engine* eng, int nd, const double* const* Ds, double*
     // two densities: stacked copies, digested by the MODE_JK2 kernels
     const double* D_dev = Ds[0];
     if (nd == 2) {
-      if (!eng->d_D2) PQ_TRY(dev_alloc(&eng->d_D2, 2 * (size_t)nn));
+      if (!eng->d_D2) {
+        PQ_TRY(dev_alloc(&eng->d_D2, 2 * (size_t)nn));
+        ++eng->gen;
+      }
       PQ_TRY(hipMemcpyAsync(eng->d_D2, Ds[0], nn * sizeof(double), hipMemcpyDeviceToDevice, st));
       PQ_TRY(hipMemcpyAsync(eng->d_D2 + nn, Ds[1], nn * sizeof(double), hipMemcpyDeviceToDevice, st));
       D_dev = eng->d_D2;
@@ -1510,7 +1559,7 @@ int pq_forces(pq_engine* eng, const double* D_dev, const double* W_dev, double* 
   return PQ_OK;
 }
 
-int pq_get1ints(pq_engine* eng, double* S_dev, double* T_dev, double* V_dev, void* stream) {
+int pq_get1ints(pq_engine* eng, double* S_dev, double* T_dev, double* V_dev, int compat_nr, void* stream) {
   if (!eng || !S_dev || !T_dev || !V_dev) return fail(PQ_E_ARG, "null argument");
   if (!eng->pairs_ok) {
     int rc = build_pair_data(eng);
@@ -1520,7 +1569,7 @@ int pq_get1ints(pq_engine* eng, double* S_dev, double* T_dev, double* V_dev, voi
   for (int c = 0; c < 6; ++c) {
     if (eng->members[c].empty()) continue;
     PQ_HIP(launch_onee(kPairL[c][0], kPairL[c][1], eng->view(), eng->d_members[c], (int)eng->members[c].size(),
-                       eng->d_atom_xyz, eng->d_atom_z, eng->natom, S_dev, T_dev, V_dev, st));
+                       eng->d_atom_xyz, eng->d_atom_z, eng->natom, S_dev, T_dev, V_dev, compat_nr ? 1 : 0, st));
   }
   return PQ_OK;
 }

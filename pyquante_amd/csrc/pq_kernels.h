@@ -88,10 +88,11 @@ hipError_t launch_digest_stored(int n, const double* ints, uint64_t len, const d
 
 hipError_t launch_fill(double* p, int64_t n, double v, hipStream_t st);
 
-// one-electron S, T, V (full matrices, row-major n x n)
+// one-electron S, T, V (full matrices, row-major n x n); compat: V with the
+// reference's EPS = 3e-7 Boys function (its values)
 hipError_t launch_onee(int la, int lb, const EngineView& ev, const int* pair_ids, int npairs,
                        const double* atom_xyz, const double* atom_z, int natom, double* S, double* T,
-                       double* V, hipStream_t st);
+                       double* V, int compat, hipStream_t st);
 
 // scalar-compatibility batches
 // HGP contracted ERIs per function quartet (batch layout of launch_contr_eri);
@@ -113,5 +114,11 @@ hipError_t launch_rys_roots(int n, int nr, const double* X, double* u, double* w
 hipError_t launch_contr_eri(int n, const int* nprim, const int* poff, const double* exps,
                             const double* coefs, const double* norms, const double* origins,
                             const int* powers, double* out, hipStream_t st);
+// the same batch by the coefficients of the integrand in u contracted with
+// Boys values (compat_nr = 1: the reference's EPS = 3e-7 Boys function, so
+// the values of its THO cints path); L_total <= 9
+hipError_t launch_tho_contr(int n, const int* nprim, const int* poff, const double* exps, const double* coefs,
+                            const double* norms, const double* origins, const int* powers, int compat_nr,
+                            double* out, hipStream_t st);
 
 }  // namespace pq

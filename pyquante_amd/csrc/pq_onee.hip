@@ -10,6 +10,98 @@
 namespace pq {
 
 // ---------------------------------------------------------------------
+// Runtime-angular-momentum 1-D Rys integrals.  The Rys integrand of a
+// primitive ERI (or nuclear-attraction integral) is, after the Gaussian
+// transform, prod_d I_d(u) e^(-X u) integrated over t in [0, 1], u = t^2,
+// and every I_d is a polynomial in u of degree <= its total power.  The
+// same recurrence therefore gives either I_d at one Rys root (S = double)
+// or the coefficients of I_d in u (S = UPoly); contracting the product's
+// coefficients c_k with Boys values F_k(X) = int_0^1 t^(2k) e^(-X t^2) dt
+// gives sum_k c_k F_k, the form in which the reference's THO expressions
+// (cints.c:110-187 ERIs, 241-307 nuclear attraction) consume their Boys
+// function.  With the reference's EPS = 3e-7 Boys values that reproduces
+// its numbers to rounding.
+// ---------------------------------------------------------------------
+constexpr int kRtMaxL = 9;   // largest total angular momentum of these paths
+constexpr int kRtMaxG = 30;  // (N+1)(M+1) for N + M <= kRtMaxL
+
+struct UPoly {
+  double c[kRtMaxL + 1];
+};
+__device__ __forceinline__ UPoly up_lin(double a, double b) {
+  UPoly r;
+#pragma unroll
+  for (int k = 0; k <= kRtMaxL; ++k) r.c[k] = 0.0;
+  r.c[0] = a;
+  r.c[1] = b;
+  return r;
+}
+__device__ __forceinline__ UPoly operator+(const UPoly& a, const UPoly& b) {
+  UPoly r;
+#pragma unroll
+  for (int k = 0; k <= kRtMaxL; ++k) r.c[k] = a.c[k] + b.c[k];
+  return r;
+}
+__device__ __forceinline__ UPoly operator*(double s, const UPoly& a) {
+  UPoly r;
+#pragma unroll
+  for (int k = 0; k <= kRtMaxL; ++k) r.c[k] = s * a.c[k];
+  return r;
+}
+// product truncated at degree kRtMaxL (the callers never exceed it)
+__device__ __forceinline__ UPoly operator*(const UPoly& a, const UPoly& b) {
+  UPoly r;
+#pragma unroll
+  for (int k = 0; k <= kRtMaxL; ++k) {
+    double s = 0.0;
+#pragma unroll
+    for (int i = 0; i <= k; ++i) s += a.c[i] * b.c[k - i];
+    r.c[k] = s;
+  }
+  return r;
+}
+template <typename S>
+__device__ __forceinline__ S rt_one();
+template <>
+__device__ __forceinline__ double rt_one<double>() { return 1.0; }
+template <>
+__device__ __forceinline__ UPoly rt_one<UPoly>() { return up_lin(1.0, 0.0); }
+
+// I(i, j, k, l) of one Cartesian direction, the runtime form of rys_1d:
+// G(n, m) = I(n, 0, m, 0) for n <= i + j, m <= k + l by the same vertical
+// recurrence, then the horizontal transfers I(a, b+1) = I(a+1, b) + AB I(a, b)
+// on the bra (in place, rows of G) and likewise on the ket (row i).  Needs
+// i + j + k + l <= kRtMaxL.
+template <typename S>
+__device__ S rys_1d_rt(int i, int j, int k, int l, const S& C00, const S& D00, const S& B10, const S& B01,
+                       const S& B00, double AB, double CD) {
+  const int N = i + j, M = k + l, W = M + 1;
+  S G[kRtMaxG];
+  G[0] = rt_one<S>();
+  if (N > 0) G[W] = C00;
+  for (int a = 2; a <= N; ++a) G[a * W] = (double)(a - 1) * (B10 * G[(a - 2) * W]) + C00 * G[(a - 1) * W];
+  if (M > 0) {
+    G[1] = D00;
+    for (int b = 2; b <= M; ++b) G[b] = (double)(b - 1) * (B01 * G[b - 2]) + D00 * G[b - 1];
+    for (int a = 1; a <= N; ++a) {
+      G[a * W + 1] = (double)a * (B00 * G[(a - 1) * W]) + D00 * G[a * W];
+      for (int b = 2; b <= M; ++b)
+        G[a * W + b] = (double)(b - 1) * (B01 * G[a * W + b - 2]) + (double)a * (B00 * G[(a - 1) * W + b - 1]) +
+                       D00 * G[a * W + b - 1];
+    }
+  }
+  for (int b = 1; b <= j; ++b)
+    for (int a = 0; a <= N - b; ++a)
+      for (int m = 0; m <= M; ++m) G[a * W + m] = G[(a + 1) * W + m] + AB * G[a * W + m];
+  S* R = G + i * W;
+  for (int d = 1; d <= l; ++d)
+    for (int c = 0; c <= M - d; ++c) R[c] = R[c + 1] + CD * R[c];
+  return R[k];
+}
+
+__device__ double fgamma_dev(double m, double x, double eps, int itmax);
+
+// ---------------------------------------------------------------------
 // Fill (graph-capturable replacement for hipMemsetAsync)
 // ---------------------------------------------------------------------
 __global__ void k_fill(double* p, int64_t n, double v) {
@@ -50,13 +142,15 @@ hipError_t launch_fill(double* p, int64_t n, double v, hipStream_t st) {
 //   recurrence; T_ab by the reference's expression in shifted overlaps
 //   (cints.c:188-212); V_ab = sum_C Z_C <a|-1/r_C|b> (cints.c:241-275) by Rys
 //   quadrature with X = p |P - C|^2.  Full matrices are written (both
-//   triangles).
+//   triangles).  compat: V from the coefficients of its integrand in u
+//   contracted with the reference's EPS = 3e-7 Boys values F_0..F_{LA+LB}
+//   (the Fgamma calls of cints.c:241-275), which reproduces its values.
 // ---------------------------------------------------------------------
 template <int LA, int LB>
 __global__ __launch_bounds__(256) void k_onee(EngineView ev, const int* __restrict__ pair_ids, int npairs,
                                               const double* __restrict__ atom_xyz,
                                               const double* __restrict__ atom_z, int natom, double* S,
-                                              double* T, double* V) {
+                                              double* T, double* V, int compat) {
   constexpr int NA = ncart(LA), NB = ncart(LB);
   constexpr int NRV = (LA + LB) / 2 + 1;
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
@@ -118,9 +212,39 @@ __global__ __launch_bounds__(256) void k_onee(EngineView ev, const int* __restri
       const double Cc[3] = {atom_xyz[3 * c], atom_xyz[3 * c + 1], atom_xyz[3 * c + 2]};
       const double PC[3] = {P[0] - Cc[0], P[1] - Cc[1], P[2] - Cc[2]};
       const double X = p * (PC[0] * PC[0] + PC[1] * PC[1] + PC[2] * PC[2]);
+      const double zc = atom_z[c] * vpre;
+      if (compat) {
+        constexpr int L = LA + LB;
+        double F[L + 1];
+#pragma unroll
+        for (int k = 0; k <= L; ++k) F[k] = fgamma_dev((double)k, X, 3.0e-7, 100);
+        // I_d(u) for every (ia, ib): C00 = PA - u PC, B10 = (1 - u) / (2p)
+        UPoly I[3][LA + 1][LB + 1];
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+          const UPoly C00 = up_lin(P[d] - A[d], -PC[d]), B10 = up_lin(i2p, -i2p);
+#pragma unroll
+          for (int ia = 0; ia <= LA; ++ia)
+#pragma unroll
+            for (int ib = 0; ib <= LB; ++ib)
+              I[d][ia][ib] = rys_1d_rt<UPoly>(ia, ib, 0, 0, C00, C00, B10, B10, B10, A[d] - B[d], 0.0);
+        }
+#pragma unroll
+        for (int a = 0; a < NA; ++a)
+#pragma unroll
+          for (int b = 0; b < NB; ++b) {
+            const UPoly pr = I[0][cart_pow(LA, a, 0)][cart_pow(LB, b, 0)] *
+                             I[1][cart_pow(LA, a, 1)][cart_pow(LB, b, 1)] *
+                             I[2][cart_pow(LA, a, 2)][cart_pow(LB, b, 2)];
+            double s = 0.0;
+#pragma unroll
+            for (int k = 0; k <= L; ++k) s += pr.c[k] * F[k];
+            v[a][b] += zc * s;
+          }
+        continue;
+      }
       double u[NRV], w[NRV];
       rys_roots<NRV, double>(X, u, w);
-      const double zc = atom_z[c] * vpre;
 #pragma unroll
       for (int r = 0; r < NRV; ++r) {
         const double B10 = i2p * (1.0 - u[r]);
@@ -175,13 +299,13 @@ __global__ __launch_bounds__(256) void k_onee(EngineView ev, const int* __restri
 
 hipError_t launch_onee(int la, int lb, const EngineView& ev, const int* pair_ids, int npairs,
                        const double* atom_xyz, const double* atom_z, int natom, double* S, double* T,
-                       double* V, hipStream_t st) {
+                       double* V, int compat, hipStream_t st) {
   if (npairs <= 0) return hipSuccess;
   dim3 g((npairs + 127) / 128), b(128);
 #define PQ_OCASE(A, B_)                                                                                 \
   if (la == A && lb == B_) {                                                                            \
     hipLaunchKernelGGL((k_onee<A, B_>), g, b, 0, st, ev, pair_ids, npairs, atom_xyz, atom_z, natom, S, T, \
-                       V);                                                                              \
+                       V, compat);                                                                      \
     return hipGetLastError();                                                                           \
   }
   PQ_OCASE(0, 0)
@@ -517,45 +641,76 @@ hipError_t launch_rys_roots(int n, int nr, const double* X, double* u, double* w
 }
 
 // ---------------------------------------------------------------------
-// Contracted ERI of four arbitrary Cartesian functions (crys.c:34-97
-// semantics): the batch behind crys/cints/chgp contr_coulomb and
-// coulomb_repulsion.  Runtime angular momentum, L_total <= 9.
+// Contracted ERI of four arbitrary Cartesian functions: the batch behind the
+// drop-in contr_coulomb / coulomb_repulsion.  Runtime angular momentum,
+// L_total <= kRtMaxL.  Function f of quartet t has nprim[4t+f] primitives
+// from poff[4t+f] in exps/coefs/norms, origin org[12t+3f..], powers pw[..].
+//   k_contr_eri: Rys quadrature with the engine's roots (crys.c:34-97
+//     semantics, one rys_1d_rt per root and direction);
+//   k_tho_contr: the integrand's coefficients in u contracted with Boys
+//     values F_0..F_L of eps (3e-7: the reference's Fgamma, so the values of
+//     cints.c:80-153; 1e-17: full precision).
 // ---------------------------------------------------------------------
-__device__ double rys_1d_rt(double u, int i, int j, int k, int l, double xi, double xj, double xk, double xl,
-                            double ai, double aj, double ak, double al) {
-  double G[10][10];
-  const int n = i + j, m = k + l;
-  const double A = ai + aj, B = ak + al;
-  const double Px = (ai * xi + aj * xj) / A, Qx = (ak * xk + al * xl) / B;
-  const double fff = u / (A + B);
-  const double B00 = 0.5 * fff, B1 = (1 - B * fff) / (2 * A), B1p = (1 - A * fff) / (2 * B);
-  const double C = (Px - xi) + B * (Qx - Px) * fff, Cp = (Qx - xk) + A * (Px - Qx) * fff;
-  G[0][0] = 1.0;
-  if (n > 0) G[1][0] = C;
-  if (m > 0) G[0][1] = Cp;
-  for (int a = 2; a <= n; a++) G[a][0] = B1 * (a - 1) * G[a - 2][0] + C * G[a - 1][0];
-  for (int b = 2; b <= m; b++) G[0][b] = B1p * (b - 1) * G[0][b - 2] + Cp * G[0][b - 1];
-  if (m > 0 && n > 0)
-    for (int a = 1; a <= n; a++) {
-      G[a][1] = a * B00 * G[a - 1][0] + Cp * G[a][0];
-      for (int b = 2; b <= m; b++) G[a][b] = B1p * (b - 1) * G[a][b - 2] + a * B00 * G[a - 1][b - 1] + Cp * G[a][b - 1];
+struct ContrQuartet {
+  int np[4], po[4], p[4][3], L;
+  double o[4][3];
+};
+
+__device__ __forceinline__ ContrQuartet load_quartet(int t, const int* __restrict__ nprim,
+                                                     const int* __restrict__ poff, const double* __restrict__ org,
+                                                     const int* __restrict__ pw) {
+  ContrQuartet q;
+  q.L = 0;
+  for (int f = 0; f < 4; ++f) {
+    q.np[f] = nprim[4 * t + f];
+    q.po[f] = poff[4 * t + f];
+    for (int d = 0; d < 3; ++d) {
+      q.p[f][d] = pw[12 * t + 3 * f + d];
+      q.o[f][d] = org[12 * t + 3 * f + d];
+      q.L += q.p[f][d];
     }
-  // transfer I(i,j,k,l) = sum binom(j,n) (xi-xj)^(j-n) binom(l,m) (xk-xl)^(l-m) G(n+i, m+k)
-  const double xij = xi - xj, xkl = xk - xl;
-  double res = 0.0;
-  for (int mm = l; mm >= 0; --mm) {
-    // binomial(l, mm) * xkl^(l-mm)
-    double bl = 1.0;
-    for (int q = 0; q < l - mm; ++q) bl *= xkl * (double)(l - q) / (double)(q + 1);
-    double ijm0 = 0.0;
-    for (int nn = j; nn >= 0; --nn) {
-      double bj = 1.0;
-      for (int q = 0; q < j - nn; ++q) bj *= xij * (double)(j - q) / (double)(q + 1);
-      ijm0 += bj * G[nn + i][mm + k];
-    }
-    res += bl * ijm0;
   }
-  return res;
+  return q;
+}
+
+// Per-primitive-quartet quantities of the recurrence coefficients (in u):
+// C00 = PA - qfac PQ u, D00 = QC + pfac PQ u, B10 = (1 - qfac u) / (2A),
+// B01 = (1 - pfac u) / (2B), B00 = u / (2 (A + B)); T = A B / (A + B) |PQ|^2;
+// pre = 2 pi^(5/2) / (A B sqrt(A + B)) exp(-ab/A |AB|^2 - cd/B |CD|^2).
+struct PrimCoefs {
+  double PA[3], QC[3], PQ[3], qfac, pfac, i2p, i2q, hr, T, pre;
+};
+
+__device__ __forceinline__ PrimCoefs prim_coefs(const ContrQuartet& q, double a0, double a1, double a2, double a3) {
+  PrimCoefs c;
+  const double A = a0 + a1, B = a2 + a3, r = 1.0 / (A + B);
+  double rpq2 = 0.0, kab = 0.0;
+  for (int d = 0; d < 3; ++d) {
+    const double P = (a0 * q.o[0][d] + a1 * q.o[1][d]) / A, Q = (a2 * q.o[2][d] + a3 * q.o[3][d]) / B;
+    c.PA[d] = P - q.o[0][d];
+    c.QC[d] = Q - q.o[2][d];
+    c.PQ[d] = P - Q;
+    rpq2 += c.PQ[d] * c.PQ[d];
+    kab += a0 * a1 / A * (q.o[0][d] - q.o[1][d]) * (q.o[0][d] - q.o[1][d]) +
+           a2 * a3 / B * (q.o[2][d] - q.o[3][d]) * (q.o[2][d] - q.o[3][d]);
+  }
+  c.qfac = B * r;
+  c.pfac = A * r;
+  c.i2p = 0.5 / A;
+  c.i2q = 0.5 / B;
+  c.hr = 0.5 * r;
+  c.T = A * B * r * rpq2;
+  c.pre = 2.0 * pow(M_PI, 2.5) / (A * B * sqrt(A + B)) * exp(-kab);
+  return c;
+}
+
+__device__ __forceinline__ double quartet_prim_scale(const ContrQuartet& q, const double* __restrict__ coefs,
+                                                     const double* __restrict__ norms, int i, int j, int k,
+                                                     int l) {
+  const int ix[4] = {q.po[0] + i, q.po[1] + j, q.po[2] + k, q.po[3] + l};
+  double s = 1.0;
+  for (int f = 0; f < 4; ++f) s *= coefs[ix[f]] * norms[ix[f]];
+  return s;
 }
 
 __global__ void k_contr_eri(int n, const int* __restrict__ nprim, const int* __restrict__ poff,
@@ -564,46 +719,55 @@ __global__ void k_contr_eri(int n, const int* __restrict__ nprim, const int* __r
                             const int* __restrict__ pw, double* out) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= n) return;
-  int np[4], po[4], p[4][3];
-  double o[4][3];
-  int L = 0;
-  for (int f = 0; f < 4; ++f) {
-    np[f] = nprim[4 * t + f];
-    po[f] = poff[4 * t + f];
-    for (int d = 0; d < 3; ++d) {
-      p[f][d] = pw[12 * t + 3 * f + d];
-      o[f][d] = org[12 * t + 3 * f + d];
-      L += p[f][d];
-    }
-  }
-  const int nr = L / 2 + 1;
+  const ContrQuartet q = load_quartet(t, nprim, poff, org, pw);
+  const int nr = q.L / 2 + 1;
   double val = 0.0;
-  for (int i = 0; i < np[0]; ++i)
-    for (int j = 0; j < np[1]; ++j)
-      for (int k = 0; k < np[2]; ++k)
-        for (int l = 0; l < np[3]; ++l) {
-          const double a0 = exps[po[0] + i], a1 = exps[po[1] + j], a2 = exps[po[2] + k], a3 = exps[po[3] + l];
-          const double A = a0 + a1, B = a2 + a3, rho = A * B / (A + B);
-          double rpq2 = 0.0, kab = 0.0;
-          for (int d = 0; d < 3; ++d) {
-            const double P = (a0 * o[0][d] + a1 * o[1][d]) / A, Q = (a2 * o[2][d] + a3 * o[3][d]) / B;
-            rpq2 += (P - Q) * (P - Q);
-            kab += a0 * a1 / A * (o[0][d] - o[1][d]) * (o[0][d] - o[1][d]) +
-                   a2 * a3 / B * (o[2][d] - o[3][d]) * (o[2][d] - o[3][d]);
-          }
+  for (int i = 0; i < q.np[0]; ++i)
+    for (int j = 0; j < q.np[1]; ++j)
+      for (int k = 0; k < q.np[2]; ++k)
+        for (int l = 0; l < q.np[3]; ++l) {
+          const PrimCoefs c = prim_coefs(q, exps[q.po[0] + i], exps[q.po[1] + j], exps[q.po[2] + k],
+                                         exps[q.po[3] + l]);
           double u[5], w[5];
-          rys_roots_rt<double>(nr, rho * rpq2, u, w);
+          rys_roots_rt<double>(nr, c.T, u, w);
           double sum = 0.0;
           for (int r = 0; r < nr; ++r) {
             double prod = w[r];
             for (int d = 0; d < 3; ++d)
-              prod *= rys_1d_rt(u[r], p[0][d], p[1][d], p[2][d], p[3][d], o[0][d], o[1][d], o[2][d], o[3][d],
-                                a0, a1, a2, a3);
+              prod *= rys_1d_rt<double>(q.p[0][d], q.p[1][d], q.p[2][d], q.p[3][d],
+                                        c.PA[d] - c.qfac * c.PQ[d] * u[r], c.QC[d] + c.pfac * c.PQ[d] * u[r],
+                                        c.i2p * (1.0 - c.qfac * u[r]), c.i2q * (1.0 - c.pfac * u[r]),
+                                        c.hr * u[r], q.o[0][d] - q.o[1][d], q.o[2][d] - q.o[3][d]);
             sum += prod;
           }
-          const double pre = 2.0 * pow(M_PI, 2.5) / (A * B * sqrt(A + B)) * exp(-kab);
-          val += coefs[po[0] + i] * coefs[po[1] + j] * coefs[po[2] + k] * coefs[po[3] + l] * norms[po[0] + i] *
-                 norms[po[1] + j] * norms[po[2] + k] * norms[po[3] + l] * pre * sum;
+          val += quartet_prim_scale(q, coefs, norms, i, j, k, l) * c.pre * sum;
+        }
+  out[t] = val;
+}
+
+__global__ void k_tho_contr(int n, const int* __restrict__ nprim, const int* __restrict__ poff,
+                            const double* __restrict__ exps, const double* __restrict__ coefs,
+                            const double* __restrict__ norms, const double* __restrict__ org,
+                            const int* __restrict__ pw, double eps, int itmax, double* out) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  const ContrQuartet q = load_quartet(t, nprim, poff, org, pw);
+  double val = 0.0;
+  for (int i = 0; i < q.np[0]; ++i)
+    for (int j = 0; j < q.np[1]; ++j)
+      for (int k = 0; k < q.np[2]; ++k)
+        for (int l = 0; l < q.np[3]; ++l) {
+          const PrimCoefs c = prim_coefs(q, exps[q.po[0] + i], exps[q.po[1] + j], exps[q.po[2] + k],
+                                         exps[q.po[3] + l]);
+          UPoly prod = up_lin(1.0, 0.0);
+          for (int d = 0; d < 3; ++d)
+            prod = prod * rys_1d_rt<UPoly>(q.p[0][d], q.p[1][d], q.p[2][d], q.p[3][d],
+                                           up_lin(c.PA[d], -c.qfac * c.PQ[d]), up_lin(c.QC[d], c.pfac * c.PQ[d]),
+                                           up_lin(c.i2p, -c.i2p * c.qfac), up_lin(c.i2q, -c.i2q * c.pfac),
+                                           up_lin(0.0, c.hr), q.o[0][d] - q.o[1][d], q.o[2][d] - q.o[3][d]);
+          double sum = 0.0;
+          for (int m = 0; m <= q.L; ++m) sum += prod.c[m] * fgamma_dev((double)m, c.T, eps, itmax);
+          val += quartet_prim_scale(q, coefs, norms, i, j, k, l) * c.pre * sum;
         }
   out[t] = val;
 }
@@ -832,6 +996,17 @@ hipError_t launch_contr_eri(int n, const int* nprim, const int* poff, const doub
   if (n <= 0) return hipSuccess;
   hipLaunchKernelGGL(k_contr_eri, dim3((n + 63) / 64), dim3(64), 0, st, n, nprim, poff, exps, coefs, norms,
                      origins, powers, out);
+  return hipGetLastError();
+}
+
+hipError_t launch_tho_contr(int n, const int* nprim, const int* poff, const double* exps, const double* coefs,
+                            const double* norms, const double* origins, const int* powers, int compat_nr,
+                            double* out, hipStream_t st) {
+  if (n <= 0) return hipSuccess;
+  const double eps = compat_nr ? 3.0e-7 : 1e-17;
+  const int itmax = compat_nr ? 100 : 2000;
+  hipLaunchKernelGGL(k_tho_contr, dim3((n + 63) / 64), dim3(64), 0, st, n, nprim, poff, exps, coefs, norms,
+                     origins, powers, eps, itmax, out);
   return hipGetLastError();
 }
 

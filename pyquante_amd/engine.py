@@ -119,6 +119,13 @@ class Engine(object):
         self.fp32cut = fp32_cut
         return self
 
+    def generation(self):
+        """Work-list generation (pq_engine_generation): changes whenever the
+        device arrays a captured Fock build reads are replaced."""
+        g = ctypes.c_uint64()
+        check(lib.pq_engine_generation(self.handle, ctypes.byref(g)))
+        return g.value
+
     def set_streams(self, n):
         """Spread class kernels over n engine streams (0 = caller's stream only)."""
         check(lib.pq_engine_set_streams(self.handle, int(n)))
@@ -333,13 +340,16 @@ class Engine(object):
             raise ValueError("method must be 'rys' or 'hgp'")
         return out
 
-    def get1ints(self):
-        """(S, T, V) device tensors (pq_get1ints); h = T + V."""
+    def get1ints(self, compat_nr=0):
+        """(S, T, V) device tensors (pq_get1ints); h = T + V.  compat_nr = 1
+        evaluates V with the reference's EPS = 3e-7 Boys function (its
+        cints values); 0 with full-precision Boys values, the V whose
+        derivative the gradient kernels contract."""
         S = torch.empty(self.mshape, dtype=torch.float64, device=self.device)
         T = torch.empty_like(S)
         V = torch.empty_like(S)
         check(lib.pq_get1ints(self.handle, ctypes.c_void_p(S.data_ptr()), ctypes.c_void_p(T.data_ptr()),
-                              ctypes.c_void_p(V.data_ptr()), _stream()))
+                              ctypes.c_void_p(V.data_ptr()), int(compat_nr), _stream()))
         return S, T, V
 
 
@@ -355,6 +365,10 @@ def digest_stored(ints, D, want_j=True, want_k=True):
     return J, K
 
 
+class StaleGraphError(RuntimeError):
+    """A captured Fock build whose engine has replaced its work lists."""
+
+
 class FockGraph:
     """One pq_fock_jk call captured into a HIP graph on fixed D/J/K buffers.
 
@@ -362,8 +376,10 @@ class FockGraph:
     and replays the graph: the fill, class and symmetrisation kernels of the
     build, with the engine's stream fork/join, are launched as one graph.
     The returned J and K are the graph's own buffers, overwritten by the
-    next replay.  The work lists must not be rebuilt (pq_build_pairs,
-    autotune) while the graph is in use: the graph holds their addresses.
+    next replay.  The graph holds the device addresses of the engine's work
+    lists: build(), set_geometry(), set_group_cap(s)() and autotune() replace
+    them, and a call after any of them raises StaleGraphError (checked on
+    the host against the engine's work-list generation) instead of replaying.
     """
 
     def __init__(self, engine, D, precision=_abi.PQ_PREC_FP64):
@@ -381,8 +397,12 @@ class FockGraph:
         with torch.cuda.graph(self.graph):
             self.J, self.K = engine.fock_jk(self.D, precision=precision)
         torch.cuda.synchronize(engine.device)
+        self.generation = engine.generation()
 
     def __call__(self, D=None):
+        if self.engine.generation() != self.generation:
+            raise StaleGraphError("the engine's work lists changed since this Fock build was captured; "
+                                  "capture it again")
         if D is not None:
             self.D.copy_(self.engine._dtensor(D))
         self.graph.replay()

@@ -7,8 +7,10 @@ by pq_forces.  D = C_occ C_occ^T and Q (W) = C_occ diag(e_occ) C_occ^T
 (Wavefunction.py:143-160).
 
 RHFEnergy is a torch.autograd.Function: forward runs the SCF at the given
-coordinates and returns E; backward returns grad_output * dE/dR from the
-gradient kernels, so E(R) composes with torch autograd.
+coordinates and returns E, keeping the engine and the converged D and W;
+backward runs the gradient kernels on them and returns grad_output * dE/dR,
+so E(R) composes with torch autograd and costs no gradient when none is
+asked for.
 """
 import numpy as np
 import torch
@@ -107,7 +109,8 @@ def hf_force(mol, wf, bname):
 
 class RHFEnergy(torch.autograd.Function):
     """E(R): forward = converged RHF energy at coordinates R (natom x 3, bohr),
-    backward = dE/dR from the gradient kernels."""
+    backward = dE/dR from the gradient kernels (pq_forces) on the converged
+    D and energy-weighted W that forward keeps."""
 
     @staticmethod
     def forward(ctx, coords, mol, basis_data, conv):
@@ -116,12 +119,16 @@ class RHFEnergy(torch.autograd.Function):
         eng = Engine(BasisSet(m, basis_data), m)
         eng.build(1e-12, 0.0)
         E, orbe, C, D, W = scf(eng, m, ConvCriteria=conv)
-        ctx.grad = eng.gradient(D, W).to(coords.device, coords.dtype)
+        ctx.engine = eng
+        ctx.save_for_backward(D, W)
         return torch.tensor(E, dtype=coords.dtype, device=coords.device)
 
     @staticmethod
     def backward(ctx, grad_out):
-        return grad_out * ctx.grad, None, None, None
+        D, W = ctx.saved_tensors
+        g = ctx.engine.gradient(D, W).to(grad_out.device, grad_out.dtype)
+        ctx.engine = None
+        return grad_out * g, None, None, None
 
 
 def rhf_energy(coords, mol, basis_data="6-31g**", conv=1e-10):

@@ -91,7 +91,10 @@ def rhf(atoms, **opts):
 
     Extra options: method "rys" (direct J/K) or "hgp" (stored HGP integrals,
     as with the reference's chgp backend); precision PQ_PREC_MIXED (direct
-    Rys with FP32 quartets below fp32_cut)."""
+    Rys with FP32 quartets below fp32_cut); reference_boys (default
+    Ints.reference_boys: V with the reference's EPS = 3e-7 Boys function, as
+    its getints computes it)."""
+    from . import Ints as _Ints
     ConvCriteria = opts.get("ConvCriteria", 1e-4)
     MaxIter = opts.get("MaxIter", 20)
     DoAveraging = opts.get("DoAveraging", False)
@@ -111,7 +114,7 @@ def rhf(atoms, **opts):
     else:
         engine = Engine(bfs, atoms)
         dev = engine.device
-        Sd, Td, Vd = engine.get1ints()
+        Sd, Td, Vd = engine.get1ints(1 if opts.get("reference_boys", _Ints.reference_boys) else 0)
         S, h = Sd, Td + Vd
         Ints = Integrals(bfs, atoms, engine=engine, method=opts.get("method", None))
     S = _t(S, dev)
@@ -126,13 +129,15 @@ def rhf(atoms, **opts):
     eold = 0.0
     avg = DIIS(S) if DoAveraging else None
     energy = 0.0
+    mixed = precision != _abi.PQ_PREC_FP64
+    if mixed:
+        # mixed-precision work lists, built once for the whole SCF
+        Ints.engine.build(opts.get("schwarz_cut", 1e-12), opts.get("fp32_cut", 1e-4))
     for i in range(MaxIter):
         D = mkdens(orbs, 0, nocc)
-        if precision == _abi.PQ_PREC_FP64:
-            from .Ints import get2JmK
-            G = get2JmK(Ints, D)
+        if not mixed:
+            G = _Ints.get2JmK(Ints, D)
         else:
-            Ints.engine.build(opts.get("schwarz_cut", 1e-12), opts.get("fp32_cut", 1e-4))
             J, K = Ints.engine.fock_jk(0.5 * (D + D.T), precision=precision)
             G = 2.0 * J - K
         F = h + G
@@ -144,6 +149,9 @@ def rhf(atoms, **opts):
         if abs(energy - eold) < ConvCriteria:
             break
         eold = energy
+    if mixed:
+        # later FP64 calls through Ints.jk rebuild their own lists
+        Ints._built_cut = None
     return energy, orbe.cpu().numpy(), orbs.cpu().numpy()
 
 
@@ -164,10 +172,11 @@ def uhf(atoms, **opts):
     if not bfs:
         bfs = getbasis(atoms, opts.get("basis_data", None))
     nalpha, nbeta = atoms.get_alphabeta()
+    from . import Ints as _Ints
     engine = Engine(bfs, atoms)
     engine.build(opts.get("schwarz_cut", 1e-12), 0.0)
     dev = engine.device
-    Sd, Td, Vd = engine.get1ints()
+    Sd, Td, Vd = engine.get1ints(1 if opts.get("reference_boys", _Ints.reference_boys) else 0)
     S, h = Sd, Td + Vd
     X = chol_orth(S)
     orbs = opts.get("orbs", None)

@@ -105,3 +105,33 @@ def test_fp32_rys_table_moments():
                 ref = O.boys(m, x)
                 got = float(np.sum(w * u ** m))
                 assert abs(got - ref) <= 1e-6 * ref + 1e-30, (n, x, m, got, ref)
+
+
+def test_stale_fock_graph_raises_on_host():
+    """A captured Fock build refuses to replay once its engine's work-list
+    generation has moved (the check runs on the host, before any replay)."""
+    from pyquante_amd.engine import FockGraph, StaleGraphError
+
+    class _Eng(object):
+        gen = 7
+
+        def generation(self):
+            return self.gen
+
+    class _Graph(object):
+        replays = 0
+
+        def replay(self):
+            _Graph.replays += 1
+
+    fg = FockGraph.__new__(FockGraph)
+    fg.engine = _Eng()
+    fg.graph = _Graph()
+    fg.generation = 7
+    fg.J = fg.K = None
+    fg()
+    assert _Graph.replays == 1
+    fg.engine.gen = 8  # e.g. build() or set_group_caps() replaced the work lists
+    with pytest.raises(StaleGraphError):
+        fg()
+    assert _Graph.replays == 1
