@@ -265,6 +265,17 @@ int pq_fock_jk_timed(pq_engine *eng, const double *D_dev, double *J_dev, double 
 int pq_digest_stored(int nbf, const double *ints_dev, uint64_t len, const double *D_dev, double *J_dev,
                      double *K_dev, void *stream);
 
+/* pq_digest_stored without any allocation.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops).  The J and K accumulators
+ * live in a library-owned static device array of 2 * 512^2 doubles (4 MiB,
+ * held by each GPU the library runs on until the process ends), so nbf <= 512
+ * (PQ_E_ARG beyond).  Every call on one GPU shares that array: calls that
+ * overlap on one GPU (different streams, or threads) race.  Asynchronous on
+ * `stream`; J_dev or K_dev may be NULL. */
+int pq_digest_stored_noalloc(int nbf, const double *ints_dev, uint64_t len, const double *D_dev, double *J_dev,
+                             double *K_dev, void *stream);
+
 /* Rows mu0 .. mu0+nmu-1 of the full 4-index tensor (mu nu|lam sig) from the
  * flat unique array (len = npair(npair+1)/2), row-major into out_dev
  * (nmu*nbf^3 doubles): the unpacking step of the reference's AO->MO transform

@@ -76,6 +76,7 @@ _proto("pq_fock_jk_multi", [_VP, _I, _c.POINTER(_VP), _c.POINTER(_VP), _c.POINTE
 _proto("pq_engine_class_stats", [_VP, _I, _PI, _PU64, _PD])
 _proto("pq_fock_jk_timed", [_VP, _VP, _VP, _VP, _I, _VP, _c.POINTER(_c.c_float)])
 _proto("pq_digest_stored", [_I, _VP, _U64, _VP, _VP, _VP, _VP])
+_proto("pq_digest_stored_noalloc", [_I, _VP, _U64, _VP, _VP, _VP, _VP])
 _proto("pq_unpack_eri", [_I, _VP, _U64, _I, _I, _VP, _VP])
 _proto("pq_get1ints", [_VP, _VP, _VP, _VP, _I, _VP])
 _proto("pq_forces", [_VP, _VP, _VP, _VP, _VP])

@@ -8,6 +8,7 @@
 #include <cstdio>
 #include <cstdint>
 #include <cstring>
+#include <mutex>
 #include <numeric>
 #include <string>
 #include <vector>
@@ -1462,6 +1463,62 @@ int pq_digest_stored(int nbf, const double* ints_dev, uint64_t len, const double
   dev_free(Jacc);
   dev_free(Kacc);
   PQ_HIP(err);
+  return PQ_OK;
+}
+
+}  // extern "C"
+
+// Scratch of pq_digest_stored_noalloc: the J and K accumulators of one call,
+// 2 * kNoAllocMaxNbf^2 doubles (4 MiB).  Each GPU holds its own copy for the
+// life of the process.  In host code `scratch` names a host-side stand-in,
+// not device memory: kernels get the address from scratch_on_device().
+constexpr int kNoAllocMaxNbf = 512;
+static __device__ __align__(256) double scratch[2 * kNoAllocMaxNbf * kNoAllocMaxNbf];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+double* g_scratch_dev[kMaxDevices] = {};
+
+// Device address of the current device's scratch copy, looked up once per
+// device (hipGetSymbolAddress with that device current) and kept.
+int scratch_on_device(double** out) {
+  int dev = 0;
+  PQ_HIP(hipGetDevice(&dev));
+  if (dev < 0 || dev >= kMaxDevices) return fail(PQ_E_ARG, "device index beyond the scratch table");
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_dev[dev]) {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p)
+      return fail(PQ_E_HIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+    g_scratch_dev[dev] = static_cast<double*>(p);
+  }
+  *out = g_scratch_dev[dev];
+  return PQ_OK;
+}
+}  // namespace
+
+extern "C" {
+
+int pq_digest_stored_noalloc(int nbf, const double* ints_dev, uint64_t len, const double* D_dev, double* J_dev,
+                             double* K_dev, void* stream) {
+  if (nbf <= 0 || !ints_dev || !D_dev) return fail(PQ_E_ARG, "bad arguments");
+  if (nbf > kNoAllocMaxNbf)
+    return fail(PQ_E_ARG, "nbf " + std::to_string(nbf) + " exceeds the no-allocation limit of " +
+                              std::to_string(kNoAllocMaxNbf) + " functions");
+  const uint64_t n = nbf, npair = n * (n + 1) / 2;
+  if (len != npair * (npair + 1) / 2) return fail(PQ_E_ARG, "integral array length mismatch");
+  double* s = nullptr;
+  const int rc = scratch_on_device(&s);
+  if (rc) return rc;
+  hipStream_t st = (hipStream_t)stream;
+  double* Jacc = s;
+  double* Kacc = s + (size_t)nbf * nbf;
+  // a fill kernel, not a memset, so that the call can be graph-captured
+  PQ_HIP(launch_fill(s, 2 * (int64_t)nbf * nbf, 0.0, st));
+  PQ_HIP(launch_digest_stored(nbf, ints_dev, len, D_dev, Jacc, Kacc, st));
+  PQ_HIP(launch_symmetrize(nbf, Jacc, Kacc, J_dev, K_dev, 1.0, 1.0, st));
   return PQ_OK;
 }
 
