@@ -16,6 +16,8 @@ Fixtures
   prim_classes.npz   random primitive quartets per angular-momentum class up
                      to L_total = 9 (crys, cints, chgp coulomb_repulsion)
   onee_prims.npz     random primitive pairs: cints overlap/kinetic/nuclear
+  h2o8_631ss.npz     20,000 seeded unique function quartets of the seeded
+                     (H2O)_8 6-31G** cluster (crys), with its nuclei
 """
 import ctypes
 import os
@@ -28,7 +30,7 @@ ROOT = os.path.dirname(os.path.dirname(HERE))
 sys.path.insert(0, ROOT)
 
 from pyquante_amd.basis import BasisSet  # noqa: E402
-from pyquante_amd.molecule import Molecule, h2o_sto3g_config  # noqa: E402
+from pyquante_amd.molecule import Molecule, h2o_sto3g_config, water_cluster  # noqa: E402
 
 REF = os.path.join(ROOT, "oracle", "_ref")
 D, I = ctypes.c_double, ctypes.c_int
@@ -240,11 +242,41 @@ def onee_fixture():
              T=np.array(T), V=np.array(V))
 
 
+def h2o8_fixture():
+    """20,000 unique function quartets (i >= j, k >= l, ij >= kl) of the
+    seeded (H2O)_8 cluster, put in canonical order, with the compiled crys
+    values: 15,000 drawn uniformly over ordered quartets and 5,000 with all
+    four functions on one water (the large integrals)."""
+    mol = water_cluster((2, 2, 2), seed=0)
+    bfs = BasisSet(mol, "6-31g**").bfs
+    rng = np.random.default_rng(21)
+    q = rng.integers(0, len(bfs), size=(15000, 4))
+    w = rng.integers(0, 8, size=(5000, 1)) * 25
+    q = np.concatenate([q, w + rng.integers(0, 25, size=(5000, 4))])
+    ijkl, val, idx = [], [], []
+    for i, j, k, l in q:
+        i, j = max(i, j), min(i, j)
+        k, l = max(k, l), min(k, l)
+        if i * (i + 1) // 2 + j < k * (k + 1) // 2 + l:
+            i, j, k, l = k, l, i, j
+        ijkl.append((i, j, k, l))
+        val.append(contr(crys, "crys", (bfs[i], bfs[j], bfs[k], bfs[l])))
+    ijkl = np.array(ijkl, dtype=np.int32)
+    xyz = np.array([a.pos() for a in mol.atoms])
+    z = np.array([a.atno for a in mol.atoms], dtype=np.int32)
+    np.savez(os.path.join(HERE, "h2o8_631ss.npz"), ijkl=ijkl, crys=np.array(val), xyz=xyz, z=z)
+
+
 if __name__ == "__main__":
+    if len(sys.argv) > 1:
+        for name in sys.argv[1:]:
+            globals()[name]()
+        sys.exit(0)
     boys_fixture()
     roots_fixture()
     index_fixture()
     prim_class_fixture()
     onee_fixture()
     molecule_fixtures()
+    h2o8_fixture()
     print("golden fixtures written")
