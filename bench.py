@@ -124,6 +124,33 @@ def cpu_baseline_all_cores(bfs, seconds=5.0, seed=0, method="crys", max_workers=
                       "-O2, %d processes x %.1f s" % (sum(d for d, _ in res), fn, workers, seconds)}
 
 
+def grad_flops(la, lb, lc, ld):
+    """Operation count of the two-electron derivative kernel (FMA = 2 flops),
+    the gradient counterpart of the F_prim / F_dig convention: per primitive
+    quartet, 30 for the pair setup, and per Rys root (nr = floor((L+1)/2) + 1)
+    60 for the root, the 1-D recurrences and transfers of the class one
+    higher on A, B and C, (la+1 lb+1 | lc+1 ld), and 51 per Cartesian
+    integral (the three yz/xz/xy products, then for each of the centres A, B,
+    C and each direction the shifted-pair derivative and its accumulation);
+    per contracted quartet 10 per integral for the density factor
+    16 D_ab D_cd - 4 (D_ac D_bd + D_ad D_bc).  Returns (per primitive
+    quartet, per contracted quartet)."""
+    ncart = [1, 3, 6, 10]
+    L = la + lb + lc + ld
+    nr = (L + 1) // 2 + 1
+    a1, b1, c1 = la + 1, lb + 1, lc + 1
+    n, m = a1 + b1, c1 + ld
+    sb = sum(n - j + 1 for j in range(1, b1 + 1))
+    sd = sum(m - l + 1 for l in range(1, ld + 1))
+    nc = ncart[la] * ncart[lb] * ncart[lc] * ncart[ld]
+    rec = 4 * (n + 1) * (m + 1) + 2 * (m + 1) * sb + 2 * (a1 + 1) * (b1 + 1) * sd
+    return 30.0 + nr * (60.0 + 3.0 * rec + 51.0 * nc), 10.0 * nc
+
+
+def grad_kernel_name(cls):
+    return "k_eri_grad<%d, %d, %d, %d>" % tuple(cls)
+
+
 def recorded(name, key, field):
     """`field` of the entry recorded for `key` (kernel and workload) in
     profiles/<name> from rocprofv3 counter runs, or None."""
@@ -231,6 +258,35 @@ def extra_workloads(args):
     tg = _timed(lambda: eng.gradient(D, W), args.steps)
     out.update({"h2o8_ms_per_gradient": tg * 1e3, "h2o8_forces_per_s": 1.0 / tg})
     print("bench: (H2O)_8 gradient %.2f ms (%.1f forces/s)" % (tg * 1e3, 1.0 / tg), file=sys.stderr, flush=True)
+    # gradient roofline: the dominant derivative kernel's operation count
+    # over its event-timed launch (HIP events on its own stream, the kernels
+    # running concurrently as in the timed gradients)
+    import numpy as _np
+    acc = _np.zeros(49)
+    for _ in range(10):
+        acc += _np.array(eng.gradient_timed(D, W)[1])
+    acc /= 10
+    cls = eng.class_stats()
+    flops = []
+    for c in cls:
+        fp, fc = grad_flops(*c["class"])
+        flops.append(fp * (c["prim_quartets_fp64"] + c["prim_quartets_fp32"]) +
+                     fc * (c["quartets_fp64"] + c["quartets_fp32"]))
+    kt = [acc[2 * i] + acc[2 * i + 1] for i in range(21)]
+    k = int(_np.argmax(kt))
+    ach = flops[k] / (kt[k] * 1e-3) / 1e12
+    gname = grad_kernel_name(cls[k]["class"])
+    out["roofline_forces"] = {
+        "bound": "valu", "kernel": "%s, (%s%s|%s%s) derivative integrals contracted with D" %
+        (gname, *("spd"[x] for x in cls[k]["class"])),
+        "achieved": ach, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": ach / FP64_PEAK_TFLOPS,
+        "launch_ms": kt[k], "flops_per_launch": flops[k],
+        "traffic": recorded_traffic("%s h2o8_631ss_grad" % gname),
+        "all_kernels_tflops": sum(flops) / (acc[48] * 1e-3) / 1e12,
+        "gradient_event_ms": acc[48]}
+    print("bench: (H2O)_8 gradient dominant kernel %s %.3f ms, %.2f TFLOP/s (%.3f of peak); all %.2f TFLOP/s"
+          % (gname, kt[k], ach, ach / FP64_PEAK_TFLOPS, out["roofline_forces"]["all_kernels_tflops"]),
+          file=sys.stderr, flush=True)
     # two densities in one pass (the unrestricted J/K) against two builds
     Db_ = (0.9 * D + 0.1 * torch.eye(D.shape[0], dtype=D.dtype, device=D.device)).contiguous()
     t2 = _timed(lambda: eng.fock_jk_multi([D, Db_]), args.steps)
@@ -531,6 +587,8 @@ def main():
         "cpu_baseline": cpu,
         "cpu_baseline_all_cores": cpu_all,
     }
+    if "roofline_forces" in line["config"]:
+        line["roofline_forces"] = line["config"].pop("roofline_forces")
     print(json.dumps(line))
     return 0
 

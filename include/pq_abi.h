@@ -298,6 +298,16 @@ int pq_get1ints(pq_engine *eng, double *S_dev, double *T_dev, double *V_dev, int
  * integrals are contracted on the fly over the current (FP64) work lists. */
 int pq_forces(pq_engine *eng, const double *D_dev, const double *W_dev, double *grad_dev, void *stream);
 
+/* pq_forces with HIP events around every kernel launch, recorded on the
+ * stream it is launched on: ms[2c] / ms[2c+1] the two-electron derivative
+ * kernel of class c over its FP64 / FP32-bucket tiles (21 classes, the order
+ * of pq_engine_class_stats; all run in FP64), ms[42+p] the one-electron
+ * derivative kernel of pair class p (ss, ps, pp, ds, dp, dd), ms[48] the
+ * whole gradient (49 floats).  Synchronises the stream.  No reference
+ * counterpart (measurement). */
+int pq_forces_timed(pq_engine *eng, const double *D_dev, const double *W_dev, double *grad_dev, void *stream,
+                    float *ms);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1575,7 +1575,8 @@ int pq_shell_quartet_blocks(pq_engine* eng, int nq, const int* shells_host, doub
   return PQ_OK;
 }
 
-int pq_forces(pq_engine* eng, const double* D_dev, const double* W_dev, double* grad_dev, void* stream) {
+static int forces_impl(pq_engine* eng, const double* D_dev, const double* W_dev, double* grad_dev, void* stream,
+                       float* ms) {
   if (!eng || !D_dev || !W_dev || !grad_dev) return fail(PQ_E_ARG, "null argument");
   if (!eng->work_ok) return fail(PQ_E_STATE, "pq_build_pairs must be called before pq_forces");
   hipStream_t st = (hipStream_t)stream;
@@ -1586,34 +1587,95 @@ int pq_forces(pq_engine* eng, const double* D_dev, const double* W_dev, double* 
     PQ_HIP(dev_alloc(&eng->d_gacc, gsize));
     eng->gacc_n = gsize;
   }
-  PQ_HIP(launch_fill(eng->d_gacc, (int64_t)gsize, 0.0, st));
-  // class kernels round-robin over the side streams, as in the Fock build
-  // (they only add into the gradient copies)
-  PQ_HIP(fork_side(eng, st));
-  int next = 0;
-  auto ks = [&]() {
-    hipStream_t s = eng->nside > 0 ? eng->side[next] : st;
-    if (eng->nside > 0) next = (next + 1) % eng->nside;
-    return s;
+  // timing: one event pair per launch on the stream the kernel runs on
+  std::vector<hipEvent_t> ev;
+  std::vector<int> slot;
+  auto mark = [&](hipStream_t s) -> hipError_t {
+    hipEvent_t e;
+    hipError_t r = hipEventCreate(&e);
+    if (r != hipSuccess) return r;
+    ev.push_back(e);
+    return hipEventRecord(e, s);
   };
-  for (const auto& c : eng->work) {
-    const int la = kPairL[c.X][0], lb = kPairL[c.X][1], lc = kPairL[c.Y][0], ld = kPairL[c.Y][1];
-    if (!c.t64.empty())
-      PQ_HIP(launch_grad_class(la, lb, lc, ld, eng->view_s(), c.d_t64, (int)c.t64.size(), eng->d_sorted_s[c.Y], D_dev,
-                               eng->d_gacc, eng->natom, ks()));
-    if (!c.t32.empty())
-      PQ_HIP(launch_grad_class(la, lb, lc, ld, eng->view_s(), c.d_t32, (int)c.t32.size(), eng->d_sorted_s[c.Y], D_dev,
-                               eng->d_gacc, eng->natom, ks()));
+  int rc = PQ_OK;
+  do {
+    hipError_t err = hipSuccess;
+#define PQ_TRY(x)                                \
+  if ((err = (x)) != hipSuccess) {               \
+    rc = fail(PQ_E_HIP, hipGetErrorString(err)); \
+    break;                                       \
   }
-  for (int c = 0; c < 6; ++c) {
-    if (eng->members[c].empty()) continue;
-    PQ_HIP(launch_onee_grad(kPairL[c][0], kPairL[c][1], eng->view(), eng->d_members[c], (int)eng->members[c].size(),
-                            eng->d_atom_xyz, eng->d_atom_z, eng->natom, D_dev, W_dev, eng->d_gacc, ks()));
-  }
-  PQ_HIP(join_side(eng, st));
-  PQ_HIP(launch_grad_finish(eng->d_gacc, ncopy, eng->natom, eng->d_atom_xyz, eng->d_atom_z,
-                            eng->nmol > 1 ? eng->d_atom_mol : nullptr, grad_dev, st));
-  return PQ_OK;
+    if (ms) PQ_TRY(mark(st));
+    PQ_TRY(launch_fill(eng->d_gacc, (int64_t)gsize, 0.0, st));
+    // class kernels round-robin over the side streams, as in the Fock build
+    // (they only add into the gradient copies)
+    PQ_TRY(fork_side(eng, st));
+    int next = 0;
+    auto ks = [&]() {
+      hipStream_t s = eng->nside > 0 ? eng->side[next] : st;
+      if (eng->nside > 0) next = (next + 1) % eng->nside;
+      return s;
+    };
+    for (size_t ci = 0; ci < eng->work.size() && rc == PQ_OK; ++ci) {
+      const auto& c = eng->work[ci];
+      const int la = kPairL[c.X][0], lb = kPairL[c.X][1], lc = kPairL[c.Y][0], ld = kPairL[c.Y][1];
+      for (int prec = 0; prec < 2; ++prec) {
+        const auto& tl = prec ? c.t32 : c.t64;
+        if (tl.empty()) continue;
+        hipStream_t s = ks();
+        if (ms) PQ_TRY(mark(s));
+        PQ_TRY(launch_grad_class(la, lb, lc, ld, eng->view_s(), prec ? c.d_t32 : c.d_t64, (int)tl.size(),
+                                 eng->d_sorted_s[c.Y], D_dev, eng->d_gacc, eng->natom, s));
+        if (ms) {
+          PQ_TRY(mark(s));
+          slot.push_back((int)(2 * ci + prec));
+        }
+      }
+    }
+    if (rc != PQ_OK) break;
+    for (int c = 0; c < 6; ++c) {
+      if (eng->members[c].empty()) continue;
+      hipStream_t s = ks();
+      if (ms) PQ_TRY(mark(s));
+      PQ_TRY(launch_onee_grad(kPairL[c][0], kPairL[c][1], eng->view(), eng->d_members[c],
+                              (int)eng->members[c].size(), eng->d_atom_xyz, eng->d_atom_z, eng->natom, D_dev, W_dev,
+                              eng->d_gacc, s));
+      if (ms) {
+        PQ_TRY(mark(s));
+        slot.push_back(42 + c);
+      }
+    }
+    if (rc != PQ_OK) break;
+    PQ_TRY(join_side(eng, st));
+    PQ_TRY(launch_grad_finish(eng->d_gacc, ncopy, eng->natom, eng->d_atom_xyz, eng->d_atom_z,
+                              eng->nmol > 1 ? eng->d_atom_mol : nullptr, grad_dev, st));
+    if (ms) {
+      PQ_TRY(mark(st));
+      PQ_TRY(hipStreamSynchronize(st));
+      for (int i = 0; i < 49; ++i) ms[i] = 0.f;
+      for (size_t k = 0; k < slot.size(); ++k) {
+        float t = 0.f;
+        (void)hipEventElapsedTime(&t, ev[1 + 2 * k], ev[2 + 2 * k]);
+        ms[slot[k]] += t;
+      }
+      float tot = 0.f;
+      (void)hipEventElapsedTime(&tot, ev.front(), ev.back());
+      ms[48] = tot;
+    }
+#undef PQ_TRY
+  } while (0);
+  for (auto e : ev) (void)hipEventDestroy(e);
+  return rc;
+}
+
+int pq_forces(pq_engine* eng, const double* D_dev, const double* W_dev, double* grad_dev, void* stream) {
+  return forces_impl(eng, D_dev, W_dev, grad_dev, stream, nullptr);
+}
+
+int pq_forces_timed(pq_engine* eng, const double* D_dev, const double* W_dev, double* grad_dev, void* stream,
+                    float* ms) {
+  if (!ms) return fail(PQ_E_ARG, "null timing array");
+  return forces_impl(eng, D_dev, W_dev, grad_dev, stream, ms);
 }
 
 int pq_get1ints(pq_engine* eng, double* S_dev, double* T_dev, double* V_dev, int compat_nr, void* stream) {

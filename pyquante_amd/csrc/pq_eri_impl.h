@@ -24,6 +24,10 @@
 #include "pq_device.h"
 #include "pq_kernels.h"
 
+#ifndef PQ_WPE_ATTR
+#define PQ_WPE_ATTR
+#endif
+
 namespace pq {
 
 constexpr double kTwoPi25 = 34.98683665524972497;  // 2 pi^(5/2)
@@ -172,6 +176,12 @@ __device__ __forceinline__ void eri_block(const EngineView& ev, const PairInfo& 
 // of the bra rows then come from LDS and only D[c,d] is gathered from memory.
 // jrow, when non-null, is the block's LDS accumulator of the bra J block
 // (NA x NB), flushed once per block instead of one global atomic per tile.
+//
+// The digestion runs in three phases so that few D elements and accumulators
+// are live at once (the register peak of the kernel, and so its occupancy):
+// the J blocks; then the K rows of each bra function a (from the D rows of
+// b), each row flushed as soon as it is complete; then the K rows of each b
+// (from the D rows of a).
 template <int LA, int LB, int LC, int LD, int A0, int NAS, int B0, int NBS, typename T>
 __device__ __forceinline__ void digest_lane(const EngineView& ev, const PairInfo& bp, const PairInfo& kp,
                                             const T (&out)[NAS * NBS * ncart(LC) * ncart(LD)], double f,
@@ -179,134 +189,159 @@ __device__ __forceinline__ void digest_lane(const EngineView& ev, const PairInfo
                                             double* Kacc, double* krow, double* jsum,
                                             const double* drow, double* jrow, double* jpair, int jn, int kpos) {
   using C = Cls<LA, LB, LC, LD>;
+  constexpr int NC = C::NC, ND = C::ND, NO = NAS * NBS * NC * ND;
   // the quartet's molecule block (bra and ket share it)
   const int n = bp.mstride;
   D += bp.moff;
   Jacc += bp.moff;
   Kacc += bp.moff;
   const int a0 = bp.bfA + A0, b0 = bp.bfB + B0, c0 = kp.bfA, d0 = kp.bfB;
-  double jab[NAS][NBS], jcd[C::NC][C::ND], kac[NAS][C::NC], kad[NAS][C::ND], kbc[NBS][C::NC], kbd[NBS][C::ND];
-  double Dab[NAS][NBS], Dcd[C::NC][C::ND], Dac[NAS][C::NC], Dad[NAS][C::ND], Dbc[NBS][C::NC], Dbd[NBS][C::ND];
   // explicit address spaces keep the two paths apart (a merged load through
   // a generic pointer would be a FLAT load on both)
   using lds_cd = const __attribute__((address_space(3))) double;
   using glb_cd = const __attribute__((address_space(1))) double;
   glb_cd* Dg = (glb_cd*)D;
-  if (drow) {
-#pragma unroll
-    for (int a = 0; a < NAS; ++a) {
-      lds_cd* ra = (lds_cd*)drow + (A0 + a) * n;
-#pragma unroll
-      for (int b = 0; b < NBS; ++b) { Dab[a][b] = ra[b0 + b]; jab[a][b] = 0; }
-#pragma unroll
-      for (int c = 0; c < C::NC; ++c) { Dac[a][c] = ra[c0 + c]; kac[a][c] = 0; }
-#pragma unroll
-      for (int d = 0; d < C::ND; ++d) { Dad[a][d] = ra[d0 + d]; kad[a][d] = 0; }
-    }
-#pragma unroll
-    for (int b = 0; b < NBS; ++b) {
-      lds_cd* rb = (lds_cd*)drow + (C::NA + B0 + b) * n;
-#pragma unroll
-      for (int c = 0; c < C::NC; ++c) { Dbc[b][c] = rb[c0 + c]; kbc[b][c] = 0; }
-#pragma unroll
-      for (int d = 0; d < C::ND; ++d) { Dbd[b][d] = rb[d0 + d]; kbd[b][d] = 0; }
-    }
-  } else {
-#pragma unroll
-  for (int a = 0; a < NAS; ++a) {
-#pragma unroll
-    for (int b = 0; b < NBS; ++b) { Dab[a][b] = Dg[(a0 + a) * n + b0 + b]; jab[a][b] = 0; }
-#pragma unroll
-    for (int c = 0; c < C::NC; ++c) { Dac[a][c] = Dg[(a0 + a) * n + c0 + c]; kac[a][c] = 0; }
-#pragma unroll
-    for (int d = 0; d < C::ND; ++d) { Dad[a][d] = Dg[(a0 + a) * n + d0 + d]; kad[a][d] = 0; }
-  }
-#pragma unroll
-  for (int b = 0; b < NBS; ++b) {
-#pragma unroll
-    for (int c = 0; c < C::NC; ++c) { Dbc[b][c] = Dg[(b0 + b) * n + c0 + c]; kbc[b][c] = 0; }
-#pragma unroll
-    for (int d = 0; d < C::ND; ++d) { Dbd[b][d] = Dg[(b0 + b) * n + d0 + d]; kbd[b][d] = 0; }
-  }
-  }
-#pragma unroll
-  for (int c = 0; c < C::NC; ++c)
-#pragma unroll
-    for (int d = 0; d < C::ND; ++d) { Dcd[c][d] = Dg[(c0 + c) * n + d0 + d]; jcd[c][d] = 0; }
-
+  lds_cd* Dl = (lds_cd*)drow;
+  // D[row a of shell A, col] and D[row b of shell B, col]
+  auto d_a = [&](int a, int col) -> double { return drow ? Dl[(A0 + a) * n + col] : Dg[(a0 + a) * n + col]; };
+  auto d_b = [&](int b, int col) -> double { return drow ? Dl[(C::NA + B0 + b) * n + col] : Dg[(b0 + b) * n + col]; };
+  // normalised integrals times the degeneracy factor
+  double w[NO];
 #pragma unroll
   for (int a = 0; a < NAS; ++a)
 #pragma unroll
     for (int b = 0; b < NBS; ++b)
 #pragma unroll
-      for (int c = 0; c < C::NC; ++c)
+      for (int c = 0; c < NC; ++c)
 #pragma unroll
-        for (int d = 0; d < C::ND; ++d) {
-          const double nf = cart_norm(LA, A0 + a) * cart_norm(LB, B0 + b) * cart_norm(LC, c) * cart_norm(LD, d);
-          const double v = (double)out[((a * NBS + b) * C::NC + c) * C::ND + d] * (nf * f);
-          jab[a][b] += v * Dcd[c][d];
-          jcd[c][d] += v * Dab[a][b];
-          kac[a][c] += v * Dbd[b][d];
-          kad[a][d] += v * Dbc[b][c];
-          kbc[b][c] += v * Dad[a][d];
-          kbd[b][d] += v * Dac[a][c];
+        for (int d = 0; d < ND; ++d) {
+          const int o = ((a * NBS + b) * NC + c) * ND + d;
+          w[o] = (double)out[o] * (cart_norm(LA, A0 + a) * cart_norm(LB, B0 + b) * cart_norm(LC, c) *
+                                   cart_norm(LD, d) * f);
         }
-  // J_ab: every lane of the wave shares the bra pair -> reduce, one atomic
   const int lane = threadIdx.x & 63;
+  // ---- J: J_ab (shared by the wavefront: reduced, one atomic) and J_cd
+  {
+    double Dcd[NC][ND], Dab[NAS][NBS], jcd[NC][ND];
 #pragma unroll
-  for (int a = 0; a < NAS; ++a)
+    for (int c = 0; c < NC; ++c)
+#pragma unroll
+      for (int d = 0; d < ND; ++d) {
+        Dcd[c][d] = Dg[(c0 + c) * n + d0 + d];
+        jcd[c][d] = 0.0;
+      }
+#pragma unroll
+    for (int a = 0; a < NAS; ++a)
+#pragma unroll
+      for (int b = 0; b < NBS; ++b) Dab[a][b] = d_a(a, b0 + b);
+#pragma unroll
+    for (int a = 0; a < NAS; ++a)
+#pragma unroll
+      for (int b = 0; b < NBS; ++b) {
+        double jab = 0.0;
+#pragma unroll
+        for (int c = 0; c < NC; ++c)
+#pragma unroll
+          for (int d = 0; d < ND; ++d) {
+            const double v = w[((a * NBS + b) * NC + c) * ND + d];
+            jab += v * Dcd[c][d];
+            jcd[c][d] += v * Dab[a][b];
+          }
+        const double s = wave_sum(jab);
+        if (lane == 0) {
+          if (jrow) atomicAdd(&jrow[(A0 + a) * C::NB + B0 + b], 2.0 * s);
+          else unsafeAtomicAdd(&Jacc[(a0 + a) * n + b0 + b], 2.0 * s);
+        }
+      }
+    if (!active) return;
+    if (jsum) {
+#pragma unroll
+      for (int c = 0; c < NC; ++c)
+#pragma unroll
+        for (int d = 0; d < ND; ++d) atomicAdd(&jsum[lane * NC * ND + c * ND + d], 2.0 * jcd[c][d]);
+    } else {
+      // ket-side J by ket position: the lanes of a tile hit consecutive
+      // addresses (coalesced atomics), scattered into J after the build
+#pragma unroll
+      for (int c = 0; c < NC; ++c)
+#pragma unroll
+        for (int d = 0; d < ND; ++d) unsafeAtomicAdd(&jpair[(c * ND + d) * jn + kpos], 2.0 * jcd[c][d]);
+    }
+  }
+  // one K row (bra function row r of the block layout, matrix row grow):
+  // columns c0.. (NC values) and d0.. (ND values)
+  auto flush_row = [&](int r, int grow, const double (&kc)[NC], const double (&kd)[ND]) {
+    if (krow) {
+#pragma unroll
+      for (int c = 0; c < NC; ++c) atomicAdd(&krow[r * n + c0 + c], kc[c]);
+#pragma unroll
+      for (int d = 0; d < ND; ++d) atomicAdd(&krow[r * n + d0 + d], kd[d]);
+    } else {
+#pragma unroll
+      for (int c = 0; c < NC; ++c) unsafeAtomicAdd(&Kacc[grow * n + c0 + c], kc[c]);
+#pragma unroll
+      for (int d = 0; d < ND; ++d) unsafeAtomicAdd(&Kacc[grow * n + d0 + d], kd[d]);
+    }
+  };
+  // ---- K rows of a: K_ac += v D_bd, K_ad += v D_bc
+  {
+    double Dbc[NBS][NC], Dbd[NBS][ND];
 #pragma unroll
     for (int b = 0; b < NBS; ++b) {
-      const double s = wave_sum(jab[a][b]);
-      if (lane == 0) {
-        if (jrow) atomicAdd(&jrow[(A0 + a) * C::NB + B0 + b], 2.0 * s);
-        else unsafeAtomicAdd(&Jacc[(a0 + a) * n + b0 + b], 2.0 * s);
-      }
+#pragma unroll
+      for (int c = 0; c < NC; ++c) Dbc[b][c] = d_b(b, c0 + c);
+#pragma unroll
+      for (int d = 0; d < ND; ++d) Dbd[b][d] = d_b(b, d0 + d);
     }
-  if (!active) return;
-  if (jsum) {
 #pragma unroll
-    for (int c = 0; c < C::NC; ++c)
+    for (int a = 0; a < NAS; ++a) {
+      double kc[NC], kd[ND];
 #pragma unroll
-      for (int d = 0; d < C::ND; ++d) atomicAdd(&jsum[lane * C::NC * C::ND + c * C::ND + d], 2.0 * jcd[c][d]);
-  } else {
-    // ket-side J by ket position: the lanes of a tile hit consecutive
-    // addresses (coalesced atomics), scattered into J after the build
+      for (int c = 0; c < NC; ++c) kc[c] = 0.0;
 #pragma unroll
-    for (int c = 0; c < C::NC; ++c)
+      for (int d = 0; d < ND; ++d) kd[d] = 0.0;
 #pragma unroll
-      for (int d = 0; d < C::ND; ++d) unsafeAtomicAdd(&jpair[(c * C::ND + d) * jn + kpos], 2.0 * jcd[c][d]);
+      for (int b = 0; b < NBS; ++b)
+#pragma unroll
+        for (int c = 0; c < NC; ++c)
+#pragma unroll
+          for (int d = 0; d < ND; ++d) {
+            const double v = w[((a * NBS + b) * NC + c) * ND + d];
+            kc[c] += v * Dbd[b][d];
+            kd[d] += v * Dbc[b][c];
+          }
+      flush_row(A0 + a, a0 + a, kc, kd);
+    }
   }
-  if (krow) {
+  // ---- K rows of b: K_bc += v D_ad, K_bd += v D_ac
+  {
+    double Dac[NAS][NC], Dad[NAS][ND];
 #pragma unroll
     for (int a = 0; a < NAS; ++a) {
 #pragma unroll
-      for (int c = 0; c < C::NC; ++c) atomicAdd(&krow[(A0 + a) * n + c0 + c], kac[a][c]);
+      for (int c = 0; c < NC; ++c) Dac[a][c] = d_a(a, c0 + c);
 #pragma unroll
-      for (int d = 0; d < C::ND; ++d) atomicAdd(&krow[(A0 + a) * n + d0 + d], kad[a][d]);
+      for (int d = 0; d < ND; ++d) Dad[a][d] = d_a(a, d0 + d);
     }
 #pragma unroll
     for (int b = 0; b < NBS; ++b) {
+      double kc[NC], kd[ND];
 #pragma unroll
-      for (int c = 0; c < C::NC; ++c) atomicAdd(&krow[(C::NA + B0 + b) * n + c0 + c], kbc[b][c]);
+      for (int c = 0; c < NC; ++c) kc[c] = 0.0;
 #pragma unroll
-      for (int d = 0; d < C::ND; ++d) atomicAdd(&krow[(C::NA + B0 + b) * n + d0 + d], kbd[b][d]);
+      for (int d = 0; d < ND; ++d) kd[d] = 0.0;
+#pragma unroll
+      for (int a = 0; a < NAS; ++a)
+#pragma unroll
+        for (int c = 0; c < NC; ++c)
+#pragma unroll
+          for (int d = 0; d < ND; ++d) {
+            const double v = w[((a * NBS + b) * NC + c) * ND + d];
+            kc[c] += v * Dad[a][d];
+            kd[d] += v * Dac[a][c];
+          }
+      flush_row(C::NA + B0 + b, b0 + b, kc, kd);
     }
-    return;
-  }
-#pragma unroll
-  for (int a = 0; a < NAS; ++a) {
-#pragma unroll
-    for (int c = 0; c < C::NC; ++c) unsafeAtomicAdd(&Kacc[(a0 + a) * n + c0 + c], kac[a][c]);
-#pragma unroll
-    for (int d = 0; d < C::ND; ++d) unsafeAtomicAdd(&Kacc[(a0 + a) * n + d0 + d], kad[a][d]);
-  }
-#pragma unroll
-  for (int b = 0; b < NBS; ++b) {
-#pragma unroll
-    for (int c = 0; c < C::NC; ++c) unsafeAtomicAdd(&Kacc[(b0 + b) * n + c0 + c], kbc[b][c]);
-#pragma unroll
-    for (int d = 0; d < C::ND; ++d) unsafeAtomicAdd(&Kacc[(b0 + b) * n + d0 + d], kbd[b][d]);
   }
 }
 
@@ -473,7 +508,7 @@ __device__ __forceinline__ void stage_rys_table(double* lds) {
 // LDS K rows are zeroed and flushed once per run.  stage_d: the bra rows of D
 // are copied to LDS once per run as well.
 template <int LA, int LB, int LC, int LD, typename T, int MODE>
-__global__ __launch_bounds__(256) void k_eri_group(EngineView ev, const Tile* __restrict__ tiles,
+__global__ __launch_bounds__(256) PQ_WPE_ATTR void k_eri_group(EngineView ev, const Tile* __restrict__ tiles,
                                                    const int2* __restrict__ groups, int ngroups,
                                                    const int* __restrict__ kets, const double* __restrict__ D,
                                                    double* Jacc, double* Kacc, int stage_d) {
@@ -541,7 +576,7 @@ __global__ __launch_bounds__(256) void k_eri_group(EngineView ev, const Tile* __
 }
 
 template <int LA, int LB, int LC, int LD, typename T, int MODE>
-__global__ __launch_bounds__(256) void k_eri_lane(EngineView ev, const Tile* __restrict__ tiles, int ntiles,
+__global__ __launch_bounds__(256) PQ_WPE_ATTR void k_eri_lane(EngineView ev, const Tile* __restrict__ tiles, int ntiles,
                                                   const int* __restrict__ kets, const double* __restrict__ D,
                                                   double* Jacc, double* Kacc, double* ints,
                                                   double* qout, int lds_mode) {

@@ -323,6 +323,21 @@ class Engine(object):
                             ctypes.c_void_p(g.data_ptr()), _stream()))
         return g
 
+    def gradient_timed(self, D, W):
+        """gradient with per-kernel times in ms (pq_forces_timed): returns
+        (dE/dR, ms[49]): ms[2c + prec] the derivative kernel of class c over
+        its FP64 / FP32-bucket tiles, ms[42 + p] the one-electron derivative
+        kernel of pair class p, ms[48] the whole gradient."""
+        if self.cut is None:
+            self.build()
+        D = self._dtensor(D)
+        W = self._dtensor(W)
+        g = torch.empty((self.natom, 3), dtype=torch.float64, device=self.device)
+        ms = (ctypes.c_float * 49)()
+        check(lib.pq_forces_timed(self.handle, ctypes.c_void_p(D.data_ptr()), ctypes.c_void_p(W.data_ptr()),
+                                  ctypes.c_void_p(g.data_ptr()), _stream(), ms))
+        return g, list(ms)
+
     def get2ints(self, method="rys", compat_nr=0):
         """The reference's flat unique-ERI array as a device tensor:
         method "rys" (pq_get2ints, shell-quartet Rys kernels) or "hgp"
