@@ -229,8 +229,10 @@ int pq_get2ints_hgp(pq_engine *eng, double *out_dev, uint64_t len, int compat_nr
  * of shell indices (shells_host[4q..4q+3], engine shell order) a row-major
  * block ((i*nb + j)*nc + k)*nd + l over the shells' functions in canonical
  * Cartesian order, blocks concatenated in request order into out_dev
- * (out_len doubles available).  The engine keeps the flat unique array it
- * gathers from (computed on first use, dropped with the geometry). */
+ * (out_len doubles available).  Only the requested quartets are computed
+ * (store-mode class kernels over tiles built from the request, each lane
+ * writing its block in place), so no array of all integrals is formed and
+ * the request arrays are freed before returning.  Synchronises the stream. */
 int pq_shell_quartet_blocks(pq_engine *eng, int nq, const int *shells_host, double *out_dev, uint64_t out_len,
                             void *stream);
 

@@ -5,6 +5,7 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include "boys_table.h"
 #include "rys_table.h"
 #include "rys_table32.h"
 
@@ -118,6 +119,10 @@ struct EngineView {
   int64_t jp_size;
   int jp_off[6];
   int jp_n[6];
+  // shell-quartet blocks (store kernels only, else null): for the lane at
+  // list position j, blk[j] = block offset | (orientation flags << 61), see
+  // kBlk* in lane_body
+  const uint64_t* blk;
 };
 
 // pair class of shells with l1 >= l2 (ss, ps, pp, ds, dp, dd)
@@ -363,6 +368,91 @@ __device__ __forceinline__ void rys_1d(T G00, T C00, T D00, T B10, T B01, T B00,
 #pragma unroll
         for (int id = 0; id <= LD; ++id) I[ia][ib][ic][id] = Kd[id][ic];
     }
+}
+
+constexpr double kTwoPi25 = 34.98683665524972497;  // 2 pi^(5/2)
+
+// Per-primitive-quartet scalars shared by all roots.
+struct PrimQ {
+  double X, pref, rpq, qfac, pfac, i2p, i2q;
+  double PA[3], PQ[3], QC[3];
+};
+
+__device__ __forceinline__ PrimQ prim_setup(const PairInfo& bp, const PrimPair& b, const PairInfo& kp,
+                                            const PrimPair& q) {
+  PrimQ s;
+  const double pq = b.p + q.p;
+  s.PA[0] = b.Px - bp.Ax;
+  s.PA[1] = b.Py - bp.Ay;
+  s.PA[2] = b.Pz - bp.Az;
+  s.QC[0] = q.Px - kp.Ax;
+  s.QC[1] = q.Py - kp.Ay;
+  s.QC[2] = q.Pz - kp.Az;
+  s.PQ[0] = b.Px - q.Px;
+  s.PQ[1] = b.Py - q.Py;
+  s.PQ[2] = b.Pz - q.Pz;
+  const double r = rsqrt(pq);
+  const double rpq = r * r;
+  s.X = b.p * q.p * rpq * (s.PQ[0] * s.PQ[0] + s.PQ[1] * s.PQ[1] + s.PQ[2] * s.PQ[2]);
+  s.pref = kTwoPi25 * b.Kp * q.Kp * r;
+  s.rpq = rpq;
+  s.qfac = q.p * rpq;
+  s.pfac = b.p * rpq;
+  s.i2p = 0.5 * b.rp;
+  s.i2q = 0.5 * q.rp;
+  return s;
+}
+
+// Boys function values F_0(T) .. F_L(T), FP64.  Below BOYS_TASYM: the Taylor
+// series of order BOYS_K about the nearest grid point T_g (dF_m/dT =
+// -F_{m+1}), from one row of the grid table (F_0 .. F_{L+K} at T_g):
+//   F_m(T) = sum_k F_{m+k}(T_g) x^k / k!,  x = T_g - T,  |x| <= 1/8,
+// truncation below 2.9e-18 relative.  Above: the asymptotic form
+// F_0 = sqrt(pi / T) / 2, F_m = F_{m-1} (2m - 1) / (2T).  LT: tab is the
+// table's LDS copy (same layout); otherwise the global table is read.
+template <int L, bool LT = false>
+__device__ __forceinline__ void boys_grid_eval(double T, double (&F)[L + 1], const double* tab = nullptr) {
+  static_assert(L + BOYS_K < BOYS_MTAB, "Boys grid table too short for this order");
+  if (T < (double)BOYS_TASYM) {
+    const int g = (int)(T * BOYS_INV_H + 0.5);
+    const double x = (double)g * (1.0 / BOYS_INV_H) - T;
+    constexpr int NV = L + BOYS_K + 1;
+    double c[NV];
+    if constexpr (LT) {
+      const double* row = tab + g * BOYS_STRIDE;
+#pragma unroll
+      for (int i = 0; i < NV; ++i) c[i] = row[i];
+    } else {
+      constexpr int NP = (NV + 1) / 2;
+      const double2* row = reinterpret_cast<const double2*>(boys_grid + g * BOYS_STRIDE);
+#pragma unroll
+      for (int i = 0; i < NP; ++i) {
+        const double2 v = row[i];
+        c[2 * i] = v.x;
+        if (2 * i + 1 < NV) c[2 * i + 1] = v.y;
+      }
+    }
+    double xk[BOYS_K];  // x / (k + 1)
+#pragma unroll
+    for (int k = 0; k < BOYS_K; ++k) xk[k] = x * (1.0 / (k + 1));
+#pragma unroll
+    for (int m = 0; m <= L; ++m) {
+      double v = c[m + BOYS_K];
+#pragma unroll
+      for (int k = BOYS_K - 1; k >= 0; --k) v = fma(v, xk[k], c[m + k]);
+      F[m] = v;
+    }
+  } else {
+    const double rs = rsqrt(T), rx = rs * rs;
+    F[0] = 0.88622692545275801365 * rs;  // sqrt(pi) / 2
+#pragma unroll
+    for (int m = 1; m <= L; ++m) F[m] = F[m - 1] * ((m - 0.5) * rx);
+  }
+}
+
+// Copies the Boys grid table into LDS (same layout).
+__device__ __forceinline__ void stage_boys_table(double* lds) {
+  for (int i = threadIdx.x; i < BOYS_NG * BOYS_STRIDE; i += blockDim.x) lds[i] = boys_grid[i];
 }
 
 // Canonical unique-ERI index (cints.c:413-425) in 64-bit arithmetic.

@@ -141,7 +141,6 @@ struct pq_engine {
   double* d_Jacc = nullptr;  // 2 n^2: room for two densities
   double* d_Kacc = nullptr;
   double* d_D2 = nullptr;    // two stacked densities (allocated on first use)
-  double* d_flat = nullptr;  // cached flat unique-ERI array for shell-quartet blocks
   double* d_gacc = nullptr;  // private gradient copies
   size_t gacc_n = 0;
 
@@ -251,7 +250,6 @@ struct pq_engine {
     dev_free(d_Jacc);
     dev_free(d_Kacc);
     dev_free(d_D2);
-    dev_free(d_flat);
     dev_free(d_gacc);
     dev_free(d_mol_moff);
     dev_free(d_mol_n);
@@ -990,7 +988,6 @@ int pq_engine_set_geometry(pq_engine* eng, const double* atom_xyz_host) {
   PQ_HIP(hipMemcpy(eng->d_atom_xyz, atom_xyz_host, 3 * eng->natom * sizeof(double), hipMemcpyHostToDevice));
   eng->free_pairs();
   eng->free_functions();
-  dev_free(eng->d_flat);
   return PQ_OK;
 }
 
@@ -1537,40 +1534,91 @@ int pq_shell_quartet_blocks(pq_engine* eng, int nq, const int* shells_host, doub
   if (nq < 0) return fail(PQ_E_ARG, "negative quartet count");
   if (eng->nmol > 1) return fail(PQ_E_STATE, "shell-quartet blocks need a single-molecule engine");
   if (nq == 0) return PQ_OK;
-  std::vector<int> first(4 * (size_t)nq), cnt(4 * (size_t)nq);
-  std::vector<int64_t> off(nq);
+  std::vector<uint64_t> off(nq);
   uint64_t tot = 0;
   for (int q = 0; q < nq; ++q) {
     uint64_t sz = 1;
     for (int s = 0; s < 4; ++s) {
       const int sh = shells_host[4 * q + s];
       if (sh < 0 || sh >= eng->nshell) return fail(PQ_E_ARG, "shell index out of range");
-      first[4 * q + s] = eng->sh_bf[sh];
-      cnt[4 * q + s] = ncart(eng->sh_l[sh]);
-      sz *= cnt[4 * q + s];
+      sz *= ncart(eng->sh_l[sh]);
     }
-    off[q] = (int64_t)tot;
+    off[q] = tot;
     tot += sz;
   }
   if (out_len < tot) return fail(PQ_E_ARG, "output buffer too small for the requested blocks");
-  hipStream_t st = (hipStream_t)stream;
-  const uint64_t n = eng->nbf, npair = n * (n + 1) / 2, len = npair * (npair + 1) / 2;
-  if (!eng->d_flat) {
-    PQ_HIP(dev_alloc(&eng->d_flat, len));
-    const int rc = pq_get2ints(eng, eng->d_flat, len, stream);
-    if (rc) {
-      dev_free(eng->d_flat);
-      return rc;
-    }
+  if (tot >= (1ull << 61)) return fail(PQ_E_ARG, "requested blocks too large");
+  if (!eng->pairs_ok) {
+    int rc = build_pair_data(eng);
+    if (rc) return rc;
   }
-  DevBuf b_first, b_cnt, b_off;
-  PQ_HIP(hipMalloc(&b_first.p, first.size() * sizeof(int)));
-  PQ_HIP(hipMalloc(&b_cnt.p, cnt.size() * sizeof(int)));
-  PQ_HIP(hipMalloc(&b_off.p, off.size() * sizeof(int64_t)));
-  PQ_HIP(hipMemcpy(b_first.p, first.data(), first.size() * sizeof(int), hipMemcpyHostToDevice));
-  PQ_HIP(hipMemcpy(b_cnt.p, cnt.data(), cnt.size() * sizeof(int), hipMemcpyHostToDevice));
-  PQ_HIP(hipMemcpy(b_off.p, off.data(), off.size() * sizeof(int64_t), hipMemcpyHostToDevice));
-  PQ_HIP(launch_shell_blocks(nq, eng->d_flat, (int*)b_first.p, (int*)b_cnt.p, (int64_t*)b_off.p, out_dev, st));
+  // pair record of two shells (stored once per unordered pair)
+  const int ns = eng->nshell;
+  std::vector<int> pid((size_t)ns * ns, -1);
+  for (size_t p = 0; p < eng->pairs.size(); ++p) {
+    const PairInfo& pr = eng->pairs[p];
+    pid[(size_t)pr.shA * ns + pr.shB] = pid[(size_t)pr.shB * ns + pr.shA] = (int)p;
+  }
+  // one entry per request: kernel bra pair (class >= ket class), ket pair,
+  // output code; grouped by class, then bra pair
+  struct Entry {
+    int cls, bra, ket;
+    uint64_t code;
+  };
+  std::vector<int> cidx(36, -1);
+  for (int X = 0, ci = 0; X < 6; ++X)
+    for (int Y = 0; Y <= X; ++Y, ++ci) cidx[X * 6 + Y] = ci;
+  std::vector<Entry> ent(nq);
+  for (int q = 0; q < nq; ++q) {
+    const int* s = shells_host + 4 * q;
+    const int p1 = pid[(size_t)s[0] * ns + s[1]], p2 = pid[(size_t)s[2] * ns + s[3]];
+    const int c1 = eng->pcls[p1], c2 = eng->pcls[p2];
+    const bool swap = c1 < c2;  // the request's first pair becomes the ket
+    const int bra = swap ? p2 : p1, ket = swap ? p1 : p2;
+    // does the request list a pair's shells in reverse of the stored order?
+    const bool r1 = s[0] != s[1] && s[0] != eng->pairs[p1].shA;
+    const bool r2 = s[2] != s[3] && s[2] != eng->pairs[p2].shA;
+    uint64_t fl = (swap ? 1u : 0u) | (r1 ? 2u : 0u) | (r2 ? 4u : 0u);
+    ent[q] = Entry{cidx[eng->pcls[bra] * 6 + eng->pcls[ket]], bra, ket, off[q] | (fl << 61)};
+  }
+  std::stable_sort(ent.begin(), ent.end(), [](const Entry& x, const Entry& y) {
+    return x.cls != y.cls ? x.cls < y.cls : x.bra < y.bra;
+  });
+  std::vector<int> kets(nq);
+  std::vector<uint64_t> codes(nq);
+  std::vector<Tile> tiles;
+  std::vector<int> cls_t0(22, 0);
+  for (int q = 0; q < nq; ++q) {
+    kets[q] = ent[q].ket;
+    codes[q] = ent[q].code;
+    const bool start = q == 0 || ent[q].cls != ent[q - 1].cls || ent[q].bra != ent[q - 1].bra ||
+                       (tiles.back().nket & 0xff) >= 64;
+    if (start) tiles.push_back(Tile{ent[q].bra, q, 0, 0});
+    tiles.back().nket += 1;
+  }
+  for (size_t t = 0, c = 0; c <= 21; ++c) {
+    while (t < tiles.size() && ent[tiles[t].ket0].cls < (int)c) ++t;
+    cls_t0[c] = (int)t;
+  }
+  DevBuf b_t, b_k, b_c;
+  PQ_HIP(hipMalloc(&b_t.p, tiles.size() * sizeof(Tile)));
+  PQ_HIP(hipMalloc(&b_k.p, kets.size() * sizeof(int)));
+  PQ_HIP(hipMalloc(&b_c.p, codes.size() * sizeof(uint64_t)));
+  PQ_HIP(hipMemcpy(b_t.p, tiles.data(), tiles.size() * sizeof(Tile), hipMemcpyHostToDevice));
+  PQ_HIP(hipMemcpy(b_k.p, kets.data(), kets.size() * sizeof(int), hipMemcpyHostToDevice));
+  PQ_HIP(hipMemcpy(b_c.p, codes.data(), codes.size() * sizeof(uint64_t), hipMemcpyHostToDevice));
+  EngineView v = eng->view();
+  v.blk = (const uint64_t*)b_c.p;
+  hipStream_t st = (hipStream_t)stream;
+  for (int X = 0, ci = 0; X < 6; ++X)
+    for (int Y = 0; Y <= X; ++Y, ++ci) {
+      const int t0 = cls_t0[ci], nt = cls_t0[ci + 1] - t0;
+      if (nt <= 0) continue;
+      PQ_HIP(launch_eri_class(kPairL[X][0], kPairL[X][1], kPairL[Y][0], kPairL[Y][1], MODE_STORE, 0, v,
+                              (const Tile*)b_t.p + t0, nt, (const int*)b_k.p, nullptr, nullptr, nullptr, out_dev,
+                              nullptr, st));
+    }
+  // the request arrays are freed on return
   PQ_HIP(hipStreamSynchronize(st));
   return PQ_OK;
 }

@@ -168,31 +168,6 @@ hipError_t launch_unpack_eri(int n, const double* ints, int mu0, int nmu, double
   return hipGetLastError();
 }
 
-// Shell-quartet blocks in the libint layout (clibint.h:37-52): block q is
-// row-major ((i nb + j) nc + k) nd + l over the functions of its four shells
-// (first function first[4q+s], count cnt[4q+s]; the canonical Cartesian
-// order, which equals the PyQuante order for s, p, d), gathered from the
-// flat unique array.  One workgroup per quartet.
-__global__ void k_shell_blocks(const double* __restrict__ ints, const int* __restrict__ first,
-                               const int* __restrict__ cnt, const int64_t* __restrict__ off, double* out) {
-  const int q = blockIdx.x;
-  const int fa = first[4 * q], fb = first[4 * q + 1], fc = first[4 * q + 2], fd = first[4 * q + 3];
-  const int na = cnt[4 * q], nb = cnt[4 * q + 1], nc = cnt[4 * q + 2], nd = cnt[4 * q + 3];
-  const int tot = na * nb * nc * nd;
-  double* o = out + off[q];
-  for (int t = threadIdx.x; t < tot; t += blockDim.x) {
-    const int l = t % nd, k = (t / nd) % nc, j = (t / (nd * nc)) % nb, i = t / (nd * nc * nb);
-    o[t] = ints[ijkl2intindex(fa + i, fb + j, fc + k, fd + l)];
-  }
-}
-
-hipError_t launch_shell_blocks(int nq, const double* ints, const int* first, const int* cnt, const int64_t* off,
-                               double* out, hipStream_t st) {
-  if (nq <= 0) return hipSuccess;
-  hipLaunchKernelGGL(k_shell_blocks, dim3(nq), dim3(64), 0, st, ints, first, cnt, off, out);
-  return hipGetLastError();
-}
-
 static hipError_t launch_diag_0000(const EngineView& ev, const Tile* tiles, int ntiles, const int* kets, const double* D,
                                    double* Jacc, double* Kacc, double* ints, double* qout, hipStream_t st) {
   return launch_cls_0000_m3(ev, tiles, ntiles, kets, D, Jacc, Kacc, ints, qout, st, nullptr, 0, 0);

@@ -22,15 +22,10 @@
 #include <type_traits>
 
 #include "pq_device.h"
+#include "pq_hgp_core.h"
 #include "pq_kernels.h"
 
-#ifndef PQ_WPE_ATTR
-#define PQ_WPE_ATTR
-#endif
-
 namespace pq {
-
-constexpr double kTwoPi25 = 34.98683665524972497;  // 2 pi^(5/2)
 
 template <int LA, int LB, int LC, int LD>
 struct Cls {
@@ -42,37 +37,6 @@ struct Cls {
   static constexpr int SPLIT = NCOMP <= kLaneMaxComp ? 0 : (NCOMP / NA <= kLaneMaxComp ? 1 : 2);
   static constexpr int NSPLIT = SPLIT == 0 ? 1 : (SPLIT == 1 ? NA : NA * NB);
 };
-
-// Per-primitive-quartet scalars shared by all roots.
-struct PrimQ {
-  double X, pref, rpq, qfac, pfac, i2p, i2q;
-  double PA[3], PQ[3], QC[3];
-};
-
-__device__ __forceinline__ PrimQ prim_setup(const PairInfo& bp, const PrimPair& b, const PairInfo& kp,
-                                            const PrimPair& q) {
-  PrimQ s;
-  const double pq = b.p + q.p;
-  s.PA[0] = b.Px - bp.Ax;
-  s.PA[1] = b.Py - bp.Ay;
-  s.PA[2] = b.Pz - bp.Az;
-  s.QC[0] = q.Px - kp.Ax;
-  s.QC[1] = q.Py - kp.Ay;
-  s.QC[2] = q.Pz - kp.Az;
-  s.PQ[0] = b.Px - q.Px;
-  s.PQ[1] = b.Py - q.Py;
-  s.PQ[2] = b.Pz - q.Pz;
-  const double r = rsqrt(pq);
-  const double rpq = r * r;
-  s.X = b.p * q.p * rpq * (s.PQ[0] * s.PQ[0] + s.PQ[1] * s.PQ[1] + s.PQ[2] * s.PQ[2]);
-  s.pref = kTwoPi25 * b.Kp * q.Kp * r;
-  s.rpq = rpq;
-  s.qfac = q.p * rpq;
-  s.pfac = b.p * rpq;
-  s.i2p = 0.5 * b.rp;
-  s.i2q = 0.5 * q.rp;
-  return s;
-}
 
 // Contracted block of one shell quartet for bra components a in [A0, A0+NAS),
 // b in [B0, B0+NBS) and every ket component; PyQuante component order, without
@@ -154,6 +118,51 @@ __device__ __forceinline__ void eri_block(const EngineView& ev, const PairInfo& 
       for (int i = i0; i < i1; ++i) quartet(bra_prim(i), q);
     }
   }
+}
+
+// ---------------------------------------------------------------------
+// The J/K classes of total angular momentum 2 ((ps|ps), (pp|ss), (ds|ss)):
+// Head-Gordon-Pople recurrences instead of Rys quadrature.  Per primitive
+// quartet the Boys values F_0..F_2 come from the grid table (staged in LDS
+// by the bra-run kernels; 13 doubles and 30 FMAs) and the Obara-Saika
+// vertical recurrence builds the [e0|f0] targets (tables of 15-18 entries
+// in registers); those are summed over the contraction and the horizontal
+// recurrence runs once per contracted block.  Two-root Rys quadrature reads
+// 48 table coefficients and spends 44 FMAs on the roots alone.  Measured on
+// (H2O)_8 (kernel time inside the concurrent build): (pp|ss) 0.206 ->
+// 0.148 ms, (ds|ss) 0.102 -> 0.093, (ps|ps) 0.309 -> 0.303.  With one root
+// (L <= 1) the two cost the same per primitive quartet and the Rys kernels
+// hold fewer registers ((ss|ss) 0.117 against 0.128 ms), so they stay Rys.
+// Same block layout as eri_block; FP64 only (the FP32 tiles of mixed
+// builds keep the Rys path).
+// ---------------------------------------------------------------------
+template <int LA, int LB, int LC, int LD, typename T, int MODE>
+constexpr bool kHgpJK = mode_is_jk(MODE) && std::is_same<T, double>::value && LA + LB + LC + LD == 2 &&
+                        HgpCls<LA, LB, LC, LD>::SIZE <= kHgpUnrollMax;
+
+template <int LA, int LB, int LC, int LD, bool LT>
+__device__ __forceinline__ void hgp_jk_block(const EngineView& ev, const PairInfo& bp, const PairInfo& kp, int i0,
+                                             int i1, int k0, int k1,
+                                             double (&out)[ncart(LA) * ncart(LB) * ncart(LC) * ncart(LD)],
+                                             const double* btab) {
+  using H = HgpCls<LA, LB, LC, LD>;
+  double acc[H::NEA * H::NFC];
+#pragma unroll
+  for (int c = 0; c < H::NEA * H::NFC; ++c) acc[c] = 0.0;
+  if (k0 < k1) {
+    // the next ket primitive pair loads while the bra loop runs
+    PrimPair q = ev.prims[kp.pp_off + k0];
+#pragma unroll 1
+    for (int k = k0; k < k1; ++k) {
+      const PrimPair qn = ev.prims[kp.pp_off + (k + 1 < k1 ? k + 1 : k)];
+#pragma unroll 1
+      for (int i = i0; i < i1; ++i)
+        hgp_vrr<LA, LB, LC, LD, true, LT>(bp, ld_uniform(ev.prims + bp.pp_off + i), kp, q, acc, btab);
+      q = qn;
+    }
+  }
+  const double AB[3] = {bp.ABx, bp.ABy, bp.ABz}, CD[3] = {kp.ABx, kp.ABy, kp.ABz};
+  hgp_hrr_block<LA, LB, LC, LD>(acc, AB, CD, out);
 }
 
 // ---------------------------------------------------------------------
@@ -389,8 +398,14 @@ __device__ __forceinline__ void lane_body(const EngineView& ev, const Tile& tl, 
   if (active) {
     const int i0 = tl.prim & 0xffff, ni = tl.prim >> 16;
     const int k0 = kc * kp.pp_n / ks, k1 = (kc + 1) * kp.pp_n / ks;
-    eri_block<LA, LB, LC, LD, A0, NAS, B0, NBS, T, UB, LT>(ev, bp, kp, i0, ni ? i0 + ni : bp.pp_n, k0, k1, out,
-                                                           rtab);
+    if constexpr (kHgpJK<LA, LB, LC, LD, T, MODE>) {
+      static_assert(NAS == C::NA && NBS == C::NB, "the HGP J/K classes are unsplit");
+      hgp_jk_block<LA, LB, LC, LD, LT>(ev, bp, kp, i0, ni ? i0 + ni : bp.pp_n, k0, k1, out,
+                                       static_cast<const double*>(rtab));
+    } else {
+      eri_block<LA, LB, LC, LD, A0, NAS, B0, NBS, T, UB, LT>(ev, bp, kp, i0, ni ? i0 + ni : bp.pp_n, k0, k1, out,
+                                                             rtab);
+    }
   } else {
 #pragma unroll
     for (int c = 0; c < NO; ++c) out[c] = (T)0;
@@ -422,6 +437,46 @@ __device__ __forceinline__ void lane_body(const EngineView& ev, const Tile& tl, 
           ev.jpair + d * ev.jp_size + ev.jp_off[Y], ev.jp_n[Y], kpos);
   } else if constexpr (MODE == MODE_STORE) {
     if (!active) return;
+    if (ev.blk) {
+      // one requested shell quartet (s1 s2|s3 s4) in the libint layout,
+      // row-major ((i n2 + j) n3 + k) n4 + l: flags say whether the request's
+      // first pair is this lane's ket pair (the kernels need bra class >=
+      // ket class) and whether either requested pair lists its shells in the
+      // reverse of the stored order
+      const uint64_t code = ev.blk[kpos];
+      const int64_t off = (int64_t)(code & ((1ull << 61) - 1));
+      const int fl = (int)(code >> 61);
+#pragma unroll
+      for (int a = 0; a < NAS; ++a)
+#pragma unroll
+        for (int b = 0; b < NBS; ++b)
+#pragma unroll
+          for (int c = 0; c < C::NC; ++c)
+#pragma unroll
+            for (int d = 0; d < C::ND; ++d) {
+              int x1 = A0 + a, x2 = B0 + b, y1 = c, y2 = d;
+              int n1 = C::NA, n2 = C::NB, m1 = C::NC, m2 = C::ND;
+              if (fl & 1) {
+                int t = x1; x1 = y1; y1 = t;
+                t = x2; x2 = y2; y2 = t;
+                t = n1; n1 = m1; m1 = t;
+                t = n2; n2 = m2; m2 = t;
+              }
+              if (fl & 2) {
+                int t = x1; x1 = x2; x2 = t;
+                t = n1; n1 = n2; n2 = t;
+              }
+              if (fl & 4) {
+                int t = y1; y1 = y2; y2 = t;
+                t = m1; m1 = m2; m2 = t;
+              }
+              const double nf =
+                  cart_norm(LA, A0 + a) * cart_norm(LB, B0 + b) * cart_norm(LC, c) * cart_norm(LD, d);
+              ints[off + ((x1 * n2 + x2) * m1 + y1) * m2 + y2] =
+                  (double)out[((a * NBS + b) * C::NC + c) * C::ND + d] * nf;
+            }
+      return;
+    }
     const bool same = bra == ket;
 #pragma unroll
     for (int a = 0; a < NAS; ++a)
@@ -481,6 +536,11 @@ template <int NR, typename T>
 constexpr int kLdsTableDoubles = std::is_same<T, double>::value
                                      ? (NR <= 3 ? kRysIntervals<NR> * kRysLdsStride<NR> : 0)
                                      : kRysIntervals32<NR> * kRysLdsStride32<NR> / 2;
+// per class and mode: the Boys grid table for the HGP J/K classes
+template <int LA, int LB, int LC, int LD, typename T, int MODE>
+constexpr int kGroupTableDoubles = kHgpJK<LA, LB, LC, LD, T, MODE>
+                                       ? BOYS_NG * BOYS_STRIDE
+                                       : kLdsTableDoubles<Cls<LA, LB, LC, LD>::NR, T>;
 
 // Copies the order-NR table into LDS with the padded interval stride.
 template <int NR, typename T>
@@ -508,7 +568,7 @@ __device__ __forceinline__ void stage_rys_table(double* lds) {
 // LDS K rows are zeroed and flushed once per run.  stage_d: the bra rows of D
 // are copied to LDS once per run as well.
 template <int LA, int LB, int LC, int LD, typename T, int MODE>
-__global__ __launch_bounds__(256) PQ_WPE_ATTR void k_eri_group(EngineView ev, const Tile* __restrict__ tiles,
+__global__ __launch_bounds__(256) void k_eri_group(EngineView ev, const Tile* __restrict__ tiles,
                                                    const int2* __restrict__ groups, int ngroups,
                                                    const int* __restrict__ kets, const double* __restrict__ D,
                                                    double* Jacc, double* Kacc, int stage_d) {
@@ -521,9 +581,10 @@ __global__ __launch_bounds__(256) PQ_WPE_ATTR void k_eri_group(EngineView ev, co
   const int2 g = groups[blockIdx.x];
   // light FP64 classes read the root table from LDS (one copy per block)
   // instead of gathering it from L1/L2 per lane
-  constexpr int NT = kLdsTableDoubles<C::NR, T>;
+  constexpr int NT = kGroupTableDoubles<LA, LB, LC, LD, T, MODE>;
   void* rtab = NT ? pq_lds : nullptr;
-  if constexpr (NT > 0) stage_rys_table<C::NR, T>(pq_lds);
+  if constexpr (kHgpJK<LA, LB, LC, LD, T, MODE>) stage_boys_table(pq_lds);
+  else if constexpr (NT > 0) stage_rys_table<C::NR, T>(pq_lds);
   constexpr int ND = mode_ndens(MODE);
   double* krow = pq_lds + NT;
   for (int i = threadIdx.x; i < ND * (C::NA + C::NB) * n; i += blockDim.x) krow[i] = 0.0;
@@ -576,7 +637,7 @@ __global__ __launch_bounds__(256) PQ_WPE_ATTR void k_eri_group(EngineView ev, co
 }
 
 template <int LA, int LB, int LC, int LD, typename T, int MODE>
-__global__ __launch_bounds__(256) PQ_WPE_ATTR void k_eri_lane(EngineView ev, const Tile* __restrict__ tiles, int ntiles,
+__global__ __launch_bounds__(256) void k_eri_lane(EngineView ev, const Tile* __restrict__ tiles, int ntiles,
                                                   const int* __restrict__ kets, const double* __restrict__ D,
                                                   double* Jacc, double* Kacc, double* ints,
                                                   double* qout, int lds_mode) {
@@ -673,7 +734,7 @@ static hipError_t launch_one(const EngineView& ev, const Tile* tiles, int ntiles
   const size_t lds = mode_ndens(MODE) * ((size_t)(C::NA + C::NB) * ev.nbf + (C::SPLIT ? 64 * C::NC * C::ND : 0)) *
                      sizeof(double);
   if constexpr (C::SPLIT == 0 && mode_is_jk(MODE)) {
-    const size_t glds = lds + kLdsTableDoubles<C::NR, T> * sizeof(double);
+    const size_t glds = lds + kGroupTableDoubles<LA, LB, LC, LD, T, MODE> * sizeof(double);
     // plus the bra J block; the staged D rows take as much LDS as the K rows
     const size_t jlds = mode_ndens(MODE) * C::NA * C::NB * sizeof(double);
     const int stage_d = (gflags & kGroupStageD) && glds + jlds + lds <= 65536;

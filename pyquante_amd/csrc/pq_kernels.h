@@ -106,9 +106,6 @@ hipError_t launch_hgp_store(uint64_t len, const int* fnp, const int* foff, const
                             double* out, hipStream_t st);
 // rows [mu0, mu0+nmu) of the full (mu nu|lam sig) tensor from the flat array
 hipError_t launch_unpack_eri(int n, const double* ints, int mu0, int nmu, double* out, hipStream_t st);
-// shell-quartet blocks (libint layout) gathered from the flat array
-hipError_t launch_shell_blocks(int nq, const double* ints, const int* first, const int* cnt, const int64_t* off,
-                               double* out, hipStream_t st);
 hipError_t launch_boys(int n, const double* m, const double* x, double eps, double* out, hipStream_t st);
 hipError_t launch_rys_roots(int n, int nr, const double* X, double* u, double* w, hipStream_t st);
 hipError_t launch_contr_eri(int n, const int* nprim, const int* poff, const double* exps,
