@@ -211,19 +211,60 @@ def _timed(fn, steps):
 
 
 def extra_workloads(args):
-    """Mixed precision on (H2O)_16 and forces on (H2O)_8 (outside the
-    headline timed region; reported alongside it)."""
-    import numpy as np
+    """The workloads reported beside the headline figure (outside its timed
+    region): (H2O)_16 FP64 against mixed precision, forces on (H2O)_8, the
+    two-density pass, MP2, the batched workload and the HGP path; --only
+    selects a subset."""
     import torch
 
-    from pyquante_amd import _abi
     from pyquante_amd.basis import BasisSet
     from pyquante_amd.engine import Engine
     from pyquante_amd.force import scf
     from pyquante_amd.molecule import water_cluster
 
     out = {}
-    # (H2O)_16: FP64 vs mixed (quartets with Schwarz bound < 1e-4 in FP32)
+    only = set(args.only.split(",")) if args.only else None
+
+    def want(name):
+        return only is None or name in only
+
+    if want("h2o16"):
+        out.update(h2o16_workload(args))
+    if not any(want(w) for w in ("forces", "multi", "mp2", "hgp", "batch")):
+        return out
+    # (H2O)_8: converged RHF for the forces, the two-density pass, MP2 and HGP
+    mol = water_cluster((2, 2, 2), seed=args.seed)
+    eng = Engine(BasisSet(mol, "6-31g**"), mol)
+    eng.build(1e-12, 0.0)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    e8, orbe8, C8, D, W = scf(eng, mol, ConvCriteria=1e-9, graph=True)
+    torch.cuda.synchronize()
+    out["h2o8_rhf_scf_graph_s"] = time.perf_counter() - t0
+    if want("forces"):
+        out.update(forces_workload(args, eng, D, W))
+    if want("multi"):
+        # two densities in one pass (the unrestricted J/K) against two builds
+        Db_ = (0.9 * D + 0.1 * torch.eye(D.shape[0], dtype=D.dtype, device=D.device)).contiguous()
+        t2 = _timed(lambda: eng.fock_jk_multi([D, Db_]), args.steps)
+        t1 = _timed(lambda: (eng.fock_jk(D), eng.fock_jk(Db_)), args.steps)
+        out.update({"h2o8_two_density_ms": t2 * 1e3, "h2o8_two_single_builds_ms": t1 * 1e3})
+        print("bench: (H2O)_8 J/K of two densities %.2f ms in one pass, %.2f ms as two builds" % (t2 * 1e3, t1 * 1e3),
+              file=sys.stderr, flush=True)
+    out.update(extra_h2o8_stored(args, mol, eng, e8, orbe8, C8, want))
+    return out
+
+
+def h2o16_workload(args):
+    """(H2O)_16 6-31G**: graph-replayed Fock builds in FP64 and in mixed
+    precision (quartets with Schwarz bound < 1e-4 in FP32), with the RHF
+    energies of both."""
+    from pyquante_amd import _abi
+    from pyquante_amd.basis import BasisSet
+    from pyquante_amd.engine import Engine
+    from pyquante_amd.force import scf
+    from pyquante_amd.molecule import water_cluster
+    out = {}
     mol = water_cluster((2, 2, 4), seed=args.seed)
     eng = Engine(BasisSet(mol, "6-31g**"), mol)
     eng.build(1e-12, 0.0)
@@ -245,16 +286,13 @@ def extra_workloads(args):
                 "h2o16_mixed_energy_error_eh": abs(emx - e64)})
     print("bench: (H2O)_16 nbf=%d FP64 %.2f ms, mixed %.2f ms (x%.2f), |dE| = %.2e Eh"
           % (eng.nbf, t64 * 1e3, tmx * 1e3, t64 / tmx, abs(emx - e64)), file=sys.stderr, flush=True)
-    del eng
-    # forces on (H2O)_8 from the converged density
-    mol = water_cluster((2, 2, 2), seed=args.seed)
-    eng = Engine(BasisSet(mol, "6-31g**"), mol)
-    eng.build(1e-12, 0.0)
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    e8, orbe8, C8, D, W = scf(eng, mol, ConvCriteria=1e-9, graph=True)
-    torch.cuda.synchronize()
-    out["h2o8_rhf_scf_graph_s"] = time.perf_counter() - t0
+    return out
+
+
+def forces_workload(args, eng, D, W):
+    """Forces per second on (H2O)_8 from the converged density, and the
+    roofline of the dominant derivative kernel."""
+    out = {}
     tg = _timed(lambda: eng.gradient(D, W), args.steps)
     out.update({"h2o8_ms_per_gradient": tg * 1e3, "h2o8_forces_per_s": 1.0 / tg})
     print("bench: (H2O)_8 gradient %.2f ms (%.1f forces/s)" % (tg * 1e3, 1.0 / tg), file=sys.stderr, flush=True)
@@ -287,15 +325,16 @@ def extra_workloads(args):
     print("bench: (H2O)_8 gradient dominant kernel %s %.3f ms, %.2f TFLOP/s (%.3f of peak); all %.2f TFLOP/s"
           % (gname, kt[k], ach, ach / FP64_PEAK_TFLOPS, out["roofline_forces"]["all_kernels_tflops"]),
           file=sys.stderr, flush=True)
-    # two densities in one pass (the unrestricted J/K) against two builds
-    Db_ = (0.9 * D + 0.1 * torch.eye(D.shape[0], dtype=D.dtype, device=D.device)).contiguous()
-    t2 = _timed(lambda: eng.fock_jk_multi([D, Db_]), args.steps)
-    t1 = _timed(lambda: (eng.fock_jk(D), eng.fock_jk(Db_)), args.steps)
-    out.update({"h2o8_two_density_ms": t2 * 1e3, "h2o8_two_single_builds_ms": t1 * 1e3})
-    print("bench: (H2O)_8 J/K of two densities %.2f ms in one pass, %.2f ms as two builds" % (t2 * 1e3, t1 * 1e3),
-          file=sys.stderr, flush=True)
+    return out
+
+
+def extra_h2o8_stored(args, mol, eng, e8, orbe8, C8, want):
+    """MP2 on (H2O)_8, the batched workload and the HGP path."""
+    import numpy as np
+    import torch
+    out = {}
     # MP2 on (H2O)_8: stored ERIs, row-block unpacking and device GEMMs
-    if not args.no_mp2:
+    if want("mp2") and not args.no_mp2:
         from pyquante_amd.Ints import Integrals
         from pyquante_amd.MP import MP2
         nocc = mol.get_closedopen()[0]
@@ -316,12 +355,12 @@ def extra_workloads(args):
         print("bench: (H2O)_8 MP2 %.3f s (E2 = %.8f Eh, transform %.0f GFLOP)" % (tm, e2, gflop),
               file=sys.stderr, flush=True)
         del ints
-    if args.batch_molecules > 0:
+    if want("batch") and args.batch_molecules > 0:
         out.update(batch_workload(args))
     # HGP path: every unique function quartet of (H2O)_8 by the GPU HGP
     # kernel, against the Rys kernels' array; RHF through the stored HGP
     # integrals against the direct Rys RHF
-    if not args.no_hgp:
+    if want("hgp") and not args.no_hgp:
         from pyquante_amd.hartree_fock import rhf
         rys = eng.get2ints("rys")
         hgp = eng.get2ints("hgp", 0)  # first call: kernel loading
@@ -439,6 +478,9 @@ def main():
     ap.add_argument("--no-extra", action="store_true", help="skip the (H2O)_16, forces and HGP workloads")
     ap.add_argument("--no-hgp", action="store_true", help="skip the HGP workload")
     ap.add_argument("--no-mp2", action="store_true", help="skip the MP2 workload")
+    ap.add_argument("--only", default="",
+                    help="comma-separated subset of the extra workloads to run (h2o16, forces, multi, mp2, batch, "
+                         "hgp); default all")
     ap.add_argument("--batch-molecules", type=int, default=512,
                     help="molecules of the batched workload (0 skips it)")
     ap.add_argument("--streams", type=int, default=4, help="engine streams for the class kernels (0-4)")
