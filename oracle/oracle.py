@@ -28,7 +28,9 @@ def _pi(a):
 
 
 def load():
-    path = os.path.join(HERE, "liboracle.so")
+    # PQ_ORACLE_LIB selects another build of the same source (the sanitizer
+    # build, liboracle_san.so)
+    path = os.path.join(HERE, os.environ.get("PQ_ORACLE_LIB", "liboracle.so"))
     if not os.path.exists(path):
         raise RuntimeError("oracle not built: run make -C oracle")
     lib = ctypes.CDLL(path)
