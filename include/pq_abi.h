@@ -165,6 +165,14 @@ int pq_engine_set_prim_screen(pq_engine *eng, double rel);
  * all.  No reference counterpart (a launch knob). */
 int pq_engine_set_streams(pq_engine *eng, int nstreams);
 
+/* Dynamic LDS the J/K launches may use, bytes: 0 restores the device's
+ * per-workgroup limit (queried at creation, 160 KB on gfx950), a smaller
+ * value lowers it (clamped to the device's); device_bytes (optional)
+ * receives the device limit.  Past the limit a class keeps the bra rows of
+ * K in global memory (scattered atomics) instead of LDS.  No reference
+ * counterpart (a launch knob). */
+int pq_engine_set_lds_limit(pq_engine *eng, int bytes, int *device_bytes);
+
 /* Bra runs: for the J/K kernels of classes with at most 108 Cartesian
  * components, one 256-thread block walks up to `cap` consecutive tiles that
  * share a bra shell pair and flushes its LDS copy of the bra K rows once.

@@ -120,9 +120,12 @@ struct EngineView {
   int jp_off[6];
   int jp_n[6];
   // shell-quartet blocks (store kernels only, else null): for the lane at
-  // list position j, blk[j] = block offset | (orientation flags << 61), see
-  // kBlk* in lane_body
+  // list position j, blk[j] = block offset | (orientation flags << 61) (the
+  // store branch of lane_body)
   const uint64_t* blk;
+  // dynamic LDS a J/K launch may request, bytes (the device's per-workgroup
+  // limit unless lowered with pq_engine_set_lds_limit)
+  int lds_max;
 };
 
 // pair class of shells with l1 >= l2 (ss, ps, pp, ds, dp, dd)

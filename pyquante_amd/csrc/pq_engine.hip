@@ -175,11 +175,18 @@ struct pq_engine {
   int* d_atom_mol = nullptr;
 
   int maxat = 0;      // largest molecule's atom count
-  EngineView view() const { return EngineView{d_pairs, d_prims, d_prim_a, maxn, msize, maxat}; }
+  int lds_dev = 65536;  // the device's LDS per workgroup, bytes
+  int lds_max = 65536;  // the J/K launches' limit (lds_dev unless lowered)
+  EngineView view() const {
+    EngineView v{d_pairs, d_prims, d_prim_a, maxn, msize, maxat};
+    v.lds_max = lds_max;
+    return v;
+  }
   // pairs with the screened primitive counts and the ket J (J/K and
   // gradient kernels)
   EngineView view_s() const {
     EngineView v{d_pairs_s, d_prims, d_prim_a, maxn, msize, maxat};
+    v.lds_max = lds_max;
     v.jpair = d_jpair;
     v.jp_size = jp_size;
     for (int y = 0; y < 6; ++y) {
@@ -947,6 +954,15 @@ int pq_engine_create_batch(int nshell, const int* shell_l, const int* shell_npri
     delete e;
     return fail(PQ_E_HIP, "copy of nuclei failed");
   }
+  // LDS per workgroup of the current device (160 KB on gfx950): the J/K
+  // launches keep the bra rows of K in LDS up to that size
+  {
+    int dev = 0, lds = 0;
+    if (hipGetDevice(&dev) == hipSuccess &&
+        hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) == hipSuccess && lds > 0) {
+      e->lds_dev = e->lds_max = lds;
+    }
+  }
   *out = e;
   return PQ_OK;
 }
@@ -1012,6 +1028,18 @@ int pq_engine_set_prim_screen(pq_engine* eng, double rel) {
   if (!eng) return fail(PQ_E_ARG, "null engine");
   if (!(rel >= 0.0) || rel > 1.0) return fail(PQ_E_ARG, "rel must be in [0, 1]");
   eng->prim_rel = rel;
+  return PQ_OK;
+}
+
+int pq_engine_set_lds_limit(pq_engine* eng, int bytes, int* device_bytes) {
+  if (!eng) return fail(PQ_E_ARG, "null engine");
+  if (bytes < 0) return fail(PQ_E_ARG, "bytes must be >= 0");
+  if (device_bytes) *device_bytes = eng->lds_dev;
+  const int lim = bytes == 0 ? eng->lds_dev : std::min(bytes, eng->lds_dev);
+  if (lim != eng->lds_max) {
+    eng->lds_max = lim;
+    ++eng->gen;  // captured builds hold launches sized for the old limit
+  }
   return PQ_OK;
 }
 

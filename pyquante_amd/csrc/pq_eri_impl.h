@@ -737,15 +737,16 @@ static hipError_t launch_one(const EngineView& ev, const Tile* tiles, int ntiles
     const size_t glds = lds + kGroupTableDoubles<LA, LB, LC, LD, T, MODE> * sizeof(double);
     // plus the bra J block; the staged D rows take as much LDS as the K rows
     const size_t jlds = mode_ndens(MODE) * C::NA * C::NB * sizeof(double);
-    const int stage_d = (gflags & kGroupStageD) && glds + jlds + lds <= 65536;
-    if (groups && ngroups > 0 && glds + jlds <= 65536) {
+    const size_t lmax = (size_t)ev.lds_max;
+    const int stage_d = (gflags & kGroupStageD) && glds + jlds + lds <= lmax;
+    if (groups && ngroups > 0 && glds + jlds <= lmax) {
       hipLaunchKernelGGL((k_eri_group<LA, LB, LC, LD, T, MODE>), dim3(ngroups), dim3(256),
                          glds + jlds + (stage_d ? lds : 0), st, ev, tiles, groups, ngroups, kets, D, Jacc, Kacc,
                          stage_d);
       return hipGetLastError();
     }
   }
-  if (mode_is_jk(MODE) && lds <= 65536) {
+  if (mode_is_jk(MODE) && lds <= (size_t)ev.lds_max) {
     const int waves = C::SPLIT ? kWaveSplit : 1;
     const int chunks = (C::NSPLIT + waves - 1) / waves;
     hipLaunchKernelGGL((k_eri_lane<LA, LB, LC, LD, T, MODE>), dim3(ntiles, chunks), dim3(64 * waves), lds, st, ev,

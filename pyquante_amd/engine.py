@@ -130,6 +130,13 @@ class Engine(object):
         """Spread class kernels over n engine streams (0 = caller's stream only)."""
         check(lib.pq_engine_set_streams(self.handle, int(n)))
 
+    def set_lds_limit(self, nbytes=0):
+        """Dynamic LDS limit of the J/K launches (pq_engine_set_lds_limit;
+        0 = the device's); returns the device's per-workgroup LDS bytes."""
+        dev = ctypes.c_int()
+        check(lib.pq_engine_set_lds_limit(self.handle, int(nbytes), ctypes.byref(dev)))
+        return dev.value
+
     def set_prim_screen(self, rel):
         """Primitive screening of the J/K and gradient work relative to the
         Schwarz cut (pq_engine_set_prim_screen; 0 = every primitive); applies

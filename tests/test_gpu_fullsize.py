@@ -93,3 +93,26 @@ def test_h2o8_hgp_full_array_matches_rys():
     scale = float(J.abs().max())
     assert float((Js - J).abs().max()) < 1e-10 * scale
     assert float((Ks - K).abs().max()) < 1e-10 * scale
+
+
+def test_lds_limit_past_680_functions():
+    """(H2O)_28 6-31G** (700 functions): the d-bra classes need more than
+    64 KB for their LDS K rows.  With the device's limit (queried, 160 KB on
+    gfx950) they keep them in LDS; lowered to 64 KB they fall back to global
+    atomics.  The two accumulation paths give the same J and K."""
+    eng = _engine((2, 2, 7))
+    assert eng.nbf == 700
+    dev = eng.set_lds_limit(0)
+    print("device LDS per workgroup: %d bytes" % dev, flush=True)
+    assert dev >= 65536
+    eng.build(1e-12)
+    D = _density(eng.nbf, 60, 11)
+    J1, K1 = eng.fock_jk(D)
+    eng.set_lds_limit(65536)
+    J2, K2 = eng.fock_jk(D)
+    eng.set_lds_limit(0)
+    scale = float(J1.abs().max())
+    dj = float((J1 - J2).abs().max())
+    dk = float((K1 - K2).abs().max())
+    print("(H2O)_28: LDS K rows vs global atomics: max |dJ| %.2e, |dK| %.2e (scale %.1f)" % (dj, dk, scale), flush=True)
+    assert dj < 1e-11 * scale and dk < 1e-11 * scale
