@@ -165,6 +165,14 @@ int pq_engine_set_prim_screen(pq_engine *eng, double rel);
  * all.  No reference counterpart (a launch knob). */
 int pq_engine_set_streams(pq_engine *eng, int nstreams);
 
+/* on = 1: the FP64 bra runs of the light classes (ss|ss), (ps|ss), (ps|ps),
+ * (pp|ss) and (ds|ss) go into one launch (a segment table maps each block
+ * to its class; heaviest class first) instead of one launch per class; 0
+ * (default) restores the per-class launches.  The fused kernel carries the
+ * registers and LDS of its largest member.  Single-density FP64 builds
+ * only.  No reference counterpart (a launch knob). */
+int pq_engine_set_fused(pq_engine *eng, int on);
+
 /* Dynamic LDS the J/K launches may use, bytes: 0 restores the device's
  * per-workgroup limit (queried at creation, 160 KB on gfx950), a smaller
  * value lowers it (clamped to the device's); device_bytes (optional)

@@ -150,6 +150,7 @@ struct pq_engine {
   hipEvent_t fork_ev = nullptr, join_ev[kSide] = {};
   bool streams_ok = false;
   int nside = kSide;  // streams in use (0 = caller's stream only)
+  bool fused = false;  // one launch for the light classes' bra runs (pq_engine_set_fused)
 
   // tiles per bra run for each (bra class, ket class, precision): -1 until
   // set by pq_engine_autotune or pq_engine_set_group_cap
@@ -1043,6 +1044,15 @@ int pq_engine_set_lds_limit(pq_engine* eng, int bytes, int* device_bytes) {
   return PQ_OK;
 }
 
+int pq_engine_set_fused(pq_engine* eng, int on) {
+  if (!eng) return fail(PQ_E_ARG, "null engine");
+  if ((on != 0) != eng->fused) {
+    eng->fused = on != 0;
+    ++eng->gen;  // a captured build holds the other launch pattern
+  }
+  return PQ_OK;
+}
+
 int pq_engine_set_streams(pq_engine* eng, int nstreams) {
   if (!eng) return fail(PQ_E_ARG, "null engine");
   if (nstreams < 0 || nstreams > pq_engine::kSide) return fail(PQ_E_ARG, "nstreams must be 0..4");
@@ -1415,12 +1425,53 @@ static int fock_jk_impl(pq_engine* eng, int nd, const double* const* Ds, double*
     const int nside = eng->nside;
     PQ_TRY(fork_side(eng, st));
     int next = 0;
+    // fused launch: the FP64 bra runs of the light classes in one grid,
+    // segments ordered by their Rys operation count (heaviest first)
+    std::vector<char> in_multi(eng->work.size(), 0);
+    if (eng->fused && nd == 1) {
+      std::vector<size_t> mc;
+      for (size_t ci = 0; ci < eng->work.size(); ++ci) {
+        const auto& c = eng->work[ci];
+        const int la = kPairL[c.X][0], lb = kPairL[c.X][1], lc = kPairL[c.Y][0], ld = kPairL[c.Y][1];
+        if (jk_multi_class(la, lb, lc, ld) && !c.t64.empty() && !c.g64.empty()) mc.push_back(ci);
+      }
+      std::sort(mc.begin(), mc.end(), [&](size_t a, size_t b) {
+        const auto &x = eng->work[a], &y = eng->work[b];
+        return x.np64 * rys_flops_prim(kPairL[x.X][0], kPairL[x.X][1], kPairL[x.Y][0], kPairL[x.Y][1]) >
+               y.np64 * rys_flops_prim(kPairL[y.X][0], kPairL[y.X][1], kPairL[y.Y][0], kPairL[y.Y][1]);
+      });
+      JkMulti mm{};
+      int nb = 0;
+      size_t lds = 0;
+      for (size_t ci : mc) {
+        if (mm.nseg >= kJkMaxSeg) break;
+        const auto& c = eng->work[ci];
+        const int la = kPairL[c.X][0], lb = kPairL[c.X][1], lc = kPairL[c.Y][0], ld = kPairL[c.Y][1];
+        const int sd = c.gflags(0) & kGroupStageD;
+        size_t need = jk_multi_lds(la, lb, lc, ld, eng->maxn, sd);
+        int stage = sd;
+        if (need > (size_t)eng->lds_max) {  // as the per-class launch: drop the staged D rows first
+          stage = 0;
+          need = jk_multi_lds(la, lb, lc, ld, eng->maxn, 0);
+        }
+        if (need > (size_t)eng->lds_max) continue;
+        mm.seg[mm.nseg++] = JkSeg{c.d_t64, c.d_g64, eng->d_sorted_s[c.Y], hgp_code(la, lb, lc, ld), nb, stage, 0};
+        nb += (int)c.g64.size();
+        lds = std::max(lds, need);
+        in_multi[ci] = 1;
+      }
+      if (mm.nseg > 0) {
+        hipStream_t ks = nside > 0 ? eng->side[next] : st;
+        if (nside > 0) next = (next + 1) % nside;
+        PQ_TRY(launch_jk_multi(eng->view_s(), mm, nb, lds, D_dev, eng->d_Jacc, eng->d_Kacc, ks));
+      }
+    }
     for (size_t ci = 0; ci < eng->work.size() && rc == PQ_OK; ++ci) {
       const auto& c = eng->work[ci];
       const int la = kPairL[c.X][0], lb = kPairL[c.X][1], lc = kPairL[c.Y][0], ld = kPairL[c.Y][1];
       for (int prec = 0; prec < 2; ++prec) {
         const auto& tl = prec ? c.t32 : c.t64;
-        if (tl.empty()) continue;
+        if (tl.empty() || (prec == 0 && in_multi[ci])) continue;
         hipStream_t ks = nside > 0 ? eng->side[next] : st;
         if (nside > 0) next = (next + 1) % nside;
         if (ms) PQ_TRY(mark(ks));

@@ -567,18 +567,20 @@ __device__ __forceinline__ void stage_rys_table(double* lds) {
 // tiles that share the bra pair (wavefront w takes tiles w, w+4, ...), so the
 // LDS K rows are zeroed and flushed once per run.  stage_d: the bra rows of D
 // are copied to LDS once per run as well.
+// The body of one block (bra run gidx) of k_eri_group, also run by the
+// fused multi-class launch (pq_eri_multi.hip); pq_lds is the block's
+// dynamic LDS.
 template <int LA, int LB, int LC, int LD, typename T, int MODE>
-__global__ __launch_bounds__(256) void k_eri_group(EngineView ev, const Tile* __restrict__ tiles,
-                                                   const int2* __restrict__ groups, int ngroups,
-                                                   const int* __restrict__ kets, const double* __restrict__ D,
-                                                   double* Jacc, double* Kacc, int stage_d) {
+__device__ __forceinline__ void group_body(const EngineView& ev, const Tile* __restrict__ tiles,
+                                           const int2* __restrict__ groups, int gidx, const int* __restrict__ kets,
+                                           const double* __restrict__ D, double* Jacc, double* Kacc, int stage_d,
+                                           double* pq_lds) {
   using C = Cls<LA, LB, LC, LD>;
   static_assert(C::SPLIT == 0, "bra groups are for unsplit classes");
-  extern __shared__ double pq_lds[];
   const int lane = threadIdx.x & 63;
   const int wv = threadIdx.x >> 6;
   const int n = ev.nbf;
-  const int2 g = groups[blockIdx.x];
+  const int2 g = groups[gidx];
   // light FP64 classes read the root table from LDS (one copy per block)
   // instead of gathering it from L1/L2 per lane
   constexpr int NT = kGroupTableDoubles<LA, LB, LC, LD, T, MODE>;
@@ -634,6 +636,25 @@ __global__ __launch_bounds__(256) void k_eri_group(EngineView ev, const Tile* __
       unsafeAtomicAdd(&Kacc[(size_t)d * ev.msize + bp.moff + grow * ms + col], v);
     }
   }
+}
+
+template <int LA, int LB, int LC, int LD, typename T, int MODE>
+__global__ __launch_bounds__(256) void k_eri_group(EngineView ev, const Tile* __restrict__ tiles,
+                                                   const int2* __restrict__ groups, int ngroups,
+                                                   const int* __restrict__ kets, const double* __restrict__ D,
+                                                   double* Jacc, double* Kacc, int stage_d) {
+  extern __shared__ double pq_lds[];
+  group_body<LA, LB, LC, LD, T, MODE>(ev, tiles, groups, (int)blockIdx.x, kets, D, Jacc, Kacc, stage_d, pq_lds);
+}
+
+// Dynamic LDS of one bra-run block: the root (or Boys) table, the K rows, the
+// bra J block and, with stage_d, the staged D rows.
+template <int LA, int LB, int LC, int LD, typename T, int MODE>
+__host__ __device__ constexpr size_t group_lds_bytes(int nbf, bool stage_d) {
+  using C = Cls<LA, LB, LC, LD>;
+  const size_t rows = (size_t)mode_ndens(MODE) * (C::NA + C::NB) * nbf;
+  return (kGroupTableDoubles<LA, LB, LC, LD, T, MODE> + rows * (stage_d ? 2 : 1) +
+          (size_t)mode_ndens(MODE) * C::NA * C::NB) * sizeof(double);
 }
 
 template <int LA, int LB, int LC, int LD, typename T, int MODE>

@@ -47,6 +47,26 @@ struct HgpMulti {
   int nseg;
 };
 bool hgp_multi_class(int la, int lb, int lc, int ld, int* split);
+
+// One launch over the bra runs (FP64 tiles) of the light J/K classes
+// (jk_multi_class): segment i covers blocks [b0, next b0) of class code =
+// hgp_code(la, lb, lc, ld), its bra runs `groups` over `tiles` and ket list.
+struct JkSeg {
+  const Tile* tiles;
+  const int2* groups;
+  const int* kets;
+  int code, b0, stage_d, pad;
+};
+constexpr int kJkMaxSeg = 8;
+struct JkMulti {
+  JkSeg seg[kJkMaxSeg];
+  int nseg;
+};
+bool jk_multi_class(int la, int lb, int lc, int ld);
+// dynamic LDS one block of that class's bra runs needs
+size_t jk_multi_lds(int la, int lb, int lc, int ld, int nbf, int stage_d);
+hipError_t launch_jk_multi(const EngineView& ev, const JkMulti& m, int nblocks, size_t lds, const double* D,
+                           double* Jacc, double* Kacc, hipStream_t st);
 hipError_t launch_hgp_multi(const EngineView& ev, const HgpMulti& m, int nblocks, double* ints, hipStream_t st);
 
 // ket order of each pair class (ids[Y][j] = pair at position j) and the

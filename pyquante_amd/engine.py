@@ -130,6 +130,11 @@ class Engine(object):
         """Spread class kernels over n engine streams (0 = caller's stream only)."""
         check(lib.pq_engine_set_streams(self.handle, int(n)))
 
+    def set_fused(self, on):
+        """One launch for the light classes' FP64 bra runs instead of one per
+        class (pq_engine_set_fused)."""
+        check(lib.pq_engine_set_fused(self.handle, 1 if on else 0))
+
     def set_lds_limit(self, nbytes=0):
         """Dynamic LDS limit of the J/K launches (pq_engine_set_lds_limit;
         0 = the device's); returns the device's per-workgroup LDS bytes."""

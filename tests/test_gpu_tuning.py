@@ -123,3 +123,35 @@ def test_scf_with_graph_replayed_fock_builds():
     e1 = scf(eng, mol, ConvCriteria=1e-10, graph=True)[0]
     assert abs(e0 - e1) < 1e-9
 
+
+
+def test_fused_light_class_launch_matches_per_class():
+    """One launch over the light classes' bra runs (pq_engine_set_fused)
+    gives the per-class launches' J and K, eager and graph-replayed."""
+    import numpy as np
+    import torch
+    from pyquante_amd import tuning
+    from pyquante_amd.basis import BasisSet
+    from pyquante_amd.engine import Engine
+    from pyquante_amd.molecule import water_cluster
+    mol = water_cluster((2, 2, 2), seed=0)
+    eng = Engine(BasisSet(mol, "6-31g**"), mol)
+    eng.build(1e-12)
+    caps = tuning.load("h2o8_631ss_fp64")
+    if caps is not None:
+        eng.set_group_caps(caps)
+    else:
+        eng.set_group_cap(16)
+    rng = np.random.default_rng(23)
+    C = rng.standard_normal((eng.nbf, 20)) * 0.1
+    D = torch.as_tensor(C @ C.T, device=eng.device)
+    J0, K0 = eng.fock_jk(D)
+    eng.set_fused(True)
+    J1, K1 = eng.fock_jk(D)
+    g = eng.capture_fock_jk(D)
+    for _ in range(3):
+        J2, K2 = g(D)
+        torch.cuda.synchronize()
+        for a, b in ((J1, J0), (K1, K0), (J2, J0), (K2, K0)):
+            assert torch.max(torch.abs(a - b)).item() < 1e-11
+    eng.set_fused(False)
