@@ -36,9 +36,31 @@ def shells_of(bfs):
         if (len(group) != len(comps) or [tuple(g.powers) for g in group] != comps or
                 any(g.origin != f.origin or g.pexps != f.pexps or g.pcoefs != f.pcoefs for g in group)):
             raise ValueError("basis functions %d.. do not form a complete shell in PyQuante order" % i)
+        _check_norms(group, i)
         out.append((L, list(f.pexps), list(f.pcoefs), getattr(f, "atid", 0), f.origin))
         i += len(comps)
     return out
+
+
+def _check_norms(group, first):
+    """The engine derives every normalisation itself: the primitive norm of
+    Src/lib/primitive-gto.c:76-83 per Cartesian component, and no contracted
+    renormalisation (norm = 1, CGBF.py:34).  A caller whose CGBFs carry other
+    values would silently get other integrals, so refuse them."""
+    from .basis import prim_norm
+    for k, g in enumerate(group):
+        norm = getattr(g, "norm", 1.0)
+        if abs(norm - 1.0) > 1e-12:
+            raise ValueError("basis function %d has norm %r: the engine uses norm = 1 for every "
+                             "contracted function (no renormalisation)" % (first + k, norm))
+        pnorms = getattr(g, "pnorms", None)
+        if pnorms is None:
+            continue
+        for a, n in zip(g.pexps, pnorms):
+            ref = prim_norm(float(a), tuple(g.powers))
+            if abs(n - ref) > 1e-10 * abs(ref):
+                raise ValueError("basis function %d has primitive norm %r for exponent %r, the engine "
+                                 "computes %r (per-component Cartesian norm)" % (first + k, n, a, ref))
 
 
 def _device():

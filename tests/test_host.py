@@ -174,3 +174,24 @@ def test_boys_grid_table_against_oracle():
                     v *= (j - 0.5) / T
             ref = O.boys(m, T)
             assert abs(v - ref) <= 2e-15 * ref, (T, m, v, ref)
+
+
+def test_cgbf_list_norms_are_checked():
+    """A flat CGBF list is regrouped into shells; the engine derives every
+    normalisation itself, so CGBFs carrying other norms are refused rather
+    than silently integrated with the engine's (CGBF.py:34,
+    primitive-gto.c:76-83)."""
+    import copy
+    from pyquante_amd.engine import shells_of
+    bs = BasisSet(water_cluster((1, 1, 1), seed=0), "6-31g**")
+    flat = list(bs.bfs)
+    sh = shells_of(flat)
+    assert [s[0] for s in sh] == [s.L for s in bs.shells]
+    bad = copy.deepcopy(flat)
+    bad[5].norm = 0.5
+    with pytest.raises(ValueError, match="norm"):
+        shells_of(bad)
+    bad = copy.deepcopy(flat)
+    bad[2].pnorms[0] *= 1.01
+    with pytest.raises(ValueError, match="primitive norm"):
+        shells_of(bad)
