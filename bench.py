@@ -230,7 +230,7 @@ def extra_workloads(args):
 
     if want("h2o16"):
         out.update(h2o16_workload(args))
-    if not any(want(w) for w in ("forces", "multi", "mp2", "hgp", "batch")):
+    if not any(want(w) for w in ("forces", "multi", "det", "mp2", "hgp", "batch")):
         return out
     # (H2O)_8: converged RHF for the forces, the two-density pass, MP2 and HGP
     mol = water_cluster((2, 2, 2), seed=args.seed)
@@ -251,8 +251,41 @@ def extra_workloads(args):
         out.update({"h2o8_two_density_ms": t2 * 1e3, "h2o8_two_single_builds_ms": t1 * 1e3})
         print("bench: (H2O)_8 J/K of two densities %.2f ms in one pass, %.2f ms as two builds" % (t2 * 1e3, t1 * 1e3),
               file=sys.stderr, flush=True)
+    if want("det"):
+        out.update(deterministic_workload(args, eng, D))
     out.update(extra_h2o8_stored(args, mol, eng, e8, orbe8, C8, want))
     return out
+
+
+def deterministic_workload(args, eng, D):
+    """(H2O)_8: the graph-replayed deterministic build (integer fixed-point
+    accumulation) against the FP64-atomic build with the same caps: times,
+    the largest J/K difference, and whether repeated replays are
+    bit-identical."""
+    import torch
+
+    from pyquante_amd import tuning
+    caps = tuning.load("h2o8_631ss_fp64")
+    if caps is not None:
+        eng.set_group_caps(caps)
+    g = eng.capture_fock_jk(D)
+    t_atomic = _timed(lambda: g(D), args.steps)
+    Ja, Ka = [x.clone() for x in g(D)]
+    eng.set_deterministic(True)
+    g = eng.capture_fock_jk(D)
+    t_det = _timed(lambda: g(D), args.steps)
+    Jd, Kd = [x.clone() for x in g(D)]
+    same = True
+    for _ in range(3):
+        J, K = g(D)
+        same = same and bool(torch.equal(J, Jd)) and bool(torch.equal(K, Kd))
+    eng.set_deterministic(False)
+    dmax = max(float((Jd - Ja).abs().max()), float((Kd - Ka).abs().max()))
+    print("bench: (H2O)_8 deterministic build %.3f ms against %.3f ms with FP64 atomics (x%.2f); max|dJ,dK| %.1e; "
+          "replays bit-identical: %s" % (t_det * 1e3, t_atomic * 1e3, t_det / t_atomic, dmax, same),
+          file=sys.stderr, flush=True)
+    return {"h2o8_deterministic_ms": t_det * 1e3, "h2o8_atomic_ms": t_atomic * 1e3,
+            "h2o8_deterministic_max_diff": dmax, "h2o8_deterministic_bit_identical": same}
 
 
 def h2o16_workload(args):
@@ -479,8 +512,8 @@ def main():
     ap.add_argument("--no-hgp", action="store_true", help="skip the HGP workload")
     ap.add_argument("--no-mp2", action="store_true", help="skip the MP2 workload")
     ap.add_argument("--only", default="",
-                    help="comma-separated subset of the extra workloads to run (h2o16, forces, multi, mp2, batch, "
-                         "hgp); default all")
+                    help="comma-separated subset of the extra workloads to run (h2o16, forces, multi, det, mp2, "
+                         "batch, hgp); default all")
     ap.add_argument("--batch-molecules", type=int, default=512,
                     help="molecules of the batched workload (0 skips it)")
     ap.add_argument("--streams", type=int, default=4, help="engine streams for the class kernels (0-4)")

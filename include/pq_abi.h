@@ -173,6 +173,16 @@ int pq_engine_set_streams(pq_engine *eng, int nstreams);
  * only.  No reference counterpart (a launch knob). */
 int pq_engine_set_fused(pq_engine *eng, int on);
 
+/* on = 1: deterministic J/K.  pq_fock_jk then accumulates every contribution
+ * as exact 64-bit integer limbs (resolution 2^-64, |contribution| < 2^62)
+ * instead of FP64 atomics, and converts once at the end, so J and K are
+ * bit-identical from call to call and independent of launch order, streams,
+ * bra-run caps and graph replay, as the reference's sequential getJ/getK loops
+ * are (Ints.py:195-223).  Single-density FP64 builds only (PQ_E_ARG
+ * otherwise); costs extra atomics and three times the LDS per accumulator.
+ * 0 (default) restores the FP64-atomic build. */
+int pq_engine_set_deterministic(pq_engine *eng, int on);
+
 /* Dynamic LDS the J/K launches may use, bytes: 0 restores the device's
  * per-workgroup limit (queried at creation, 160 KB on gfx950), a smaller
  * value lowers it (clamped to the device's); device_bytes (optional)

@@ -65,6 +65,7 @@ _proto("pq_build_pairs", [_VP, _D, _D])
 _proto("pq_engine_set_streams", [_VP, _I])
 _proto("pq_engine_set_lds_limit", [_VP, _I, _PI])
 _proto("pq_engine_set_fused", [_VP, _I])
+_proto("pq_engine_set_deterministic", [_VP, _I])
 _proto("pq_engine_set_prim_screen", [_VP, _D])
 _proto("pq_engine_set_group_cap", [_VP, _I])
 _proto("pq_engine_set_group_caps", [_VP, _PI])

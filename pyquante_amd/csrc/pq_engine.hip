@@ -151,6 +151,11 @@ struct pq_engine {
   bool streams_ok = false;
   int nside = kSide;  // streams in use (0 = caller's stream only)
   bool fused = false;  // one launch for the light classes' bra runs (pq_engine_set_fused)
+  // deterministic J/K (pq_engine_set_deterministic): fixed-point limbs of J,
+  // K and the ket J, 3 words per element each (allocated on first use)
+  bool det = false;
+  double* d_fx = nullptr;
+  size_t fx_cap = 0;
 
   // tiles per bra run for each (bra class, ket class, precision): -1 until
   // set by pq_engine_autotune or pq_engine_set_group_cap
@@ -257,6 +262,7 @@ struct pq_engine {
     dev_free(d_atom_z);
     dev_free(d_Jacc);
     dev_free(d_Kacc);
+    dev_free(d_fx);
     dev_free(d_D2);
     dev_free(d_gacc);
     dev_free(d_mol_moff);
@@ -1053,6 +1059,15 @@ int pq_engine_set_fused(pq_engine* eng, int on) {
   return PQ_OK;
 }
 
+int pq_engine_set_deterministic(pq_engine* eng, int on) {
+  if (!eng) return fail(PQ_E_ARG, "null engine");
+  if ((on != 0) != eng->det) {
+    eng->det = on != 0;
+    ++eng->gen;  // a captured build holds the other kernels and accumulators
+  }
+  return PQ_OK;
+}
+
 int pq_engine_set_streams(pq_engine* eng, int nstreams) {
   if (!eng) return fail(PQ_E_ARG, "null engine");
   if (nstreams < 0 || nstreams > pq_engine::kSide) return fail(PQ_E_ARG, "nstreams must be 0..4");
@@ -1419,7 +1434,30 @@ static int fock_jk_impl(pq_engine* eng, int nd, const double* const* Ds, double*
       PQ_TRY(hipMemcpyAsync(eng->d_D2 + nn, Ds[1], nn * sizeof(double), hipMemcpyDeviceToDevice, st));
       D_dev = eng->d_D2;
     }
-    PQ_TRY(launch_zero3(eng->d_Jacc, nd * nn, eng->d_Kacc, nd * nn, eng->d_jpair, nd * eng->jp_size, st));
+    // the deterministic build accumulates into fixed-point limbs instead
+    const bool det = eng->det;
+    EngineView kview = eng->view_s();
+    double *Jacc = eng->d_Jacc, *Kacc = eng->d_Kacc;
+    if (det) {
+      if (nd != 1 || precision != PQ_PREC_FP64) {
+        rc = fail(PQ_E_ARG, "the deterministic build digests one FP64 density");
+        break;
+      }
+      const size_t need = 6 * (size_t)nn + 3 * (size_t)eng->jp_size;
+      if (eng->fx_cap < need) {
+        dev_free(eng->d_fx);
+        eng->fx_cap = 0;
+        PQ_TRY(dev_alloc(&eng->d_fx, need));
+        eng->fx_cap = need;
+        ++eng->gen;
+      }
+      Jacc = eng->d_fx;
+      Kacc = eng->d_fx + 3 * nn;
+      kview.jpair = eng->d_fx + 6 * nn;
+      PQ_TRY(launch_zero3(Jacc, 3 * nn, Kacc, 3 * nn, kview.jpair, 3 * eng->jp_size, st));
+    } else {
+      PQ_TRY(launch_zero3(Jacc, nd * nn, Kacc, nd * nn, eng->d_jpair, nd * eng->jp_size, st));
+    }
     // fork: class kernels spread over the side streams, joined before the
     // symmetrisation (all of them only add into Jacc/Kacc)
     const int nside = eng->nside;
@@ -1428,7 +1466,7 @@ static int fock_jk_impl(pq_engine* eng, int nd, const double* const* Ds, double*
     // fused launch: the FP64 bra runs of the light classes in one grid,
     // segments ordered by their Rys operation count (heaviest first)
     std::vector<char> in_multi(eng->work.size(), 0);
-    if (eng->fused && nd == 1) {
+    if (eng->fused && nd == 1 && !det) {
       std::vector<size_t> mc;
       for (size_t ci = 0; ci < eng->work.size(); ++ci) {
         const auto& c = eng->work[ci];
@@ -1475,9 +1513,9 @@ static int fock_jk_impl(pq_engine* eng, int nd, const double* const* Ds, double*
         hipStream_t ks = nside > 0 ? eng->side[next] : st;
         if (nside > 0) next = (next + 1) % nside;
         if (ms) PQ_TRY(mark(ks));
-        PQ_TRY(launch_eri_class(la, lb, lc, ld, nd == 2 ? MODE_JK2 : MODE_JK, prec, eng->view_s(),
+        PQ_TRY(launch_eri_class(la, lb, lc, ld, det ? MODE_JKD : nd == 2 ? MODE_JK2 : MODE_JK, prec, kview,
                                 prec ? c.d_t32 : c.d_t64,
-                                (int)tl.size(), eng->d_sorted_s[c.Y], D_dev, eng->d_Jacc, eng->d_Kacc, nullptr,
+                                (int)tl.size(), eng->d_sorted_s[c.Y], D_dev, Jacc, Kacc, nullptr,
                                 nullptr, ks, prec ? c.d_g32 : c.d_g64, (int)(prec ? c.g32 : c.g64).size(),
                                 c.gflags(prec)));
         if (ms) {
@@ -1494,8 +1532,15 @@ static int fock_jk_impl(pq_engine* eng, int nd, const double* const* Ds, double*
       jmap.ids[y] = eng->d_sorted_s[y];
       jmap.ndf[y] = ncart(kPairL[y][1]);
     }
-    PQ_TRY(launch_jpair_scatter(eng->view_s(), jmap, nd, eng->d_Jacc, st));
-    for (int d = 0; d < nd && rc == PQ_OK; ++d)
+    if (det) {
+      PQ_TRY(launch_jpair_scatter_fx(kview, jmap, reinterpret_cast<long long*>(Jacc), st));
+      PQ_TRY(launch_symmetrize_fx(eng->nmol, eng->maxn, eng->nmol == 1 ? nullptr : eng->d_mol_moff, eng->d_mol_n,
+                                  nn, reinterpret_cast<const long long*>(Jacc),
+                                  reinterpret_cast<const long long*>(Kacc), Js[0], Ks[0], st));
+    } else {
+      PQ_TRY(launch_jpair_scatter(eng->view_s(), jmap, nd, eng->d_Jacc, st));
+    }
+    for (int d = 0; d < nd && rc == PQ_OK && !det; ++d)
       PQ_TRY(eng->nmol == 1
                  ? launch_symmetrize(eng->nbf, eng->d_Jacc + d * nn, eng->d_Kacc + d * nn, Js[d], Ks[d], 1.0, 1.0,
                                      st)

@@ -2,6 +2,7 @@
 // compiled in its own translation unit from pq_eri_cls.hip), the stored-array
 // digestion kernel and the J/K symmetrisation.
 #include "pq_device.h"
+#include "pq_fixed.h"
 #include "pq_kernels.h"
 
 namespace pq {
@@ -12,7 +13,8 @@ namespace pq {
                                               double* ints, double* qout, hipStream_t st, const int2* groups, \
                                               int ngroups, int gflags);
 #define PQ_DECL(A, B, C_, D_) \
-  PQ_DECL1(A, B, C_, D_, 0) PQ_DECL1(A, B, C_, D_, 1) PQ_DECL1(A, B, C_, D_, 2) PQ_DECL1(A, B, C_, D_, 5)
+  PQ_DECL1(A, B, C_, D_, 0) PQ_DECL1(A, B, C_, D_, 1) PQ_DECL1(A, B, C_, D_, 2) PQ_DECL1(A, B, C_, D_, 5) \
+  PQ_DECL1(A, B, C_, D_, 6)
 PQ_DECL(0, 0, 0, 0)
 PQ_DECL(1, 0, 0, 0)
 PQ_DECL(1, 0, 1, 0)
@@ -298,6 +300,10 @@ hipError_t launch_eri_class(int la, int lb, int lc, int ld, int mode, int fp32, 
       return launch_cls_##A##B##C_##D_##_m2(ev, tiles, ntiles, kets, D, Jacc, Kacc, ints, qout, st, nullptr, 0, 0); \
     if (mode == MODE_JK2)                                                                          \
       return launch_cls_##A##B##C_##D_##_m5(ev, tiles, ntiles, kets, D, Jacc, Kacc, ints, qout, st, groups, ngroups, gflags); \
+    if (mode == MODE_JKD && !fp32)                                                                 \
+      return launch_cls_##A##B##C_##D_##_m6(ev, tiles, ntiles, kets, D, Jacc, Kacc, ints, qout, st, groups, ngroups, gflags); \
+    if (mode == MODE_JKD)                                                                          \
+      return hipErrorInvalidValue;                                                                 \
     return launch_diag_##A##B##C_##D_(ev, tiles, ntiles, kets, D, Jacc, Kacc, ints, qout, st);      \
   }
   PQ_CASE(0, 0, 0, 0)
@@ -409,6 +415,65 @@ hipError_t launch_grad_class(int la, int lb, int lc, int ld, const EngineView& e
   PQ_GCASE(2, 2, 2, 2)
 #undef PQ_GCASE
   return hipErrorInvalidValue;
+}
+
+// ---------------------------------------------------------------------
+// The deterministic build's fixed-point accumulators (pq_fixed.h): limb k of
+// J/K element e at acc[k * msize + e], of ket position i at
+// jpair[k * jp_size + i].  Integer adds throughout, so the result does not
+// depend on the order of any accumulation.
+// ---------------------------------------------------------------------
+__global__ void k_jpair_scatter_fx(EngineView ev, JpairMap map, long long* Jfx) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= ev.jp_size) return;
+  int Y = 0;
+#pragma unroll
+  for (int y = 1; y < 6; ++y)
+    if (i >= ev.jp_off[y]) Y = y;
+  const int np = ev.jp_n[Y], ndf = map.ndf[Y];
+  const int local = i - ev.jp_off[Y];
+  const int cd = local / np, j = local - cd * np;
+  const PairInfo p = ev.pairs[map.ids[Y][j]];
+  const int64_t e = p.moff + (int64_t)(p.bfA + cd / ndf) * p.mstride + p.bfB + cd % ndf;
+  const long long* jp = reinterpret_cast<const long long*>(ev.jpair);
+  for (int k = 0; k < 3; ++k) {
+    const long long v = jp[k * ev.jp_size + i];
+    if (v) Jfx[k * ev.msize + e] += v;
+  }
+}
+
+hipError_t launch_jpair_scatter_fx(const EngineView& ev, const JpairMap& map, long long* Jfx, hipStream_t st) {
+  if (ev.jp_size <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_jpair_scatter_fx, dim3((unsigned)((ev.jp_size + 255) / 256)), dim3(256), 0, st, ev, map,
+                     Jfx);
+  return hipGetLastError();
+}
+
+// J = Jacc + Jacc^T, K = Kacc + Kacc^T from the limb sums (added as integers,
+// then converted once), per molecule block m (blockIdx.y): block m starts at
+// moff[m] with n = mn[m], or at 0 with n = n1 when moff is null.
+__global__ void k_symmetrize_fx(const int* __restrict__ moff, const int* __restrict__ mn, int n1, int64_t msize,
+                                const long long* __restrict__ Jfx, const long long* __restrict__ Kfx, double* J,
+                                double* K) {
+  const int m = blockIdx.y;
+  const int n = moff ? mn[m] : n1;
+  const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= (int64_t)n * n) return;
+  const int i = (int)(idx / n), j = (int)(idx % n);
+  const int64_t o = moff ? moff[m] : 0;
+  const int64_t a = o + idx, b = o + (int64_t)j * n + i;
+  if (J)
+    J[a] = fx_value(Jfx[a] + Jfx[b], Jfx[msize + a] + Jfx[msize + b], Jfx[2 * msize + a] + Jfx[2 * msize + b]);
+  if (K)
+    K[a] = fx_value(Kfx[a] + Kfx[b], Kfx[msize + a] + Kfx[msize + b], Kfx[2 * msize + a] + Kfx[2 * msize + b]);
+}
+
+hipError_t launch_symmetrize_fx(int nmol, int maxn, const int* moff, const int* mn, int64_t msize,
+                                const long long* Jfx, const long long* Kfx, double* J, double* K, hipStream_t st) {
+  if (nmol <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_symmetrize_fx, dim3((unsigned)(((int64_t)maxn * maxn + 255) / 256), nmol), dim3(256), 0, st,
+                     moff, mn, maxn, msize, Jfx, Kfx, J, K);
+  return hipGetLastError();
 }
 
 hipError_t launch_symmetrize(int n, const double* Jacc, const double* Kacc, double* J, double* K, double jscale,

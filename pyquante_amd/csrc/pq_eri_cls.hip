@@ -1,7 +1,8 @@
 // pq_eri_cls.hip -- one angular-momentum class and one mode of the Rys ERI
 // kernels per translation unit (PQ_LA..PQ_LD and PQ_MODE given on the
 // compiler command line: 0 J/K FP64, 1 J/K FP32, 2 store, 3 Schwarz
-// diagonal, 4 nuclear gradient, 5 J/K of two densities FP64), so the
+// diagonal, 4 nuclear gradient, 5 J/K of two densities FP64, 6 deterministic
+// J/K FP64 with fixed-point accumulators), so the
 // instantiations compile in parallel.
 #include "pq_eri_impl.h"
 #include "pq_grad_impl.h"
@@ -32,6 +33,9 @@ hipError_t PQ_CAT(PQ_LA, PQ_LB, PQ_LC, PQ_LD, PQ_MODE)(const EngineView& ev, con
                                                                     qout, st, nullptr, 0, 0);
 #elif PQ_MODE == 5
   return launch_one<PQ_LA, PQ_LB, PQ_LC, PQ_LD, double, MODE_JK2>(ev, tiles, ntiles, kets, D, Jacc, Kacc, ints,
+                                                                  qout, st, groups, ngroups, gflags);
+#elif PQ_MODE == 6
+  return launch_one<PQ_LA, PQ_LB, PQ_LC, PQ_LD, double, MODE_JKD>(ev, tiles, ntiles, kets, D, Jacc, Kacc, ints,
                                                                   qout, st, groups, ngroups, gflags);
 #else
   return launch_one<PQ_LA, PQ_LB, PQ_LC, PQ_LD, double, MODE_DIAG>(ev, tiles, ntiles, kets, D, Jacc, Kacc, ints,

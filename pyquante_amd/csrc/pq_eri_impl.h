@@ -22,6 +22,7 @@
 #include <type_traits>
 
 #include "pq_device.h"
+#include "pq_fixed.h"
 #include "pq_hgp_core.h"
 #include "pq_kernels.h"
 
@@ -191,7 +192,44 @@ __device__ __forceinline__ void hgp_jk_block(const EngineView& ev, const PairInf
 // the J blocks; then the K rows of each bra function a (from the D rows of
 // b), each row flushed as soon as it is complete; then the K rows of each b
 // (from the D rows of a).
-template <int LA, int LB, int LC, int LD, int A0, int NAS, int B0, int NBS, typename T>
+// Adds x into accumulator element p: an FP64 atomic (global memory, or LDS
+// when GLB is false), or in MODE_JKD the three fixed-point limbs of x into
+// the int64 sums at p, p + stride and p + 2 stride.
+template <int MODE, bool GLB>
+__device__ __forceinline__ void acc_add(double* p, int64_t stride, double x) {
+  if constexpr (MODE == MODE_JKD) {
+    const FxLimbs l = fx_split(x);
+    unsigned long long* q = reinterpret_cast<unsigned long long*>(p);
+    if (l.l0) atomicAdd(q, (unsigned long long)l.l0);
+    if (l.l1) atomicAdd(q + stride, (unsigned long long)l.l1);
+    if (l.l2) atomicAdd(q + 2 * stride, (unsigned long long)l.l2);
+  } else if constexpr (GLB) {
+    unsafeAtomicAdd(p, x);
+  } else {
+    atomicAdd(p, x);
+  }
+}
+
+// Flushes accumulator element i of a block's LDS array (limbs stride lstride
+// apart in MODE_JKD) into global element p (limbs gstride apart); zero
+// elements are skipped.
+template <int MODE>
+__device__ __forceinline__ void acc_flush(const double* lds, int64_t i, int64_t lstride, double* p,
+                                          int64_t gstride) {
+  if constexpr (MODE == MODE_JKD) {
+    const long long* l = reinterpret_cast<const long long*>(lds);
+    const long long s0 = l[i], s1 = l[i + lstride], s2 = l[i + 2 * lstride];
+    unsigned long long* q = reinterpret_cast<unsigned long long*>(p);
+    if (s0) atomicAdd(q, (unsigned long long)s0);
+    if (s1) atomicAdd(q + gstride, (unsigned long long)s1);
+    if (s2) atomicAdd(q + 2 * gstride, (unsigned long long)s2);
+  } else {
+    const double v = lds[i];
+    if (v != 0.0) unsafeAtomicAdd(p, v);
+  }
+}
+
+template <int LA, int LB, int LC, int LD, int A0, int NAS, int B0, int NBS, typename T, int MODE>
 __device__ __forceinline__ void digest_lane(const EngineView& ev, const PairInfo& bp, const PairInfo& kp,
                                             const T (&out)[NAS * NBS * ncart(LC) * ncart(LD)], double f,
                                             bool active, const double* __restrict__ D, double* Jacc,
@@ -258,8 +296,8 @@ __device__ __forceinline__ void digest_lane(const EngineView& ev, const PairInfo
           }
         const double s = wave_sum(jab);
         if (lane == 0) {
-          if (jrow) atomicAdd(&jrow[(A0 + a) * C::NB + B0 + b], 2.0 * s);
-          else unsafeAtomicAdd(&Jacc[(a0 + a) * n + b0 + b], 2.0 * s);
+          if (jrow) acc_add<MODE, false>(&jrow[(A0 + a) * C::NB + B0 + b], C::NA * C::NB, 2.0 * s);
+          else acc_add<MODE, true>(&Jacc[(a0 + a) * n + b0 + b], ev.msize, 2.0 * s);
         }
       }
     if (!active) return;
@@ -267,29 +305,33 @@ __device__ __forceinline__ void digest_lane(const EngineView& ev, const PairInfo
 #pragma unroll
       for (int c = 0; c < NC; ++c)
 #pragma unroll
-        for (int d = 0; d < ND; ++d) atomicAdd(&jsum[lane * NC * ND + c * ND + d], 2.0 * jcd[c][d]);
+        for (int d = 0; d < ND; ++d)
+          acc_add<MODE, false>(&jsum[lane * NC * ND + c * ND + d], 64 * NC * ND, 2.0 * jcd[c][d]);
     } else {
       // ket-side J by ket position: the lanes of a tile hit consecutive
       // addresses (coalesced atomics), scattered into J after the build
 #pragma unroll
       for (int c = 0; c < NC; ++c)
 #pragma unroll
-        for (int d = 0; d < ND; ++d) unsafeAtomicAdd(&jpair[(c * ND + d) * jn + kpos], 2.0 * jcd[c][d]);
+        for (int d = 0; d < ND; ++d)
+          acc_add<MODE, true>(&jpair[(c * ND + d) * jn + kpos], ev.jp_size, 2.0 * jcd[c][d]);
     }
   }
   // one K row (bra function row r of the block layout, matrix row grow):
   // columns c0.. (NC values) and d0.. (ND values)
+  // (LDS rows: NA + NB rows of the largest molecule's nbf columns per limb)
+  const int64_t kstride = (int64_t)(C::NA + C::NB) * ev.nbf;
   auto flush_row = [&](int r, int grow, const double (&kc)[NC], const double (&kd)[ND]) {
     if (krow) {
 #pragma unroll
-      for (int c = 0; c < NC; ++c) atomicAdd(&krow[r * n + c0 + c], kc[c]);
+      for (int c = 0; c < NC; ++c) acc_add<MODE, false>(&krow[r * n + c0 + c], kstride, kc[c]);
 #pragma unroll
-      for (int d = 0; d < ND; ++d) atomicAdd(&krow[r * n + d0 + d], kd[d]);
+      for (int d = 0; d < ND; ++d) acc_add<MODE, false>(&krow[r * n + d0 + d], kstride, kd[d]);
     } else {
 #pragma unroll
-      for (int c = 0; c < NC; ++c) unsafeAtomicAdd(&Kacc[grow * n + c0 + c], kc[c]);
+      for (int c = 0; c < NC; ++c) acc_add<MODE, true>(&Kacc[grow * n + c0 + c], ev.msize, kc[c]);
 #pragma unroll
-      for (int d = 0; d < ND; ++d) unsafeAtomicAdd(&Kacc[grow * n + d0 + d], kd[d]);
+      for (int d = 0; d < ND; ++d) acc_add<MODE, true>(&Kacc[grow * n + d0 + d], ev.msize, kd[d]);
     }
   };
   // ---- K rows of a: K_ac += v D_bd, K_ad += v D_bc
@@ -430,7 +472,7 @@ __device__ __forceinline__ void lane_body(const EngineView& ev, const Tile& tl, 
     constexpr int Y = pair_class_of(LC, LD);
 #pragma unroll
     for (int d = 0; d < mode_ndens(MODE); ++d)
-      digest_lane<LA, LB, LC, LD, A0, NAS, B0, NBS, T>(
+      digest_lane<LA, LB, LC, LD, A0, NAS, B0, NBS, T, MODE>(
           ev, bp, kp, out, f, active, D + d * nn, Jacc + d * nn, Kacc + d * nn,
           krow ? krow + d * (C::NA + C::NB) * ev.nbf : nullptr, jsum ? jsum + d * 64 * C::NC * C::ND : nullptr,
           drow ? drow + d * (C::NA + C::NB) * ev.nbf : nullptr, jrow ? jrow + d * C::NA * C::NB : nullptr,
@@ -588,17 +630,19 @@ __device__ __forceinline__ void group_body(const EngineView& ev, const Tile* __r
   if constexpr (kHgpJK<LA, LB, LC, LD, T, MODE>) stage_boys_table(pq_lds);
   else if constexpr (NT > 0) stage_rys_table<C::NR, T>(pq_lds);
   constexpr int ND = mode_ndens(MODE);
+  constexpr int NL = mode_limbs(MODE);  // accumulator words per element
+  static_assert(NL == 1 || ND == 1, "fixed-point limbs are for single-density builds");
   double* krow = pq_lds + NT;
-  for (int i = threadIdx.x; i < ND * (C::NA + C::NB) * n; i += blockDim.x) krow[i] = 0.0;
+  for (int i = threadIdx.x; i < ND * NL * (C::NA + C::NB) * n; i += blockDim.x) krow[i] = 0.0;
   const PairInfo bp = ld_uniform(ev.pairs + tiles[g.x].bra);
   const int ms = bp.mstride;
   const int rows = (C::NA + C::NB) * ms, region = (C::NA + C::NB) * n;
   // J bra block accumulator after the K rows (and the staged D rows)
-  double* jrow = krow + ND * region * (stage_d ? 2 : 1);
-  for (int i = threadIdx.x; i < ND * C::NA * C::NB; i += blockDim.x) jrow[i] = 0.0;
+  double* jrow = krow + ND * region * (NL + (stage_d ? 1 : 0));
+  for (int i = threadIdx.x; i < ND * NL * C::NA * C::NB; i += blockDim.x) jrow[i] = 0.0;
   double* drow = nullptr;
   if (stage_d) {
-    drow = krow + ND * region;
+    drow = krow + ND * NL * region;
     for (int i = threadIdx.x; i < ND * rows; i += blockDim.x) {
       const int d = i / rows, ir = i - d * rows;
       const int r = ir / ms, col = ir - r * ms;
@@ -623,18 +667,16 @@ __device__ __forceinline__ void group_body(const EngineView& ev, const Tile* __r
   __syncthreads();
   for (int i = threadIdx.x; i < ND * C::NA * C::NB; i += blockDim.x) {
     const int d = i / (C::NA * C::NB), ab = i - d * (C::NA * C::NB);
-    const double v = jrow[i];
-    if (v != 0.0)
-      unsafeAtomicAdd(&Jacc[(size_t)d * ev.msize + bp.moff + (bp.bfA + ab / C::NB) * ms + bp.bfB + ab % C::NB], v);
+    acc_flush<MODE>(jrow, i, C::NA * C::NB,
+                    &Jacc[(size_t)d * ev.msize + bp.moff + (bp.bfA + ab / C::NB) * ms + bp.bfB + ab % C::NB],
+                    ev.msize);
   }
   for (int i = threadIdx.x; i < ND * rows; i += blockDim.x) {
     const int d = i / rows, ir = i - d * rows;
-    const double v = krow[d * region + ir];
-    if (v != 0.0) {
-      const int r = ir / ms, col = ir - r * ms;
-      const int grow = r < C::NA ? bp.bfA + r : bp.bfB + (r - C::NA);
-      unsafeAtomicAdd(&Kacc[(size_t)d * ev.msize + bp.moff + grow * ms + col], v);
-    }
+    const int r = ir / ms, col = ir - r * ms;
+    const int grow = r < C::NA ? bp.bfA + r : bp.bfB + (r - C::NA);
+    acc_flush<MODE>(krow, d * region + ir, region, &Kacc[(size_t)d * ev.msize + bp.moff + grow * ms + col],
+                    ev.msize);
   }
 }
 
@@ -652,9 +694,10 @@ __global__ __launch_bounds__(256) void k_eri_group(EngineView ev, const Tile* __
 template <int LA, int LB, int LC, int LD, typename T, int MODE>
 __host__ __device__ constexpr size_t group_lds_bytes(int nbf, bool stage_d) {
   using C = Cls<LA, LB, LC, LD>;
+  constexpr size_t NL = mode_limbs(MODE);
   const size_t rows = (size_t)mode_ndens(MODE) * (C::NA + C::NB) * nbf;
-  return (kGroupTableDoubles<LA, LB, LC, LD, T, MODE> + rows * (stage_d ? 2 : 1) +
-          (size_t)mode_ndens(MODE) * C::NA * C::NB) * sizeof(double);
+  return (kGroupTableDoubles<LA, LB, LC, LD, T, MODE> + rows * (NL + (stage_d ? 1 : 0)) +
+          (size_t)mode_ndens(MODE) * NL * C::NA * C::NB) * sizeof(double);
 }
 
 template <int LA, int LB, int LC, int LD, typename T, int MODE>
@@ -671,10 +714,12 @@ __global__ __launch_bounds__(256) void k_eri_lane(EngineView ev, const Tile* __r
   double* krow = nullptr;
   double* jsum = nullptr;
   constexpr int ND = mode_ndens(MODE);
+  constexpr int NL = mode_limbs(MODE);  // accumulator words per element
+  static_assert(NL == 1 || ND == 1, "fixed-point limbs are for single-density builds");
   if (lds_mode) {
     krow = pq_lds;
-    if (C::SPLIT) jsum = pq_lds + ND * (C::NA + C::NB) * n;
-    const int tot = ND * ((C::NA + C::NB) * n + (C::SPLIT ? 64 * C::NC * C::ND : 0));
+    if (C::SPLIT) jsum = pq_lds + ND * NL * (C::NA + C::NB) * n;
+    const int tot = ND * NL * ((C::NA + C::NB) * n + (C::SPLIT ? 64 * C::NC * C::ND : 0));
     for (int i = threadIdx.x; i < tot; i += blockDim.x) pq_lds[i] = 0.0;
     __syncthreads();
   } else if (t >= ntiles) {
@@ -719,12 +764,10 @@ __global__ __launch_bounds__(256) void k_eri_lane(EngineView ev, const Tile* __r
   const int rows = (C::NA + C::NB) * ms, region = (C::NA + C::NB) * n;
   for (int i = threadIdx.x; i < ND * rows; i += blockDim.x) {
     const int d = i / rows, ir = i - d * rows;
-    const double v = krow[d * region + ir];
-    if (v != 0.0) {
-      const int r = ir / ms, col = ir - r * ms;
-      const int grow = r < C::NA ? bp.bfA + r : bp.bfB + (r - C::NA);
-      unsafeAtomicAdd(&Kacc[(size_t)d * ev.msize + bp.moff + grow * ms + col], v);
-    }
+    const int r = ir / ms, col = ir - r * ms;
+    const int grow = r < C::NA ? bp.bfA + r : bp.bfB + (r - C::NA);
+    acc_flush<MODE>(krow, d * region + ir, region, &Kacc[(size_t)d * ev.msize + bp.moff + grow * ms + col],
+                    ev.msize);
   }
   if constexpr (C::SPLIT != 0) {
     constexpr int NCD = C::NC * C::ND;
@@ -736,9 +779,9 @@ __global__ __launch_bounds__(256) void k_eri_lane(EngineView ev, const Tile* __r
         const int cd = i >> 6, l = i & 63;
         const TileLane tl_l = tile_lane(tl.nket, l);
         if (!tl_l.active) continue;
-        const double v = jsum[d * 64 * NCD + l * NCD + cd];
-        if (v != 0.0)
-          unsafeAtomicAdd(&ev.jpair[d * ev.jp_size + ev.jp_off[Y] + cd * ev.jp_n[Y] + tl.ket0 + tl_l.slot], v);
+        acc_flush<MODE>(jsum, d * 64 * NCD + l * NCD + cd, 64 * NCD,
+                        &ev.jpair[d * ev.jp_size + ev.jp_off[Y] + cd * ev.jp_n[Y] + tl.ket0 + tl_l.slot],
+                        ev.jp_size);
       }
   }
 }
@@ -752,17 +795,21 @@ static hipError_t launch_one(const EngineView& ev, const Tile* tiles, int ntiles
                              hipStream_t st, const int2* groups, int ngroups, int gflags) {
   if (ntiles <= 0) return hipSuccess;
   using C = Cls<LA, LB, LC, LD>;
-  const size_t lds = mode_ndens(MODE) * ((size_t)(C::NA + C::NB) * ev.nbf + (C::SPLIT ? 64 * C::NC * C::ND : 0)) *
-                     sizeof(double);
+  // the K rows (and split classes' ket J sums), one word per element and
+  // limb; the staged D rows take one word per element
+  constexpr size_t NL = mode_limbs(MODE);
+  const size_t drows = mode_ndens(MODE) * (size_t)(C::NA + C::NB) * ev.nbf * sizeof(double);
+  const size_t lds = NL * mode_ndens(MODE) *
+                     ((size_t)(C::NA + C::NB) * ev.nbf + (C::SPLIT ? 64 * C::NC * C::ND : 0)) * sizeof(double);
   if constexpr (C::SPLIT == 0 && mode_is_jk(MODE)) {
     const size_t glds = lds + kGroupTableDoubles<LA, LB, LC, LD, T, MODE> * sizeof(double);
-    // plus the bra J block; the staged D rows take as much LDS as the K rows
-    const size_t jlds = mode_ndens(MODE) * C::NA * C::NB * sizeof(double);
+    // plus the bra J block
+    const size_t jlds = NL * mode_ndens(MODE) * C::NA * C::NB * sizeof(double);
     const size_t lmax = (size_t)ev.lds_max;
-    const int stage_d = (gflags & kGroupStageD) && glds + jlds + lds <= lmax;
+    const int stage_d = (gflags & kGroupStageD) && glds + jlds + drows <= lmax;
     if (groups && ngroups > 0 && glds + jlds <= lmax) {
       hipLaunchKernelGGL((k_eri_group<LA, LB, LC, LD, T, MODE>), dim3(ngroups), dim3(256),
-                         glds + jlds + (stage_d ? lds : 0), st, ev, tiles, groups, ngroups, kets, D, Jacc, Kacc,
+                         glds + jlds + (stage_d ? drows : 0), st, ev, tiles, groups, ngroups, kets, D, Jacc, Kacc,
                          stage_d);
       return hipGetLastError();
     }

@@ -6,11 +6,16 @@
 
 namespace pq {
 
-enum { MODE_JK = 0, MODE_STORE = 1, MODE_DIAG = 2, MODE_JK2 = 3 };
+enum { MODE_JK = 0, MODE_STORE = 1, MODE_DIAG = 2, MODE_JK2 = 3, MODE_JKD = 4 };
 // J/K modes and the number of densities they digest (MODE_JK2: two stacked
-// n x n densities into two stacked J/K accumulators)
-constexpr bool mode_is_jk(int m) { return m == MODE_JK || m == MODE_JK2; }
+// n x n densities into two stacked J/K accumulators).  MODE_JKD is the
+// deterministic single-density build: every accumulator element (global J/K,
+// position-indexed ket J, LDS K rows and J sums) is three int64 fixed-point
+// limbs (pq_fixed.h) at a stride of one array apart, so the sums do not depend
+// on the order of the atomics.
+constexpr bool mode_is_jk(int m) { return m == MODE_JK || m == MODE_JK2 || m == MODE_JKD; }
 constexpr int mode_ndens(int m) { return m == MODE_JK2 ? 2 : 1; }
+constexpr int mode_limbs(int m) { return m == MODE_JKD ? 3 : 1; }
 
 // classes with at most this many Cartesian integrals run one lane per quartet
 constexpr int kLaneMaxComp = 108;
@@ -101,6 +106,11 @@ hipError_t launch_symmetrize_packed(int nmol, int maxn, const int* moff, const i
                                     const double* Kacc, double* J, double* K, hipStream_t st);
 hipError_t launch_symmetrize(int n, const double* Jacc, const double* Kacc, double* J, double* K,
                              double jscale, double kscale, hipStream_t st);
+// MODE_JKD: the ket J limbs into the J limbs, and J/K from the limb sums
+// (moff null: one molecule of maxn functions)
+hipError_t launch_jpair_scatter_fx(const EngineView& ev, const JpairMap& map, long long* Jfx, hipStream_t st);
+hipError_t launch_symmetrize_fx(int nmol, int maxn, const int* moff, const int* mn, int64_t msize,
+                                const long long* Jfx, const long long* Kfx, double* J, double* K, hipStream_t st);
 
 // J/K from a stored flat unique-integral array (Ints.py:195-223 over Ints)
 hipError_t launch_digest_stored(int n, const double* ints, uint64_t len, const double* D, double* Jacc,

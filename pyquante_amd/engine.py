@@ -157,6 +157,13 @@ class Engine(object):
         class (pq_engine_set_fused)."""
         check(lib.pq_engine_set_fused(self.handle, 1 if on else 0))
 
+    def set_deterministic(self, on):
+        """Bit-reproducible J/K (pq_engine_set_deterministic): exact integer
+        fixed-point accumulation instead of FP64 atomics, so every build of
+        the same density gives the same bits, whatever the launch order,
+        streams, bra-run caps or graph replay.  Single-density FP64 builds."""
+        check(lib.pq_engine_set_deterministic(self.handle, 1 if on else 0))
+
     def set_lds_limit(self, nbytes=0):
         """Dynamic LDS limit of the J/K launches (pq_engine_set_lds_limit;
         0 = the device's); returns the device's per-workgroup LDS bytes."""

@@ -1,0 +1,147 @@
+"""GPU: the deterministic J/K build (pq_engine_set_deterministic).
+
+Every contribution is accumulated as exact integer limbs, so J and K must be
+bit-identical (torch.equal) across repeated builds, stream counts, bra-run
+caps, the LDS limit (K rows in LDS or global atomics), graph replay and eager
+work between replays, and agree with the FP64-atomic build and the stored
+integrals digested on the GPU to 1e-11 absolute (the atomic build's own
+summation-order noise is about 1e-13).  The oracle pins the plain build in
+test_gpu_eri.py; the limb arithmetic is pinned on the CPU in
+test_fixed_point.py."""
+import numpy as np
+import pytest
+import torch
+
+from pyquante_amd._abi import PQError
+
+pytestmark = pytest.mark.gpu
+
+
+def _engine(lattice, seed):
+    from pyquante_amd.basis import BasisSet
+    from pyquante_amd.engine import Engine
+    from pyquante_amd.molecule import water_cluster
+    mol = water_cluster(lattice, seed=seed)
+    return Engine(BasisSet(mol, "6-31g**"), mol)
+
+
+def _density(n, seed):
+    rng = np.random.default_rng(seed)
+    C = rng.standard_normal((n, 20)) * 0.1
+    return torch.as_tensor(C @ C.T, device="cuda")
+
+
+def _close(a, b, tol=1e-11):
+    return torch.max(torch.abs(a - b)).item() < tol
+
+
+def test_deterministic_matches_stored_and_atomic_builds():
+    from pyquante_amd.engine import digest_stored
+    eng = _engine((2, 1, 1), 1)
+    D = _density(eng.nbf, 11)
+    Js, Ks = digest_stored(eng.get2ints(), D)
+    eng.build(0.0, 0.0)
+    Ja, Ka = eng.fock_jk(D)
+    eng.set_deterministic(True)
+    Jd, Kd = eng.fock_jk(D)
+    print("deterministic vs stored: %.2e %.2e" % (torch.max(torch.abs(Jd - Js)).item(),
+                                                 torch.max(torch.abs(Kd - Ks)).item()), flush=True)
+    assert _close(Jd, Js) and _close(Kd, Ks)
+    assert _close(Jd, Ja) and _close(Kd, Ka)
+    # symmetric by construction (J = Jacc + Jacc^T from the same integers)
+    assert torch.equal(Jd, Jd.T) and torch.equal(Kd, Kd.T)
+    eng.set_deterministic(False)
+    J2, K2 = eng.fock_jk(D)
+    assert _close(J2, Ja, 1e-12) and _close(K2, Ka, 1e-12)
+
+
+def test_bit_identical_across_launch_shapes():
+    from pyquante_amd import _abi
+    eng = _engine((2, 2, 1), 2)
+    eng.build(1e-12, 0.0)
+    D = _density(eng.nbf, 12)
+    eng.set_deterministic(True)
+    J0, K0 = eng.fock_jk(D)
+    S = _abi.PQ_CAP_STAGE_D
+    variants = [("again", lambda: None), ("cap 0", lambda: eng.set_group_cap(0)),
+                ("cap 4", lambda: eng.set_group_cap(4)), ("cap 16 + D rows", lambda: eng.set_group_cap(16 | S)),
+                ("1 stream", lambda: eng.set_streams(1)), ("no side streams", lambda: eng.set_streams(0)),
+                ("4 streams", lambda: eng.set_streams(4)),
+                ("K rows in global memory", lambda: eng.set_lds_limit(4096)),
+                ("device LDS", lambda: eng.set_lds_limit(0))]
+    for name, change in variants:
+        change()
+        J, K = eng.fock_jk(D)
+        print("deterministic:", name, flush=True)
+        assert torch.equal(J, J0) and torch.equal(K, K0), name
+
+
+def test_graph_replays_bit_identical():
+    eng = _engine((2, 1, 1), 3)
+    eng.build(1e-12, 0.0)
+    eng.set_group_cap(4)
+    eng.set_deterministic(True)
+    D1 = _density(eng.nbf, 13)
+    D2 = _density(eng.nbf, 14)
+    J1, K1 = eng.fock_jk(D1)
+    J2, K2 = eng.fock_jk(D2)
+    g = eng.capture_fock_jk(D1)
+    for rep in range(4):
+        print("deterministic graph replay %d" % rep, flush=True)
+        D, Jr, Kr = (D1, J1, K1) if rep % 2 == 0 else (D2, J2, K2)
+        J, K = g(D)
+        torch.cuda.synchronize()
+        assert torch.equal(J, Jr) and torch.equal(K, Kr)
+        Je, Ke = eng.fock_jk(D2)   # eager work between replays
+        torch.cuda.synchronize()
+        assert torch.equal(Je, J2) and torch.equal(Ke, K2)
+
+
+def test_h2o8_deterministic_against_atomic_build():
+    """The benchmark molecule: bit-identical repeats, within 1e-11 of the
+    FP64-atomic build, and the RHF energy functional to 1e-10 Eh."""
+    eng = _engine((2, 2, 2), 0)
+    eng.build(1e-12, 0.0)
+    D = _density(eng.nbf, 15)
+    Ja, Ka = eng.fock_jk(D)
+    eng.set_deterministic(True)
+    Jd, Kd = eng.fock_jk(D)
+    for _ in range(2):
+        J, K = eng.fock_jk(D)
+        assert torch.equal(J, Jd) and torch.equal(K, Kd)
+    assert _close(Jd, Ja) and _close(Kd, Ka)
+    ea = torch.sum(D * (2 * Ja - Ka)).item()
+    ed = torch.sum(D * (2 * Jd - Kd)).item()
+    assert abs(ea - ed) < 1e-10
+
+
+def test_batch_engine_deterministic():
+    from pyquante_amd.batch import BatchEngine, seeded_water_batch
+    be = BatchEngine(seeded_water_batch(nmol=6, nmax=3, seed=9))
+    be.build(1e-12, 0.0)
+    rng = np.random.default_rng(4)
+    blocks = []
+    for n in be.mol_nbf:
+        C = rng.standard_normal((n, 6)) * 0.1
+        blocks.append((C @ C.T).reshape(-1))
+    D = torch.as_tensor(np.concatenate(blocks), device="cuda")
+    Ja, Ka = be.fock_jk(D)
+    be.set_deterministic(True)
+    J0, K0 = be.fock_jk(D)
+    assert _close(J0, Ja) and _close(K0, Ka)
+    be.set_group_cap(4)
+    J1, K1 = be.fock_jk(D)
+    assert torch.equal(J1, J0) and torch.equal(K1, K0)
+
+
+def test_deterministic_rejects_two_densities_and_mixed():
+    from pyquante_amd import _abi
+    eng = _engine((1, 1, 1), 5)
+    eng.build(1e-12, 0.0)
+    eng.set_deterministic(True)
+    D = _density(eng.nbf, 16)
+    with pytest.raises(PQError):
+        eng.fock_jk_multi([D, D])
+    eng.build(1e-12, 1e-4)
+    with pytest.raises(PQError):
+        eng.fock_jk(D, precision=_abi.PQ_PREC_MIXED)
