@@ -175,10 +175,11 @@ int pq_engine_set_fused(pq_engine *eng, int on);
 
 /* on = 1: deterministic J/K.  pq_fock_jk then accumulates every contribution
  * as exact 64-bit integer limbs (resolution 2^-64, |contribution| < 2^62)
- * instead of FP64 atomics, and converts once at the end, so J and K are
- * bit-identical from call to call and independent of launch order, streams,
- * bra-run caps and graph replay, as the reference's sequential getJ/getK loops
- * are (Ints.py:195-223).  Single-density FP64 builds only (PQ_E_ARG
+ * instead of FP64 atomics, and converts once at the end, so for one launch
+ * configuration J and K are bit-identical from call to call whatever the
+ * order of execution (streams, graph replay), as the reference's sequential
+ * getJ/getK loops are (Ints.py:195-223).  Bra-run caps and the LDS limit
+ * select the kernels, whose integrals may differ in the last bits.  Single-density FP64 builds only (PQ_E_ARG
  * otherwise); costs extra atomics and three times the LDS per accumulator.
  * 0 (default) restores the FP64-atomic build. */
 int pq_engine_set_deterministic(pq_engine *eng, int on);

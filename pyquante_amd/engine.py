@@ -160,8 +160,9 @@ class Engine(object):
     def set_deterministic(self, on):
         """Bit-reproducible J/K (pq_engine_set_deterministic): exact integer
         fixed-point accumulation instead of FP64 atomics, so every build of
-        the same density gives the same bits, whatever the launch order,
-        streams, bra-run caps or graph replay.  Single-density FP64 builds."""
+        the same density with the same caps gives the same bits, whatever the
+        execution order (streams, graph replay).  Single-density FP64
+        builds."""
         check(lib.pq_engine_set_deterministic(self.handle, 1 if on else 0))
 
     def set_lds_limit(self, nbytes=0):
