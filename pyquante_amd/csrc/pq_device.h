@@ -95,12 +95,20 @@ __device__ __forceinline__ TileLane tile_lane(int nket_field, int lane) {
   const int nk = nket_field & 0xff, ks = 1 << ((nket_field >> 8) & 7);
   TileLane t;
   t.active = lane < nk * ks;
-  t.slot = t.active ? lane % nk : 0;
-  t.kc = t.active ? lane / nk : 0;
+  // lane / nk without an integer division: (lane + 1/2) / nk lies at least
+  // 1/(2 nk) >= 1/128 from an integer, far beyond the float reciprocal's
+  // error, so the truncation is exact
+  const int q = (int)(((float)lane + 0.5f) * __builtin_amdgcn_rcpf((float)nk));
+  t.slot = t.active ? lane - q * nk : 0;
+  t.kc = t.active ? q : 0;
   t.ks = ks;
   t.nk = nk;
   return t;
 }
+
+// First primitive pair of ket-primitive chunk kc of n pairs split ks ways
+// (ks a power of two): kc * n / ks.
+__device__ __forceinline__ int ket_chunk(int kc, int n, int ks) { return (kc * n) >> __builtin_ctz(ks); }
 
 // Everything a kernel needs about the basis and the pair data.
 struct EngineView {

@@ -79,6 +79,7 @@ struct ClassWork {
   std::vector<Tile> t64, t32;
   Tile* d_t64 = nullptr;
   Tile* d_t32 = nullptr;
+  std::vector<float> w64, w32;  // estimated work of each tile (for the run order)
   std::vector<int2> g64, g32;  // runs of tiles with one bra pair
   int2* d_g64 = nullptr;
   int2* d_g32 = nullptr;
@@ -500,6 +501,9 @@ int build_pair_data(pq_engine* e) {
 // about kPrimBudget primitive quartets: heavily contracted quartets are
 // spread over several tiles (their partial sums digest independently).
 constexpr int kPrimBudget = 36;
+// A tile's setup and digestion, in primitive quartets per lane (the bra-run
+// order's work estimate).
+constexpr int kTileFixedWork = 8;
 
 // Default tiles per bra run when no tuned value is set: a run saves
 // (tiles - 1) flushes of the bra K rows, which pays off when the rows are long.
@@ -520,17 +524,33 @@ int set_groups(ClassWork& cw, int prec, int cap) {
     if (g.empty() || t[i].bra != t[g.back().x].bra || g.back().y >= run) g.push_back(make_int2(i, 0));
     g.back().y += 1;
   }
+  // heaviest runs first: blocks are dispatched in index order as compute
+  // units free up, so the longest runs start early and the launch ends on
+  // short ones instead of a few long stragglers
+  const std::vector<float>& w = prec ? cw.w32 : cw.w64;
+  if (w.size() == t.size()) {
+    std::vector<float> gw(g.size(), 0.f);
+    for (size_t k = 0; k < g.size(); ++k)
+      for (int i = g[k].x; i < g[k].x + g[k].y; ++i) gw[k] += w[i];
+    std::vector<int> ord(g.size());
+    std::iota(ord.begin(), ord.end(), 0);
+    std::stable_sort(ord.begin(), ord.end(), [&](int a, int b) { return gw[a] > gw[b]; });
+    std::vector<int2> gs(g.size());
+    for (size_t k = 0; k < g.size(); ++k) gs[k] = g[ord[k]];
+    g.swap(gs);
+  }
   PQ_HIP(dev_alloc(&d, g.size()));
   if (!g.empty()) PQ_HIP(hipMemcpy(d, g.data(), g.size() * sizeof(int2), hipMemcpyHostToDevice));
   return PQ_OK;
 }
 
 // Rotates the tiles [first, end) of bra number i by a bra-dependent offset.
-void rotate_tiles(std::vector<Tile>& t, size_t first, size_t i) {
+void rotate_tiles(std::vector<Tile>& t, std::vector<float>& w, size_t first, size_t i) {
   const size_t cnt = t.size() - first;
   if (cnt < 2) return;
   const size_t r = (i * 2654435761u) % cnt;
   std::rotate(t.begin() + first, t.begin() + first + r, t.end());
+  std::rotate(w.begin() + first, w.begin() + first + r, w.end());
 }
 
 // Work lists over the pair copy P (its pp_n) and its per-class order S.
@@ -617,13 +637,27 @@ int build_work(pq_engine* e, std::vector<ClassWork>& out, double cut, double fp3
             while (ksplit && lg < 6 && (cnt << (lg + 1)) <= tile && (1 << (lg + 1)) <= kpn) ++lg;
             return cnt | (lg << 8);
           };
+          // work of a tile: primitive quartets per lane plus a fixed share
+          // for the tile's setup and digestion
+          int np_w = 0;
+          auto tile_work = [&](int nf) {
+            const int ks = 1 << ((nf >> 8) & 7);
+            return (float)(np_w * ((kpn + ks - 1) / ks) + kTileFixedWork);
+          };
           for (int p0 = 0; p0 < bpn; p0 += chunk) {
             const int np = std::min(chunk, bpn - p0);
             const int prim = np == bpn ? 0 : (p0 | (np << 16));
-            for (int k = g0; k < e64; k += tile)
-              cw.t64.push_back(Tile{bras[i], k, nket_field(std::min(tile, e64 - k)), prim});
-            for (int k = e64; k < end; k += tile)
-              cw.t32.push_back(Tile{bras[i], k, nket_field(std::min(tile, end - k)), prim});
+            np_w = np;
+            for (int k = g0; k < e64; k += tile) {
+              const int nf = nket_field(std::min(tile, e64 - k));
+              cw.t64.push_back(Tile{bras[i], k, nf, prim});
+              cw.w64.push_back(tile_work(nf));
+            }
+            for (int k = e64; k < end; k += tile) {
+              const int nf = nket_field(std::min(tile, end - k));
+              cw.t32.push_back(Tile{bras[i], k, nf, prim});
+              cw.w32.push_back(tile_work(nf));
+            }
           }
           cw.nq64 += e64 - g0;
           cw.nq32 += end - e64;
@@ -642,8 +676,8 @@ int build_work(pq_engine* e, std::vector<ClassWork>& out, double cut, double fp3
         // start each bra's tiles at a different ket tile: blocks that run
         // at the same time then add into different J[c,d] entries instead of
         // all contending for the atomics of the first kets
-        rotate_tiles(cw.t64, t64_0, i);
-        rotate_tiles(cw.t32, t32_0, i);
+        rotate_tiles(cw.t64, cw.w64, t64_0, i);
+        rotate_tiles(cw.t32, cw.w32, t32_0, i);
       }
       for (int prec = 0; prec < 2 && split_prims; ++prec) {
         const int tuned = e->group_cap[X][Y][prec];

@@ -195,18 +195,30 @@ __device__ __forceinline__ void hgp_jk_block(const EngineView& ev, const PairInf
 // Adds x into accumulator element p: an FP64 atomic (global memory, or LDS
 // when GLB is false), or in MODE_JKD the three fixed-point limbs of x into
 // the int64 sums at p, p + stride and p + 2 stride.
+//
+// The atomics go through pointers of their own address space (LDS or global
+// memory).  Through generic pointers the compiler merges an LDS branch and a
+// global branch of the same add into one FLAT atomic on a selected address,
+// which then takes the vector-memory path (and its latency) for the LDS
+// accumulators as well.
+template <typename T>
+__device__ __forceinline__ void atomic_add_as(T* p, T x, bool glb) {
+  using lds_t = __attribute__((address_space(3))) T;
+  using glb_t = __attribute__((address_space(1))) T;
+  if (glb) __hip_atomic_fetch_add((glb_t*)p, x, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  else __hip_atomic_fetch_add((lds_t*)p, x, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+
 template <int MODE, bool GLB>
 __device__ __forceinline__ void acc_add(double* p, int64_t stride, double x) {
   if constexpr (MODE == MODE_JKD) {
     const FxLimbs l = fx_split(x);
     unsigned long long* q = reinterpret_cast<unsigned long long*>(p);
-    if (l.l0) atomicAdd(q, (unsigned long long)l.l0);
-    if (l.l1) atomicAdd(q + stride, (unsigned long long)l.l1);
-    if (l.l2) atomicAdd(q + 2 * stride, (unsigned long long)l.l2);
-  } else if constexpr (GLB) {
-    unsafeAtomicAdd(p, x);
+    if (l.l0) atomic_add_as(q, (unsigned long long)l.l0, GLB);
+    if (l.l1) atomic_add_as(q + stride, (unsigned long long)l.l1, GLB);
+    if (l.l2) atomic_add_as(q + 2 * stride, (unsigned long long)l.l2, GLB);
   } else {
-    atomicAdd(p, x);
+    atomic_add_as(p, x, GLB);
   }
 }
 
@@ -220,12 +232,12 @@ __device__ __forceinline__ void acc_flush(const double* lds, int64_t i, int64_t 
     const long long* l = reinterpret_cast<const long long*>(lds);
     const long long s0 = l[i], s1 = l[i + lstride], s2 = l[i + 2 * lstride];
     unsigned long long* q = reinterpret_cast<unsigned long long*>(p);
-    if (s0) atomicAdd(q, (unsigned long long)s0);
-    if (s1) atomicAdd(q + gstride, (unsigned long long)s1);
-    if (s2) atomicAdd(q + 2 * gstride, (unsigned long long)s2);
+    if (s0) atomic_add_as(q, (unsigned long long)s0, true);
+    if (s1) atomic_add_as(q + gstride, (unsigned long long)s1, true);
+    if (s2) atomic_add_as(q + 2 * gstride, (unsigned long long)s2, true);
   } else {
     const double v = lds[i];
-    if (v != 0.0) unsafeAtomicAdd(p, v);
+    if (v != 0.0) atomic_add_as(p, v, true);
   }
 }
 
@@ -439,7 +451,7 @@ __device__ __forceinline__ void lane_body(const EngineView& ev, const Tile& tl, 
   T out[NO];
   if (active) {
     const int i0 = tl.prim & 0xffff, ni = tl.prim >> 16;
-    const int k0 = kc * kp.pp_n / ks, k1 = (kc + 1) * kp.pp_n / ks;
+    const int k0 = ket_chunk(kc, kp.pp_n, ks), k1 = ket_chunk(kc + 1, kp.pp_n, ks);
     if constexpr (kHgpJK<LA, LB, LC, LD, T, MODE>) {
       static_assert(NAS == C::NA && NBS == C::NB, "the HGP J/K classes are unsplit");
       hgp_jk_block<LA, LB, LC, LD, LT>(ev, bp, kp, i0, ni ? i0 + ni : bp.pp_n, k0, k1, out,
@@ -636,19 +648,19 @@ __device__ __forceinline__ void group_body(const EngineView& ev, const Tile* __r
   for (int i = threadIdx.x; i < ND * NL * (C::NA + C::NB) * n; i += blockDim.x) krow[i] = 0.0;
   const PairInfo bp = ld_uniform(ev.pairs + tiles[g.x].bra);
   const int ms = bp.mstride;
-  const int rows = (C::NA + C::NB) * ms, region = (C::NA + C::NB) * n;
+  const int region = (C::NA + C::NB) * n;
   // J bra block accumulator after the K rows (and the staged D rows)
   double* jrow = krow + ND * region * (NL + (stage_d ? 1 : 0));
   for (int i = threadIdx.x; i < ND * NL * C::NA * C::NB; i += blockDim.x) jrow[i] = 0.0;
   double* drow = nullptr;
   if (stage_d) {
     drow = krow + ND * NL * region;
-    for (int i = threadIdx.x; i < ND * rows; i += blockDim.x) {
-      const int d = i / rows, ir = i - d * rows;
-      const int r = ir / ms, col = ir - r * ms;
-      const int grow = r < C::NA ? bp.bfA + r : bp.bfB + (r - C::NA);
-      drow[d * region + ir] = D[(size_t)d * ev.msize + bp.moff + grow * ms + col];
-    }
+    for (int d = 0; d < ND; ++d)
+      for (int r = 0; r < C::NA + C::NB; ++r) {
+        const int grow = r < C::NA ? bp.bfA + r : bp.bfB + (r - C::NA);
+        for (int col = threadIdx.x; col < ms; col += blockDim.x)
+          drow[d * region + r * ms + col] = D[(size_t)d * ev.msize + bp.moff + grow * ms + col];
+      }
   }
   __syncthreads();
   for (int tt = wv; tt < g.y; tt += 4) {
@@ -671,17 +683,28 @@ __device__ __forceinline__ void group_body(const EngineView& ev, const Tile* __r
                     &Jacc[(size_t)d * ev.msize + bp.moff + (bp.bfA + ab / C::NB) * ms + bp.bfB + ab % C::NB],
                     ev.msize);
   }
-  for (int i = threadIdx.x; i < ND * rows; i += blockDim.x) {
-    const int d = i / rows, ir = i - d * rows;
-    const int r = ir / ms, col = ir - r * ms;
-    const int grow = r < C::NA ? bp.bfA + r : bp.bfB + (r - C::NA);
-    acc_flush<MODE>(krow, d * region + ir, region, &Kacc[(size_t)d * ev.msize + bp.moff + grow * ms + col],
-                    ev.msize);
-  }
+  for (int d = 0; d < ND; ++d)
+    for (int r = 0; r < C::NA + C::NB; ++r) {
+      const int grow = r < C::NA ? bp.bfA + r : bp.bfB + (r - C::NA);
+      for (int col = threadIdx.x; col < ms; col += blockDim.x)
+        acc_flush<MODE>(krow, d * region + r * ms + col, region,
+                        &Kacc[(size_t)d * ev.msize + bp.moff + grow * ms + col], ev.msize);
+    }
 }
 
+// Minimum waves per SIMD that the J/K class kernels' registers must allow
+// (PQ_WPE, set per class translation unit; 0 leaves it to the compiler).
+#ifndef PQ_WPE
+#define PQ_WPE 0
+#endif
+#if PQ_WPE > 0
+#define PQ_WPE_ATTR __attribute__((amdgpu_waves_per_eu(PQ_WPE)))
+#else
+#define PQ_WPE_ATTR
+#endif
+
 template <int LA, int LB, int LC, int LD, typename T, int MODE>
-__global__ __launch_bounds__(256) void k_eri_group(EngineView ev, const Tile* __restrict__ tiles,
+__global__ __launch_bounds__(256) PQ_WPE_ATTR void k_eri_group(EngineView ev, const Tile* __restrict__ tiles,
                                                    const int2* __restrict__ groups, int ngroups,
                                                    const int* __restrict__ kets, const double* __restrict__ D,
                                                    double* Jacc, double* Kacc, int stage_d) {
@@ -701,7 +724,7 @@ __host__ __device__ constexpr size_t group_lds_bytes(int nbf, bool stage_d) {
 }
 
 template <int LA, int LB, int LC, int LD, typename T, int MODE>
-__global__ __launch_bounds__(256) void k_eri_lane(EngineView ev, const Tile* __restrict__ tiles, int ntiles,
+__global__ __launch_bounds__(256) PQ_WPE_ATTR void k_eri_lane(EngineView ev, const Tile* __restrict__ tiles, int ntiles,
                                                   const int* __restrict__ kets, const double* __restrict__ D,
                                                   double* Jacc, double* Kacc, double* ints,
                                                   double* qout, int lds_mode) {
@@ -761,14 +784,14 @@ __global__ __launch_bounds__(256) void k_eri_lane(EngineView ev, const Tile* __r
   // flush the K rows (coalesced) and the ket-side J accumulator
   const PairInfo bp = ev.pairs[tl.bra];
   const int ms = bp.mstride;
-  const int rows = (C::NA + C::NB) * ms, region = (C::NA + C::NB) * n;
-  for (int i = threadIdx.x; i < ND * rows; i += blockDim.x) {
-    const int d = i / rows, ir = i - d * rows;
-    const int r = ir / ms, col = ir - r * ms;
-    const int grow = r < C::NA ? bp.bfA + r : bp.bfB + (r - C::NA);
-    acc_flush<MODE>(krow, d * region + ir, region, &Kacc[(size_t)d * ev.msize + bp.moff + grow * ms + col],
-                    ev.msize);
-  }
+  const int region = (C::NA + C::NB) * n;
+  for (int d = 0; d < ND; ++d)
+    for (int r = 0; r < C::NA + C::NB; ++r) {
+      const int grow = r < C::NA ? bp.bfA + r : bp.bfB + (r - C::NA);
+      for (int col = threadIdx.x; col < ms; col += blockDim.x)
+        acc_flush<MODE>(krow, d * region + r * ms + col, region,
+                        &Kacc[(size_t)d * ev.msize + bp.moff + grow * ms + col], ev.msize);
+    }
   if constexpr (C::SPLIT != 0) {
     constexpr int NCD = C::NC * C::ND;
     constexpr int Y = pair_class_of(LC, LD);

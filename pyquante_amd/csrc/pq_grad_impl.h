@@ -69,7 +69,7 @@ __device__ __forceinline__ void grad_lane(const EngineView& ev, const Tile& tl, 
     const double CD[3] = {kp.ABx, kp.ABy, kp.ABz};
     // the next ket primitive pair is loaded while the current one is
     // processed (its latency hides behind the bra loop)
-    const int k0 = kc * kp.pp_n / ks, k1 = (kc + 1) * kp.pp_n / ks;
+    const int k0 = ket_chunk(kc, kp.pp_n, ks), k1 = ket_chunk(kc + 1, kp.pp_n, ks);
     PrimPair qn{};
     double gn = 0.0;
     if (k0 < k1) {

@@ -58,7 +58,7 @@ __device__ __forceinline__ void hgp_tile(const EngineView& ev, const Tile& tl, c
   const PairInfo bp = ev.pairs[bra], kp = ev.pairs[ket];
   const int i0 = tl.prim & 0xffff, ni = tl.prim >> 16;
   const int i1 = ni ? i0 + ni : bp.pp_n;
-  const int k0 = tn.kc * kp.pp_n / tn.ks, k1 = (tn.kc + 1) * kp.pp_n / tn.ks;
+  const int k0 = ket_chunk(tn.kc, kp.pp_n, tn.ks), k1 = ket_chunk(tn.kc + 1, kp.pp_n, tn.ks);
   // a lane holding part of the primitive sum adds its block into the zeroed
   // array; a whole quartet is stored
   const bool part = tl.prim != 0 || tn.ks > 1;
