@@ -327,6 +327,17 @@ int pq_get1ints(pq_engine *eng, double *S_dev, double *T_dev, double *V_dev, int
  * integrals are contracted on the fly over the current (FP64) work lists. */
 int pq_forces(pq_engine *eng, const double *D_dev, const double *W_dev, double *grad_dev, void *stream);
 
+/* UHF nuclear gradient (force.py:72-89 uhf_force: der_oneeE and der_dmat of
+ * each spin plus der_twoeE_uhf and der_enuke), natom x 3 into grad_dev:
+ * tr(P dH) + 1/2 sum (ij|kl)^x [P_ij P_kl - Pa_ik Pa_jl - Pb_ik Pb_jl]
+ * - tr((Qa + Qb) dS) + dV_nn, with the spin densities Pa = Ca_occ Ca_occ^T,
+ * Pb likewise, P = Pa + Pb and Qs = Cs_occ diag(es_occ) Cs_occ^T
+ * (Wavefunction.py:143-160).  The caller passes Da_dev = Pa, Db_dev = Pb and,
+ * for the one-electron terms, D_dev = P / 2 and W_dev = (Qa + Qb) / 2 (the
+ * closed-shell convention of pq_forces, whose gradient this is for Pa = Pb). */
+int pq_forces_uhf(pq_engine *eng, const double *Da_dev, const double *Db_dev, const double *D_dev,
+                  const double *W_dev, double *grad_dev, void *stream);
+
 /* pq_forces with HIP events around every kernel launch, recorded on the
  * stream it is launched on: ms[2c] / ms[2c+1] the two-electron derivative
  * kernel of class c over its FP64 / FP32-bucket tiles (21 classes, the order

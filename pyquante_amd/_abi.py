@@ -83,6 +83,7 @@ _proto("pq_digest_stored_noalloc", [_I, _VP, _U64, _VP, _VP, _VP, _VP])
 _proto("pq_unpack_eri", [_I, _VP, _U64, _I, _I, _VP, _VP])
 _proto("pq_get1ints", [_VP, _VP, _VP, _VP, _I, _VP])
 _proto("pq_forces", [_VP, _VP, _VP, _VP, _VP])
+_proto("pq_forces_uhf", [_VP, _VP, _VP, _VP, _VP, _VP, _VP])
 _proto("pq_forces_timed", [_VP, _VP, _VP, _VP, _VP, _c.POINTER(_c.c_float)])
 
 

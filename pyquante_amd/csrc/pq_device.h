@@ -134,6 +134,9 @@ struct EngineView {
   // dynamic LDS a J/K launch may request, bytes (the device's per-workgroup
   // limit unless lowered with pq_engine_set_lds_limit)
   int lds_max;
+  // gradient kernels: the beta density of an unrestricted wavefunction (the
+  // kernels' D is then the alpha density), null for a closed shell
+  const double* grad_db;
 };
 
 // pair class of shells with l1 >= l2 (ss, ps, pp, ds, dp, dd)

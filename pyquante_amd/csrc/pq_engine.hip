@@ -1781,9 +1781,13 @@ int pq_shell_quartet_blocks(pq_engine* eng, int nq, const int* shells_host, doub
   return PQ_OK;
 }
 
-static int forces_impl(pq_engine* eng, const double* D_dev, const double* W_dev, double* grad_dev, void* stream,
-                       float* ms) {
+// D_dev, W_dev: the one-electron terms' density and energy-weighted density;
+// Da_dev, Db_dev: the spin densities of the two-electron term (both null for
+// a closed shell, whose two-electron term uses D_dev).
+static int forces_impl(pq_engine* eng, const double* D_dev, const double* W_dev, const double* Da_dev,
+                       const double* Db_dev, double* grad_dev, void* stream, float* ms) {
   if (!eng || !D_dev || !W_dev || !grad_dev) return fail(PQ_E_ARG, "null argument");
+  if ((Da_dev == nullptr) != (Db_dev == nullptr)) return fail(PQ_E_ARG, "both spin densities are needed");
   if (!eng->work_ok) return fail(PQ_E_STATE, "pq_build_pairs must be called before pq_forces");
   hipStream_t st = (hipStream_t)stream;
   const int ncopy = kGradCopies;
@@ -1830,8 +1834,10 @@ static int forces_impl(pq_engine* eng, const double* D_dev, const double* W_dev,
         if (tl.empty()) continue;
         hipStream_t s = ks();
         if (ms) PQ_TRY(mark(s));
-        PQ_TRY(launch_grad_class(la, lb, lc, ld, eng->view_s(), prec ? c.d_t32 : c.d_t64, (int)tl.size(),
-                                 eng->d_sorted_s[c.Y], D_dev, eng->d_gacc, eng->natom, s));
+        EngineView gview = eng->view_s();
+        gview.grad_db = Db_dev;
+        PQ_TRY(launch_grad_class(la, lb, lc, ld, gview, prec ? c.d_t32 : c.d_t64, (int)tl.size(),
+                                 eng->d_sorted_s[c.Y], Da_dev ? Da_dev : D_dev, eng->d_gacc, eng->natom, s));
         if (ms) {
           PQ_TRY(mark(s));
           slot.push_back((int)(2 * ci + prec));
@@ -1875,13 +1881,19 @@ static int forces_impl(pq_engine* eng, const double* D_dev, const double* W_dev,
 }
 
 int pq_forces(pq_engine* eng, const double* D_dev, const double* W_dev, double* grad_dev, void* stream) {
-  return forces_impl(eng, D_dev, W_dev, grad_dev, stream, nullptr);
+  return forces_impl(eng, D_dev, W_dev, nullptr, nullptr, grad_dev, stream, nullptr);
+}
+
+int pq_forces_uhf(pq_engine* eng, const double* Da_dev, const double* Db_dev, const double* D_dev,
+                  const double* W_dev, double* grad_dev, void* stream) {
+  if (!Da_dev || !Db_dev) return fail(PQ_E_ARG, "null argument");
+  return forces_impl(eng, D_dev, W_dev, Da_dev, Db_dev, grad_dev, stream, nullptr);
 }
 
 int pq_forces_timed(pq_engine* eng, const double* D_dev, const double* W_dev, double* grad_dev, void* stream,
                     float* ms) {
   if (!ms) return fail(PQ_E_ARG, "null timing array");
-  return forces_impl(eng, D_dev, W_dev, grad_dev, stream, ms);
+  return forces_impl(eng, D_dev, W_dev, nullptr, nullptr, grad_dev, stream, ms);
 }
 
 int pq_get1ints(pq_engine* eng, double* S_dev, double* T_dev, double* V_dev, int compat_nr, void* stream) {

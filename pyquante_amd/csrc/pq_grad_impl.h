@@ -9,8 +9,13 @@
 // Gaussian d/dA_x g(l) = 2 alpha g(l+1) - l g(l-1) is applied on the fly, and
 // the result is contracted at once with the density factor
 //   Gamma = f (16 D_ab D_cd - 4 (D_ac D_bd + D_ad D_bc))
-// (f the permutational degeneracy of the unique quartet).  The D-centre
-// derivative follows from translational invariance.
+// (f the permutational degeneracy of the unique quartet).  For an
+// unrestricted wavefunction (uhf_force, force.py:72-89, whose two-electron
+// energy is 1/2 sum (ij|kl) [P_ij P_kl - Pa_ik Pa_jl - Pb_ik Pb_jl] with
+// P = Pa + Pb) the factor is
+//   Gamma = f (4 P_ab P_cd - 2 sum_s (Ps_ac Ps_bd + Ps_ad Ps_bc)),
+// which is the closed-shell one for Pa = Pb = D.  The D-centre derivative
+// follows from translational invariance.
 #pragma once
 #include "pq_eri_impl.h"
 
@@ -44,6 +49,8 @@ __device__ __forceinline__ void grad_lane(const EngineView& ev, const Tile& tl, 
   double G[NO];
   {
     const double f = active ? quartet_degeneracy(bp, kp, bra, ket) : 0.0;
+    const double* Db = ev.grad_db;
+    if (Db) Db += bp.moff;
 #pragma unroll
     for (int a = 0; a < NAS; ++a)
 #pragma unroll
@@ -54,9 +61,16 @@ __device__ __forceinline__ void grad_lane(const EngineView& ev, const Tile& tl, 
 #pragma unroll
           for (int d = 0; d < C::ND; ++d) {
             const double nf = cart_norm(LA, A0 + a) * cart_norm(LB, B0 + b) * cart_norm(LC, c) * cart_norm(LD, d);
-            const double g = 16.0 * dab * D[(c0 + c) * n + d0 + d] -
-                             4.0 * (D[(a0 + a) * n + c0 + c] * D[(b0 + b) * n + d0 + d] +
-                                    D[(a0 + a) * n + d0 + d] * D[(b0 + b) * n + c0 + c]);
+            const int ab = (a0 + a) * n + b0 + b, cd = (c0 + c) * n + d0 + d;
+            const int ac = (a0 + a) * n + c0 + c, bd = (b0 + b) * n + d0 + d;
+            const int ad = (a0 + a) * n + d0 + d, bc = (b0 + b) * n + c0 + c;
+            double g;
+            if (Db) {
+              g = 4.0 * (dab + Db[ab]) * (D[cd] + Db[cd]) -
+                  2.0 * (D[ac] * D[bd] + D[ad] * D[bc] + Db[ac] * Db[bd] + Db[ad] * Db[bc]);
+            } else {
+              g = 16.0 * dab * D[cd] - 4.0 * (D[ac] * D[bd] + D[ad] * D[bc]);
+            }
             G[((a * NBS + b) * C::NC + c) * C::ND + d] = f * nf * g;
           }
       }

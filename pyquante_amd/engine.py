@@ -365,6 +365,21 @@ class Engine(object):
                             ctypes.c_void_p(g.data_ptr()), _stream()))
         return g
 
+    def gradient_uhf(self, Da, Db, Qa, Qb):
+        """dE/dR (natom x 3, hartree/bohr) of the UHF energy at spin densities
+        Da, Db and energy-weighted densities Qa, Qb (pq_forces_uhf)."""
+        if self.cut is None:
+            self.build()
+        Da, Db = self._dtensor(Da), self._dtensor(Db)
+        Qa, Qb = self._dtensor(Qa), self._dtensor(Qb)
+        D = (0.5 * (Da + Db)).contiguous()
+        W = (0.5 * (Qa + Qb)).contiguous()
+        g = torch.empty((self.natom, 3), dtype=torch.float64, device=self.device)
+        check(lib.pq_forces_uhf(self.handle, ctypes.c_void_p(Da.data_ptr()), ctypes.c_void_p(Db.data_ptr()),
+                                ctypes.c_void_p(D.data_ptr()), ctypes.c_void_p(W.data_ptr()),
+                                ctypes.c_void_p(g.data_ptr()), _stream()))
+        return g
+
     def gradient_timed(self, D, W):
         """gradient with per-kernel times in ms (pq_forces_timed): returns
         (dE/dR, ms[49]): ms[2c + prec] the derivative kernel of class c over

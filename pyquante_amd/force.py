@@ -41,6 +41,33 @@ class Wavefunction(object):
         return (c * self.orbe[:self.nclosed]) @ c.T
 
 
+class UHFWavefunction(object):
+    """Unrestricted wavefunction: orbitals (orbsa, orbsb) (columns), energies
+    (orbea, orbeb), nalpha and nbeta occupied orbitals; mkdens and mkQmatrix
+    return the alpha and beta matrices (Wavefunction.py:143-160, unrestricted
+    branch)."""
+
+    restricted = False
+    unrestricted = True
+    fixedocc = False
+
+    def __init__(self, orbs, orbe, nalpha, nbeta):
+        self.orbs = orbs
+        self.orbe = orbe
+        self.nalpha = nalpha
+        self.nbeta = nbeta
+
+    def mkdens(self):
+        ca = self.orbs[0][:, :self.nalpha]
+        cb = self.orbs[1][:, :self.nbeta]
+        return ca @ ca.T, cb @ cb.T
+
+    def mkQmatrix(self):
+        ca = self.orbs[0][:, :self.nalpha]
+        cb = self.orbs[1][:, :self.nbeta]
+        return (ca * self.orbe[0][:self.nalpha]) @ ca.T, (cb * self.orbe[1][:self.nbeta]) @ cb.T
+
+
 def scf(engine, mol, ConvCriteria=1e-10, MaxIter=100, DoAveraging=True, precision=0, graph=False):
     """RHF on an engine (hartree_fock.rhf semantics, DIIS on by default here
     for tight convergence).  Returns (E, orbe, orbs, D, W) as device tensors.
@@ -100,11 +127,48 @@ def rhf_force(mol, wf, bset, engine=None):
     return -g
 
 
+def uhf_force(mol, wf, bset, engine=None):
+    """force.py:72-89: sets atom.forces = -dE/dR_A for an unrestricted
+    wavefunction (spin densities and energy-weighted densities of
+    wf.mkdens / wf.mkQmatrix)."""
+    eng = engine if engine is not None else Engine(bset, mol)
+    if eng.cut is None:
+        eng.build()
+    dev = eng.device
+    Da, Db = (torch.as_tensor(np.asarray(x, dtype=np.float64), device=dev) for x in wf.mkdens())
+    Qa, Qb = (torch.as_tensor(np.asarray(x, dtype=np.float64), device=dev) for x in wf.mkQmatrix())
+    g = eng.gradient_uhf(Da, Db, Qa, Qb).cpu().numpy()
+    for atom, gi in zip(mol.atoms, g):
+        atom.forces = -gi
+    return -g
+
+
+def uhf_gradient(mol, basis_data="6-31g**", ConvCriteria=1e-11, MaxIter=400, schwarz_cut=1e-12):
+    """(E, dE/dR) for an unrestricted wavefunction of mol (its multiplicity),
+    from hartree_fock.uhf."""
+    from .hartree_fock import uhf
+    bs = BasisSet(mol, basis_data)
+    # full-precision Boys values in V: the one-electron derivative kernels
+    # differentiate that V
+    E, orbe, orbs = uhf(mol, bfs=bs.bfs, ConvCriteria=ConvCriteria, MaxIter=MaxIter, schwarz_cut=schwarz_cut,
+                        reference_boys=False)
+    na, nb = mol.get_alphabeta()
+    wf = UHFWavefunction(orbs, orbe, na, nb)
+    eng = Engine(bs, mol)
+    eng.build(schwarz_cut, 0.0)
+    Da, Db = (torch.as_tensor(x, device=eng.device) for x in wf.mkdens())
+    Qa, Qb = (torch.as_tensor(x, device=eng.device) for x in wf.mkQmatrix())
+    return E, eng.gradient_uhf(Da, Db, Qa, Qb)
+
+
 def hf_force(mol, wf, bname):
-    """force.py:31-46 (restricted wavefunctions)."""
-    if not wf.restricted:
-        raise NotImplementedError("only restricted wavefunctions have GPU gradients here")
-    return rhf_force(mol, wf, BasisSet(mol, bname))
+    """force.py:31-46: forces of a restricted or unrestricted wavefunction
+    (the reference's fixed-occupation branch has no counterpart here)."""
+    if wf.restricted:
+        return rhf_force(mol, wf, BasisSet(mol, bname))
+    if wf.unrestricted:
+        return uhf_force(mol, wf, BasisSet(mol, bname))
+    raise NotImplementedError("fixed-occupation wavefunctions have no GPU gradient here")
 
 
 class RHFEnergy(torch.autograd.Function):

@@ -67,7 +67,7 @@ def test_autograd_function_and_hf_force():
     E = rhf_energy(R, mol, "sto-3g")
     E.backward()
     E2, g = rhf_gradient(mol, "sto-3g")
-    assert abs(float(E) - E2) < 1e-9
+    assert abs(float(E.detach()) - E2) < 1e-9
     assert torch.max(torch.abs(R.grad - g.cpu())).item() < 1e-9
 
 
@@ -94,3 +94,48 @@ def test_gradient_streams_agree():
     assert torch.max(torch.abs(g0 - g4)).item() < 1e-10
     # translational invariance holds for any D, W
     assert torch.max(torch.abs(g4.sum(dim=0))).item() < 1e-8
+
+
+def _oh_radical():
+    from pyquante_amd.molecule import Molecule
+    return Molecule("oh", [(8, (0.0, 0.05, -0.02)), (1, (1.81, 0.12, 0.06))], multiplicity=2)
+
+
+def test_uhf_gradient_closed_shell_equals_rhf():
+    """force.py:72-89 on a closed shell: with equal spin densities the UHF
+    gradient is the RHF one."""
+    from pyquante_amd.force import rhf_gradient, uhf_gradient
+    mol = _h2o_bent()
+    e_r, g_r = rhf_gradient(mol, "sto-3g", ConvCriteria=1e-12)
+    e_u, g_u = uhf_gradient(mol, "sto-3g", ConvCriteria=1e-12)
+    assert abs(e_u - e_r) < 1e-9
+    assert torch.max(torch.abs(g_u - g_r)).item() < 1e-7
+
+
+def test_uhf_gradient_vs_own_finite_difference_oh():
+    """UHF forces of the OH radical (doublet) against central finite
+    differences (h = 1e-4 bohr) of the GPU UHF energy, every coordinate;
+    translational invariance; hf_force dispatch on an unrestricted
+    wavefunction."""
+    from pyquante_amd.basis import BasisSet
+    from pyquante_amd.force import UHFWavefunction, hf_force, uhf_gradient
+    from pyquante_amd.hartree_fock import uhf
+    mol = _oh_radical()
+    E, g = uhf_gradient(mol, "6-31g**", ConvCriteria=1e-12)
+    g = g.cpu().numpy()
+    assert np.max(np.abs(g.sum(0))) < 1e-8
+    X = mol.coords()
+    h = 1e-4
+    for a in range(2):
+        for x in range(3):
+            Xp = X.copy(); Xp[a, x] += h
+            Xm = X.copy(); Xm[a, x] -= h
+            ep = uhf_gradient(mol.copy_with_coords(Xp), "6-31g**", ConvCriteria=1e-12)[0]
+            em = uhf_gradient(mol.copy_with_coords(Xm), "6-31g**", ConvCriteria=1e-12)[0]
+            fd = (ep - em) / (2 * h)
+            assert abs(fd - g[a, x]) < 1e-6, (a, x, fd, g[a, x])
+    bs = BasisSet(mol, "6-31g**")
+    e2, orbe, orbs = uhf(mol, bfs=bs.bfs, ConvCriteria=1e-12, MaxIter=400, reference_boys=False)
+    na, nb = mol.get_alphabeta()
+    f = hf_force(mol, UHFWavefunction(orbs, orbe, na, nb), "6-31g**")
+    assert np.max(np.abs(f + g)) < 1e-9
