@@ -6,6 +6,7 @@
 #include <stdint.h>
 
 #include "boys_table.h"
+#include "pq_types.h"
 #include "rys_table.h"
 #include "rys_table32.h"
 
@@ -16,7 +17,6 @@ namespace pq {
 // angular momentum L, x-power runs from L down, then y-power from L-lx down.
 // D = xx, xy, xz, yy, yz, zz.
 // ---------------------------------------------------------------------
-__host__ __device__ constexpr int ncart(int L) { return (L + 1) * (L + 2) / 2; }
 
 __host__ __device__ constexpr int cart_pow(int L, int i, int d) {
   int idx = 0;
@@ -47,38 +47,6 @@ __host__ __device__ constexpr double cart_norm(int L, int i) {
   return inv_sqrt_dfprod(dfact(2 * cart_pow(L, i, 0) - 1) * dfact(2 * cart_pow(L, i, 1) - 1) *
                          dfact(2 * cart_pow(L, i, 2) - 1));
 }
-
-// ---------------------------------------------------------------------
-// Shell-pair records in HBM.  A pair (A,B) is stored with l_A >= l_B.
-// ---------------------------------------------------------------------
-struct __align__(16) PrimPair {
-  double p;    // alpha_a + alpha_b
-  double rp;   // 1 / p
-  double Kp;   // K / p, K = c_a c_b Nrad_a Nrad_b exp(-alpha_a alpha_b |AB|^2 / p)
-  double Px, Py, Pz;
-};
-
-// Matrices (D, J, K, S, ...) are stored per molecule: molecule m's n_m x n_m
-// block starts at element moff and has row stride mstride = n_m; bfA/bfB are
-// function indices within the molecule.  A single-molecule engine has
-// moff = 0 and mstride = nbf.  Pairs never mix molecules.
-struct __align__(16) PairInfo {
-  double Ax, Ay, Az;     // centre of shell A
-  double ABx, ABy, ABz;  // A - B
-  int pp_off, pp_n;      // primitive pairs [pp_off, pp_off + pp_n)
-  int bfA, bfB;          // first basis function of each shell (within its molecule)
-  int shA, shB;          // shell indices
-  int atA, atB;          // atom indices
-  int moff, mstride;     // the molecule's matrix block: first element, row stride
-  int mat0, mnat;        // the molecule's atoms [mat0, mat0 + mnat)
-};
-
-struct Tile {
-  int bra;   // pair id
-  int ket0;  // first entry of the ket pair list covered by this tile
-  int nket;  // number of kets (bits 0-7) and log2 of the ket-primitive split ks (bits 8-10)
-  int prim;  // bra primitive-pair range: first (low 16 bits), count (high 16 bits; 0 = all)
-};
 
 // Lanes of a tile: with nk kets and a ket-primitive split ks, lane l takes
 // ket l % nk and the (l / nk)-th of ks near-equal chunks of that ket's

@@ -16,6 +16,7 @@
 #include "../../include/pq_abi.h"
 #include "pq_device.h"
 #include "pq_kernels.h"
+#include "pq_worklist.h"
 
 using namespace pq;
 
@@ -49,7 +50,6 @@ void dev_free(T*& p) {
 }
 
 int pair_class(int l1, int l2) { return l1 * (l1 + 1) / 2 + l2; }  // l1 >= l2
-const int kPairL[6][2] = {{0, 0}, {1, 0}, {1, 1}, {2, 0}, {2, 1}, {2, 2}};
 
 double fact2(int n) { return n <= 1 ? 1.0 : n * fact2(n - 2); }
 
@@ -74,21 +74,15 @@ double rys_flops_dig(int la, int lb, int lc, int ld) {
   return 12.0 * ncart(la) * ncart(lb) * ncart(lc) * ncart(ld);
 }
 
-struct ClassWork {
-  int X, Y;  // pair classes, X >= Y
-  std::vector<Tile> t64, t32;
+// A class work list with its device copies (tiles and bra runs).
+struct ClassWork : ClassTiles {
   Tile* d_t64 = nullptr;
   Tile* d_t32 = nullptr;
-  std::vector<float> w64, w32;  // estimated work of each tile (for the run order)
-  std::vector<int2> g64, g32;  // runs of tiles with one bra pair
   int2* d_g64 = nullptr;
   int2* d_g32 = nullptr;
-  int cap[2] = {0, 0};         // tiles per run (0 = no runs) for FP64 / FP32 tiles, | PQ_CAP_STAGE_D
   int gflags(int prec) const { return (cap[prec] & PQ_CAP_STAGE_D) ? kGroupStageD : 0; }
-  uint64_t nq64 = 0, nq32 = 0;
-  double np64 = 0, np32 = 0;
-  double nf64 = 0, nf32 = 0;  // unique function quartets
 };
+static_assert(sizeof(Run) == sizeof(int2), "bra runs are uploaded as int2");
 
 }  // namespace
 
@@ -497,14 +491,6 @@ int build_pair_data(pq_engine* e) {
   return PQ_OK;
 }
 
-// Bra primitive pairs per tile are capped so that one lane evaluates at most
-// about kPrimBudget primitive quartets: heavily contracted quartets are
-// spread over several tiles (their partial sums digest independently).
-constexpr int kPrimBudget = 36;
-// A tile's setup and digestion, in primitive quartets per lane (the bra-run
-// order's work estimate).
-constexpr int kTileFixedWork = 8;
-
 // Default tiles per bra run when no tuned value is set: a run saves
 // (tiles - 1) flushes of the bra K rows, which pays off when the rows are long.
 int default_group_cap(int nbf) { return nbf >= 300 ? 16 : 0; }
@@ -512,45 +498,13 @@ int default_group_cap(int nbf) { return nbf >= 300 ? 16 : 0; }
 // Rebuilds the bra runs of one precision of a class for a new cap (0 = none)
 // and uploads them.
 int set_groups(ClassWork& cw, int prec, int cap) {
-  const std::vector<Tile>& t = prec ? cw.t32 : cw.t64;
-  std::vector<int2>& g = prec ? cw.g32 : cw.g64;
   int2*& d = prec ? cw.d_g32 : cw.d_g64;
-  g.clear();
   dev_free(d);
-  cw.cap[prec] = cap;
-  const int run = cap > 0 ? cap & (PQ_CAP_STAGE_D - 1) : 0;
-  if (run <= 0) return PQ_OK;
-  for (int i = 0; i < (int)t.size(); ++i) {
-    if (g.empty() || t[i].bra != t[g.back().x].bra || g.back().y >= run) g.push_back(make_int2(i, 0));
-    g.back().y += 1;
-  }
-  // heaviest runs first: blocks are dispatched in index order as compute
-  // units free up, so the longest runs start early and the launch ends on
-  // short ones instead of a few long stragglers
-  const std::vector<float>& w = prec ? cw.w32 : cw.w64;
-  if (w.size() == t.size()) {
-    std::vector<float> gw(g.size(), 0.f);
-    for (size_t k = 0; k < g.size(); ++k)
-      for (int i = g[k].x; i < g[k].x + g[k].y; ++i) gw[k] += w[i];
-    std::vector<int> ord(g.size());
-    std::iota(ord.begin(), ord.end(), 0);
-    std::stable_sort(ord.begin(), ord.end(), [&](int a, int b) { return gw[a] > gw[b]; });
-    std::vector<int2> gs(g.size());
-    for (size_t k = 0; k < g.size(); ++k) gs[k] = g[ord[k]];
-    g.swap(gs);
-  }
+  make_runs(cw, prec, cap);
+  const std::vector<Run>& g = prec ? cw.g32 : cw.g64;
   PQ_HIP(dev_alloc(&d, g.size()));
   if (!g.empty()) PQ_HIP(hipMemcpy(d, g.data(), g.size() * sizeof(int2), hipMemcpyHostToDevice));
   return PQ_OK;
-}
-
-// Rotates the tiles [first, end) of bra number i by a bra-dependent offset.
-void rotate_tiles(std::vector<Tile>& t, std::vector<float>& w, size_t first, size_t i) {
-  const size_t cnt = t.size() - first;
-  if (cnt < 2) return;
-  const size_t r = (i * 2654435761u) % cnt;
-  std::rotate(t.begin() + first, t.begin() + first + r, t.end());
-  std::rotate(w.begin() + first, w.begin() + first + r, w.end());
 }
 
 // Work lists over the pair copy P (its pp_n) and its per-class order S.
@@ -564,121 +518,11 @@ int build_work(pq_engine* e, std::vector<ClassWork>& out, double cut, double fp3
     dev_free(c.d_g32);
   }
   out.clear();
+  const WorkInput in{P, e->pairs, S, e->Q, e->sh_mol, e->nmol};
   for (int X = 0; X < 6; ++X)
     for (int Y = 0; Y <= X; ++Y) {
       ClassWork cw;
-      cw.X = X;
-      cw.Y = Y;
-      const auto& bras = S[X];
-      const auto& kets = S[Y];
-      if (bras.empty() || kets.empty()) {
-        out.push_back(cw);
-        continue;
-      }
-      const int la = kPairL[X][0], lb = kPairL[X][1], lc = kPairL[Y][0], ld = kPairL[Y][1];
-      const int tile = 64;
-      std::vector<double> kq(kets.size());
-      std::vector<double> kpre(kets.size() + 1, 0.0), fpre(kets.size() + 1, 0.0);
-      auto ufun = [&](int pid, int l1, int l2) -> double {  // unique function pairs of a shell pair
-        const PairInfo& p = P[pid];
-        return p.shA == p.shB ? ncart(l1) * (ncart(l1) + 1) / 2.0 : (double)ncart(l1) * ncart(l2);
-      };
-      for (size_t j = 0; j < kets.size(); ++j) {
-        kq[j] = e->Q[kets[j]];
-        kpre[j + 1] = kpre[j] + P[kets[j]].pp_n;
-        fpre[j + 1] = fpre[j] + ufun(kets[j], lc, ld);
-      }
-      // ket groups: runs of one molecule and equal (unscreened) contraction
-      // length, each sorted by Q descending; gmol[m] = first group of
-      // molecule m
-      auto kmol = [&](int j) { return e->sh_mol[P[kets[j]].shA]; };
-      std::vector<int> gs;
-      for (size_t j = 0; j < kets.size(); ++j)
-        if (j == 0 || e->pairs[kets[j]].pp_n != e->pairs[kets[j - 1]].pp_n || kmol(j) != kmol(j - 1))
-          gs.push_back((int)j);
-      gs.push_back((int)kets.size());
-      std::vector<int> gmol(e->nmol + 1, (int)gs.size() - 1);
-      for (int g = (int)gs.size() - 2; g >= 0; --g) gmol[kmol(gs[g])] = g;
-      for (int m = e->nmol - 1; m >= 0; --m)
-        if (gmol[m] > gmol[m + 1]) gmol[m] = gmol[m + 1];
-      auto count_ge = [&](int g0, int g1, double thr) -> int {  // entries of [g0,g1) with Q >= thr
-        return (int)(std::upper_bound(kq.begin() + g0, kq.begin() + g1, thr,
-                                      [](double t, double q) { return t > q; }) -
-                     (kq.begin() + g0));
-      };
-      for (size_t i = 0; i < bras.size(); ++i) {
-        const double qi = e->Q[bras[i]];
-        const int bpn = P[bras[i]].pp_n;
-        const double ua = ufun(bras[i], la, lb);
-        const int bm = e->sh_mol[P[bras[i]].shA];
-        const size_t t64_0 = cw.t64.size(), t32_0 = cw.t32.size();
-        for (int g = gmol[bm]; g < gmol[bm + 1]; ++g) {
-          const int g0 = gs[g], g1 = gs[g + 1];
-          if (X == Y && g0 > (int)i) break;
-          int c;
-          if (cut <= 0.0) c = g1 - g0;
-          else if (qi <= 0.0) c = 0;
-          else c = count_ge(g0, g1, cut / qi);
-          int end = g0 + c;
-          if (X == Y) end = std::min(end, (int)i + 1);
-          int e64 = end;
-          if (fp32cut > 0.0) e64 = qi <= 0.0 ? g0 : std::min(end, g0 + count_ge(g0, g1, fp32cut / qi));
-          // split the bra primitive sum when the quartets are heavily contracted
-          const int kpn = P[kets[g0]].pp_n;
-          int chunk = bpn;
-          if (split_prims) chunk = std::max(1, std::min(bpn, kPrimBudget / std::max(1, kpn)));
-          // a tile with fewer than 64 kets spreads each ket's primitive
-          // pairs over ks lanes (ks a power of two, ks * kets <= 64), whose
-          // blocks are summed by shuffles before digestion; (ss|ss) has too
-          // little work per lane for that to pay (measured)
-          const bool ksplit = split_prims && la + lb + lc + ld > 0;
-          auto nket_field = [&](int cnt) {
-            int lg = 0;
-            while (ksplit && lg < 6 && (cnt << (lg + 1)) <= tile && (1 << (lg + 1)) <= kpn) ++lg;
-            return cnt | (lg << 8);
-          };
-          // work of a tile: primitive quartets per lane plus a fixed share
-          // for the tile's setup and digestion
-          int np_w = 0;
-          auto tile_work = [&](int nf) {
-            const int ks = 1 << ((nf >> 8) & 7);
-            return (float)(np_w * ((kpn + ks - 1) / ks) + kTileFixedWork);
-          };
-          for (int p0 = 0; p0 < bpn; p0 += chunk) {
-            const int np = std::min(chunk, bpn - p0);
-            const int prim = np == bpn ? 0 : (p0 | (np << 16));
-            np_w = np;
-            for (int k = g0; k < e64; k += tile) {
-              const int nf = nket_field(std::min(tile, e64 - k));
-              cw.t64.push_back(Tile{bras[i], k, nf, prim});
-              cw.w64.push_back(tile_work(nf));
-            }
-            for (int k = e64; k < end; k += tile) {
-              const int nf = nket_field(std::min(tile, end - k));
-              cw.t32.push_back(Tile{bras[i], k, nf, prim});
-              cw.w32.push_back(tile_work(nf));
-            }
-          }
-          cw.nq64 += e64 - g0;
-          cw.nq32 += end - e64;
-          cw.np64 += bpn * (kpre[e64] - kpre[g0]);
-          cw.np32 += bpn * (kpre[end] - kpre[e64]);
-          // unique function quartets; the diagonal quartet (bra == ket) has
-          // ua(ua+1)/2 instead of ua^2
-          double f64 = ua * (fpre[e64] - fpre[g0]), f32 = ua * (fpre[end] - fpre[e64]);
-          if (X == Y && end == (int)i + 1 && (int)i >= g0) {
-            const double corr = ua * ua - ua * (ua + 1) / 2.0;
-            if ((int)i < e64) f64 -= corr; else f32 -= corr;
-          }
-          cw.nf64 += f64;
-          cw.nf32 += f32;
-        }
-        // start each bra's tiles at a different ket tile: blocks that run
-        // at the same time then add into different J[c,d] entries instead of
-        // all contending for the atomics of the first kets
-        rotate_tiles(cw.t64, cw.w64, t64_0, i);
-        rotate_tiles(cw.t32, cw.w32, t32_0, i);
-      }
+      build_class_tiles(in, X, Y, cut, fp32cut, split_prims, cw);
       for (int prec = 0; prec < 2 && split_prims; ++prec) {
         const int tuned = e->group_cap[X][Y][prec];
         const int rc = set_groups(cw, prec, tuned >= 0 ? tuned : default_group_cap(e->maxn));
