@@ -241,12 +241,23 @@ __device__ __forceinline__ void acc_flush(const double* lds, int64_t i, int64_t 
   }
 }
 
-template <int LA, int LB, int LC, int LD, int A0, int NAS, int B0, int NBS, typename T, int MODE>
+// Bra runs (k_eri_group) of classes whose bra block has at most 3 elements
+// per density keep each lane's J_ab partial sums in registers over the whole
+// run, with one wave reduction per run instead of one per tile.
+template <int LA, int LB, int MODE>
+constexpr bool kRunJab = ncart(LA) * ncart(LB) * mode_ndens(MODE) <= 3;
+
+// RUN: called from a bra run of k_eri_group, where krow and jrow are always
+// the block's LDS arrays and jsum is null (compile-time, so that no global
+// fallback path is generated); jab_acc (RUN only, kRunJab classes): the
+// lane's J_ab accumulators of the run, added to instead of reduced per tile.
+template <int LA, int LB, int LC, int LD, int A0, int NAS, int B0, int NBS, typename T, int MODE, bool RUN = false>
 __device__ __forceinline__ void digest_lane(const EngineView& ev, const PairInfo& bp, const PairInfo& kp,
                                             const T (&out)[NAS * NBS * ncart(LC) * ncart(LD)], double f,
                                             bool active, const double* __restrict__ D, double* Jacc,
                                             double* Kacc, double* krow, double* jsum,
-                                            const double* drow, double* jrow, double* jpair, int jn, int kpos) {
+                                            const double* drow, double* jrow, double* jpair, int jn, int kpos,
+                                            double* jab_acc = nullptr) {
   using C = Cls<LA, LB, LC, LD>;
   constexpr int NC = C::NC, ND = C::ND, NO = NAS * NBS * NC * ND;
   // the quartet's molecule block (bra and ket share it)
@@ -306,14 +317,18 @@ __device__ __forceinline__ void digest_lane(const EngineView& ev, const PairInfo
             jab += v * Dcd[c][d];
             jcd[c][d] += v * Dab[a][b];
           }
-        const double s = wave_sum(jab);
-        if (lane == 0) {
-          if (jrow) acc_add<MODE, false>(&jrow[(A0 + a) * C::NB + B0 + b], C::NA * C::NB, 2.0 * s);
-          else acc_add<MODE, true>(&Jacc[(a0 + a) * n + b0 + b], ev.msize, 2.0 * s);
+        if constexpr (RUN && kRunJab<LA, LB, MODE>) {
+          jab_acc[a * NBS + b] += jab;
+        } else {
+          const double s = wave_sum(jab);
+          if (lane == 0) {
+            if (RUN || jrow) acc_add<MODE, false>(&jrow[(A0 + a) * C::NB + B0 + b], C::NA * C::NB, 2.0 * s);
+            else acc_add<MODE, true>(&Jacc[(a0 + a) * n + b0 + b], ev.msize, 2.0 * s);
+          }
         }
       }
     if (!active) return;
-    if (jsum) {
+    if (!RUN && jsum) {
 #pragma unroll
       for (int c = 0; c < NC; ++c)
 #pragma unroll
@@ -334,7 +349,7 @@ __device__ __forceinline__ void digest_lane(const EngineView& ev, const PairInfo
   // (LDS rows: NA + NB rows of the largest molecule's nbf columns per limb)
   const int64_t kstride = (int64_t)(C::NA + C::NB) * ev.nbf;
   auto flush_row = [&](int r, int grow, const double (&kc)[NC], const double (&kd)[ND]) {
-    if (krow) {
+    if (RUN || krow) {
 #pragma unroll
       for (int c = 0; c < NC; ++c) acc_add<MODE, false>(&krow[r * n + c0 + c], kstride, kc[c]);
 #pragma unroll
@@ -437,12 +452,14 @@ __device__ __forceinline__ void atomic_max_nonneg(double* p, double v) {
 }
 
 // One lane, one shell quartet, bra components [A0, A0+NAS) x [B0, B0+NBS).
-template <int LA, int LB, int LC, int LD, typename T, int MODE, int A0, int NAS, int B0, int NBS, bool LT = false>
+template <int LA, int LB, int LC, int LD, typename T, int MODE, int A0, int NAS, int B0, int NBS, bool LT = false,
+          bool RUN = false>
 __device__ __forceinline__ void lane_body(const EngineView& ev, const Tile& tl, bool active, int bra, int ket,
                                           const double* __restrict__ D, double* Jacc, double* Kacc,
                                           double* ints, double* qout, double* krow, double* jsum,
                                           const void* rtab = nullptr, int kc = 0, int ks = 1, int nk = 64,
-                                          const double* drow = nullptr, double* jrow = nullptr, int kpos = 0) {
+                                          const double* drow = nullptr, double* jrow = nullptr, int kpos = 0,
+                                          double* jab_acc = nullptr) {
   using C = Cls<LA, LB, LC, LD>;
   constexpr bool UB = MODE != MODE_DIAG;  // Schwarz diagonal: bra = ket per lane
   const PairInfo bp = UB ? ld_uniform(ev.pairs + bra) : ev.pairs[bra];
@@ -484,11 +501,12 @@ __device__ __forceinline__ void lane_body(const EngineView& ev, const Tile& tl, 
     constexpr int Y = pair_class_of(LC, LD);
 #pragma unroll
     for (int d = 0; d < mode_ndens(MODE); ++d)
-      digest_lane<LA, LB, LC, LD, A0, NAS, B0, NBS, T, MODE>(
+      digest_lane<LA, LB, LC, LD, A0, NAS, B0, NBS, T, MODE, RUN>(
           ev, bp, kp, out, f, active, D + d * nn, Jacc + d * nn, Kacc + d * nn,
           krow ? krow + d * (C::NA + C::NB) * ev.nbf : nullptr, jsum ? jsum + d * 64 * C::NC * C::ND : nullptr,
           drow ? drow + d * (C::NA + C::NB) * ev.nbf : nullptr, jrow ? jrow + d * C::NA * C::NB : nullptr,
-          ev.jpair + d * ev.jp_size + ev.jp_off[Y], ev.jp_n[Y], kpos);
+          ev.jpair + d * ev.jp_size + ev.jp_off[Y], ev.jp_n[Y], kpos,
+          jab_acc ? jab_acc + d * C::NA * C::NB : nullptr);
   } else if constexpr (MODE == MODE_STORE) {
     if (!active) return;
     if (ev.blk) {
@@ -663,6 +681,11 @@ __device__ __forceinline__ void group_body(const EngineView& ev, const Tile* __r
       }
   }
   __syncthreads();
+  // the lanes' J_ab partial sums over the run (kRunJab classes)
+  constexpr bool RJ = kRunJab<LA, LB, MODE>;
+  double jab_run[RJ ? ND * C::NA * C::NB : 1];
+#pragma unroll
+  for (int i = 0; i < (RJ ? ND * C::NA * C::NB : 1); ++i) jab_run[i] = 0.0;
   for (int tt = wv; tt < g.y; tt += 4) {
     Tile tl = ld_uniform(tiles + g.x + tt);
     tl.bra = __builtin_amdgcn_readfirstlane(tl.bra);
@@ -671,10 +694,16 @@ __device__ __forceinline__ void group_body(const EngineView& ev, const Tile* __r
     tl.prim = __builtin_amdgcn_readfirstlane(tl.prim);
     const TileLane tn = tile_lane(tl.nket, lane);
     const int ket = kets[tl.ket0 + tn.slot];
-    lane_body<LA, LB, LC, LD, T, MODE, 0, C::NA, 0, C::NB, (NT > 0)>(ev, tl, tn.active, tl.bra, ket, D, Jacc,
-                                                                       Kacc, nullptr, nullptr, krow, nullptr, rtab,
-                                                                       tn.kc, tn.ks, tn.nk, drow, jrow,
-                                                                       tl.ket0 + tn.slot);
+    lane_body<LA, LB, LC, LD, T, MODE, 0, C::NA, 0, C::NB, (NT > 0), true>(
+        ev, tl, tn.active, tl.bra, ket, D, Jacc, Kacc, nullptr, nullptr, krow, nullptr, rtab, tn.kc, tn.ks, tn.nk,
+        drow, jrow, tl.ket0 + tn.slot, RJ ? jab_run : nullptr);
+  }
+  if constexpr (RJ) {
+#pragma unroll
+    for (int i = 0; i < ND * C::NA * C::NB; ++i) {
+      const double s = wave_sum(jab_run[i]);
+      if (lane == 0) acc_add<MODE, false>(&jrow[i], C::NA * C::NB, 2.0 * s);
+    }
   }
   __syncthreads();
   for (int i = threadIdx.x; i < ND * C::NA * C::NB; i += blockDim.x) {
