@@ -174,6 +174,8 @@ struct pq_engine {
   int* d_mol_moff = nullptr;
   int* d_mol_n = nullptr;
   int* d_atom_mol = nullptr;
+  int* d_mol_at0 = nullptr;  // first atom and atom count of each molecule
+  int* d_mol_nat = nullptr;
 
   int maxat = 0;      // largest molecule's atom count
   int lds_dev = 65536;  // the device's LDS per workgroup, bytes
@@ -263,6 +265,8 @@ struct pq_engine {
     dev_free(d_mol_moff);
     dev_free(d_mol_n);
     dev_free(d_atom_mol);
+    dev_free(d_mol_at0);
+    dev_free(d_mol_nat);
   }
 };
 
@@ -824,13 +828,16 @@ int pq_engine_create_batch(int nshell, const int* shell_l, const int* shell_npri
       dev_alloc(&e->d_Jacc, 2 * (size_t)e->msize) != hipSuccess ||
       dev_alloc(&e->d_Kacc, 2 * (size_t)e->msize) != hipSuccess ||
       dev_alloc(&e->d_mol_moff, e->nmol) != hipSuccess || dev_alloc(&e->d_mol_n, e->nmol) != hipSuccess ||
-      dev_alloc(&e->d_atom_mol, natom) != hipSuccess) {
+      dev_alloc(&e->d_atom_mol, natom) != hipSuccess || dev_alloc(&e->d_mol_at0, e->nmol) != hipSuccess ||
+      dev_alloc(&e->d_mol_nat, e->nmol) != hipSuccess) {
     delete e;
     return fail(PQ_E_ALLOC, "device allocation failed");
   }
   if (hipMemcpy(e->d_mol_moff, moff32.data(), e->nmol * sizeof(int), hipMemcpyHostToDevice) != hipSuccess ||
       hipMemcpy(e->d_mol_n, e->mol_nbf.data(), e->nmol * sizeof(int), hipMemcpyHostToDevice) != hipSuccess ||
-      hipMemcpy(e->d_atom_mol, e->atom_mol.data(), natom * sizeof(int), hipMemcpyHostToDevice) != hipSuccess) {
+      hipMemcpy(e->d_atom_mol, e->atom_mol.data(), natom * sizeof(int), hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(e->d_mol_at0, e->mol_at0.data(), e->nmol * sizeof(int), hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(e->d_mol_nat, e->mol_nat.data(), e->nmol * sizeof(int), hipMemcpyHostToDevice) != hipSuccess) {
     delete e;
     return fail(PQ_E_HIP, "copy of the molecule layout failed");
   }
@@ -1704,7 +1711,8 @@ static int forces_impl(pq_engine* eng, const double* D_dev, const double* W_dev,
     if (rc != PQ_OK) break;
     PQ_TRY(join_side(eng, st));
     PQ_TRY(launch_grad_finish(eng->d_gacc, ncopy, eng->natom, eng->d_atom_xyz, eng->d_atom_z,
-                              eng->nmol > 1 ? eng->d_atom_mol : nullptr, grad_dev, st));
+                              eng->nmol > 1 ? eng->d_atom_mol : nullptr, eng->d_mol_at0, eng->d_mol_nat,
+                              grad_dev, st));
     if (ms) {
       PQ_TRY(mark(st));
       PQ_TRY(hipStreamSynchronize(st));

@@ -45,35 +45,39 @@ __device__ __forceinline__ void grad_lane(const EngineView& ev, const Tile& tl, 
   const int n = bp.mstride;
   D += bp.moff;
   const int a0 = bp.bfA + A0, b0 = bp.bfB + B0, c0 = kp.bfA, d0 = kp.bfB;
-  // density factor per component (normalisation folded in)
+  // density factor per component (normalisation folded in); the
+  // unrestricted form only when a beta density is given (one branch per
+  // quartet, outside the component loops)
   double G[NO];
   {
     const double f = active ? quartet_degeneracy(bp, kp, bra, ket) : 0.0;
-    const double* Db = ev.grad_db;
-    if (Db) Db += bp.moff;
+    auto fill = [&](auto unrestricted) __attribute__((always_inline)) {
+      constexpr bool U = decltype(unrestricted)::value;
+      const double* Db = ev.grad_db + (U ? bp.moff : 0);
 #pragma unroll
-    for (int a = 0; a < NAS; ++a)
+      for (int a = 0; a < NAS; ++a)
 #pragma unroll
-      for (int b = 0; b < NBS; ++b) {
-        const double dab = D[(a0 + a) * n + b0 + b];
+        for (int b = 0; b < NBS; ++b)
 #pragma unroll
-        for (int c = 0; c < C::NC; ++c)
+          for (int c = 0; c < C::NC; ++c)
 #pragma unroll
-          for (int d = 0; d < C::ND; ++d) {
-            const double nf = cart_norm(LA, A0 + a) * cart_norm(LB, B0 + b) * cart_norm(LC, c) * cart_norm(LD, d);
-            const int ab = (a0 + a) * n + b0 + b, cd = (c0 + c) * n + d0 + d;
-            const int ac = (a0 + a) * n + c0 + c, bd = (b0 + b) * n + d0 + d;
-            const int ad = (a0 + a) * n + d0 + d, bc = (b0 + b) * n + c0 + c;
-            double g;
-            if (Db) {
-              g = 4.0 * (dab + Db[ab]) * (D[cd] + Db[cd]) -
-                  2.0 * (D[ac] * D[bd] + D[ad] * D[bc] + Db[ac] * Db[bd] + Db[ad] * Db[bc]);
-            } else {
-              g = 16.0 * dab * D[cd] - 4.0 * (D[ac] * D[bd] + D[ad] * D[bc]);
+            for (int d = 0; d < C::ND; ++d) {
+              const double nf =
+                  cart_norm(LA, A0 + a) * cart_norm(LB, B0 + b) * cart_norm(LC, c) * cart_norm(LD, d);
+              const int ab = (a0 + a) * n + b0 + b, cd = (c0 + c) * n + d0 + d;
+              const int ac = (a0 + a) * n + c0 + c, bd = (b0 + b) * n + d0 + d;
+              const int ad = (a0 + a) * n + d0 + d, bc = (b0 + b) * n + c0 + c;
+              double g;
+              if constexpr (U)
+                g = 4.0 * (D[ab] + Db[ab]) * (D[cd] + Db[cd]) -
+                    2.0 * (D[ac] * D[bd] + D[ad] * D[bc] + Db[ac] * Db[bd] + Db[ad] * Db[bc]);
+              else
+                g = 16.0 * D[ab] * D[cd] - 4.0 * (D[ac] * D[bd] + D[ad] * D[bc]);
+              G[((a * NBS + b) * C::NC + c) * C::ND + d] = f * nf * g;
             }
-            G[((a * NBS + b) * C::NC + c) * C::ND + d] = f * nf * g;
-          }
-      }
+    };
+    if (ev.grad_db) fill(std::true_type{});
+    else fill(std::false_type{});
   }
   double gA[3] = {0, 0, 0}, gB[3] = {0, 0, 0}, gC[3] = {0, 0, 0};
   if (active) {

@@ -99,7 +99,8 @@ hipError_t launch_onee_grad(int la, int lb, const EngineView& ev, const int* pai
 
 // grad = sum of the copies + nuclear-repulsion gradient
 hipError_t launch_grad_finish(const double* gacc, int ncopies, int natom, const double* atom_xyz,
-                              const double* atom_z, const int* atom_mol, double* grad, hipStream_t st);
+                              const double* atom_z, const int* atom_mol, const int* mol_at0, const int* mol_nat,
+                              double* grad, hipStream_t st);
 
 // per-molecule blocks of packed matrices (block m at moff[m], n = mn[m] <= maxn)
 hipError_t launch_symmetrize_packed(int nmol, int maxn, const int* moff, const int* mn, const double* Jacc,

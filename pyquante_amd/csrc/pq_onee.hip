@@ -524,29 +524,45 @@ hipError_t launch_onee_grad(int la, int lb, const EngineView& ev, const int* pai
 // grad[i] = sum over the private copies + d/dR of sum_{A<B} Z_A Z_B / R_AB
 // atom_mol (nullable): molecule of each atom; nuclear repulsion acts within a
 // molecule only
-__global__ void k_grad_finish(const double* __restrict__ gacc, int ncopies, int natom,
-                              const double* __restrict__ xyz, const double* __restrict__ z,
-                              const int* __restrict__ atom_mol, double* grad) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= 3 * natom) return;
+// One wavefront per gradient component: its lanes sum the private copies
+// and the nuclear-repulsion terms of the atom's molecule (atoms [at0, at0 +
+// nat) when the engine holds several molecules, else every atom), then
+// reduce.  dV_nn/dR_a = -Z_a sum_b Z_b (R_a - R_b) / |R_a - R_b|^3
+// (force.py:172-193, der_enuke).
+__global__ __launch_bounds__(256) void k_grad_finish(const double* __restrict__ gacc, int ncopies, int natom,
+                                                     const double* __restrict__ xyz, const double* __restrict__ z,
+                                                     const int* __restrict__ atom_mol, const int* __restrict__ mol_at0,
+                                                     const int* __restrict__ mol_nat, double* grad) {
+  const int i = (int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6);
+  const int lane = threadIdx.x & 63;
+  if (i >= 3 * natom) return;  // uniform per wavefront
   double s = 0.0;
-  for (int c = 0; c < ncopies; ++c) s += gacc[(size_t)c * 3 * natom + i];
+  for (int c = lane; c < ncopies; c += 64) s += gacc[(size_t)c * 3 * natom + i];
   const int a = i / 3, x = i % 3;
-  for (int b = 0; b < natom; ++b) {
-    if (b == a || (atom_mol && atom_mol[b] != atom_mol[a])) continue;
+  int b0 = 0, b1 = natom;
+  if (atom_mol) {
+    const int m = atom_mol[a];
+    b0 = mol_at0[m];
+    b1 = b0 + mol_nat[m];
+  }
+  for (int b = b0 + lane; b < b1; b += 64) {
+    if (b == a) continue;
     const double dx = xyz[3 * a] - xyz[3 * b], dy = xyz[3 * a + 1] - xyz[3 * b + 1], dz = xyz[3 * a + 2] - xyz[3 * b + 2];
     const double r2 = dx * dx + dy * dy + dz * dz;
     const double r = sqrt(r2);
     const double dd = x == 0 ? dx : (x == 1 ? dy : dz);
     s -= z[a] * z[b] * dd / (r2 * r);
   }
-  grad[i] = s;
+  s = wave_sum(s);
+  if (lane == 0) grad[i] = s;
 }
 
 hipError_t launch_grad_finish(const double* gacc, int ncopies, int natom, const double* atom_xyz,
-                              const double* atom_z, const int* atom_mol, double* grad, hipStream_t st) {
-  hipLaunchKernelGGL(k_grad_finish, dim3((3 * natom + 127) / 128), dim3(128), 0, st, gacc, ncopies, natom, atom_xyz,
-                     atom_z, atom_mol, grad);
+                              const double* atom_z, const int* atom_mol, const int* mol_at0, const int* mol_nat,
+                              double* grad, hipStream_t st) {
+  const int waves = 3 * natom;
+  hipLaunchKernelGGL(k_grad_finish, dim3((waves + 3) / 4), dim3(256), 0, st, gacc, ncopies, natom, atom_xyz, atom_z,
+                     atom_mol, mol_at0, mol_nat, grad);
   return hipGetLastError();
 }
 

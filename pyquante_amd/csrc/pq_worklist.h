@@ -8,7 +8,6 @@
 #include <algorithm>
 #include <cstddef>
 #include <cstdint>
-#include <numeric>
 #include <vector>
 
 #include "../../include/pq_abi.h"
@@ -23,9 +22,6 @@ constexpr int kPairL[6][2] = {{0, 0}, {1, 0}, {1, 1}, {2, 0}, {2, 1}, {2, 2}};
 // about kPrimBudget primitive quartets: heavily contracted quartets are
 // spread over several tiles (their partial sums digest independently).
 constexpr int kPrimBudget = 36;
-// A tile's setup and digestion, in primitive quartets per lane (the bra-run
-// order's work estimate).
-constexpr int kTileFixedWork = 8;
 constexpr int kTileKets = 64;
 
 // A bra run: tiles [x, x + y) of one bra pair (the int2 the kernels read).
@@ -34,12 +30,10 @@ struct Run {
 };
 
 // Work list of one class (X|Y), X >= Y pair classes: FP64 and FP32 tiles,
-// each tile's work estimate, the bra runs, and the counts of what the
-// tiles cover.
+// the bra runs, and the counts of what the tiles cover.
 struct ClassTiles {
   int X = 0, Y = 0;
   std::vector<Tile> t64, t32;
-  std::vector<float> w64, w32;  // estimated work of each tile (for the run order)
   std::vector<Run> g64, g32;    // runs of tiles with one bra pair
   int cap[2] = {0, 0};          // tiles per run (0 = no runs) for FP64 / FP32 tiles, | PQ_CAP_STAGE_D
   uint64_t nq64 = 0, nq32 = 0;  // shell quartets
@@ -62,12 +56,11 @@ struct WorkInput {
 };
 
 // Rotates the tiles [first, end) of bra number i by a bra-dependent offset.
-inline void rotate_tiles(std::vector<Tile>& t, std::vector<float>& w, size_t first, size_t i) {
+inline void rotate_tiles(std::vector<Tile>& t, size_t first, size_t i) {
   const size_t cnt = t.size() - first;
   if (cnt < 2) return;
   const size_t r = (i * 2654435761u) % cnt;
   std::rotate(t.begin() + first, t.begin() + first + r, t.end());
-  std::rotate(w.begin() + first, w.begin() + first + r, w.end());
 }
 
 // The tiles of class (X|Y): every (bra, ket) with Q_bra Q_ket >= cut (all
@@ -147,26 +140,14 @@ inline void build_class_tiles(const WorkInput& in, int X, int Y, double cut, dou
         while (ksplit && lg < 6 && (cnt << (lg + 1)) <= tile && (1 << (lg + 1)) <= kpn) ++lg;
         return cnt | (lg << 8);
       };
-      // work of a tile: primitive quartets per lane plus a fixed share for
-      // the tile's setup and digestion
-      int np_w = 0;
-      auto tile_work = [&](int nf) {
-        const int ks = 1 << ((nf >> 8) & 7);
-        return (float)(np_w * ((kpn + ks - 1) / ks) + kTileFixedWork);
-      };
       for (int p0 = 0; p0 < bpn; p0 += chunk) {
         const int np = std::min(chunk, bpn - p0);
         const int prim = np == bpn ? 0 : (p0 | (np << 16));
-        np_w = np;
         for (int k = g0; k < e64; k += tile) {
-          const int nf = nket_field(std::min(tile, e64 - k));
-          cw.t64.push_back(Tile{bras[i], k, nf, prim});
-          cw.w64.push_back(tile_work(nf));
+          cw.t64.push_back(Tile{bras[i], k, nket_field(std::min(tile, e64 - k)), prim});
         }
         for (int k = e64; k < end; k += tile) {
-          const int nf = nket_field(std::min(tile, end - k));
-          cw.t32.push_back(Tile{bras[i], k, nf, prim});
-          cw.w32.push_back(tile_work(nf));
+          cw.t32.push_back(Tile{bras[i], k, nket_field(std::min(tile, end - k)), prim});
         }
       }
       cw.nq64 += e64 - g0;
@@ -187,18 +168,18 @@ inline void build_class_tiles(const WorkInput& in, int X, int Y, double cut, dou
     // start each bra's tiles at a different ket tile: blocks that run at the
     // same time then add into different J[c,d] entries instead of all
     // contending for the atomics of the first kets
-    rotate_tiles(cw.t64, cw.w64, t64_0, i);
-    rotate_tiles(cw.t32, cw.w32, t32_0, i);
+    rotate_tiles(cw.t64, t64_0, i);
+    rotate_tiles(cw.t32, t32_0, i);
   }
 }
 
 // The bra runs of one precision of a class for a run cap (0 = none; the
-// PQ_CAP_STAGE_D flag is ignored here), heaviest run first: blocks are
-// dispatched in index order as compute units free up, so the longest runs
-// start early and a launch ends on short ones instead of a few stragglers.
+// PQ_CAP_STAGE_D flag is ignored here), in tile order.  (Ordering the runs
+// heaviest first, by an estimate of their work, gained nothing on one
+// molecule and cost 15% on the 512-molecule batch, whose concurrent blocks
+// then touch many molecules' matrices at once.)
 inline void make_runs(ClassTiles& cw, int prec, int cap) {
   const std::vector<Tile>& t = prec ? cw.t32 : cw.t64;
-  const std::vector<float>& w = prec ? cw.w32 : cw.w64;
   std::vector<Run>& g = prec ? cw.g32 : cw.g64;
   g.clear();
   cw.cap[prec] = cap;
@@ -208,16 +189,6 @@ inline void make_runs(ClassTiles& cw, int prec, int cap) {
     if (g.empty() || t[i].bra != t[g.back().x].bra || g.back().y >= run) g.push_back(Run{i, 0});
     g.back().y += 1;
   }
-  if (w.size() != t.size()) return;
-  std::vector<float> gw(g.size(), 0.f);
-  for (size_t k = 0; k < g.size(); ++k)
-    for (int i = g[k].x; i < g[k].x + g[k].y; ++i) gw[k] += w[i];
-  std::vector<int> ord(g.size());
-  std::iota(ord.begin(), ord.end(), 0);
-  std::stable_sort(ord.begin(), ord.end(), [&](int a, int b) { return gw[a] > gw[b]; });
-  std::vector<Run> gsorted(g.size());
-  for (size_t k = 0; k < g.size(); ++k) gsorted[k] = g[ord[k]];
-  g.swap(gsorted);
 }
 
 }  // namespace pq

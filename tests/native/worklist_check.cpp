@@ -96,8 +96,6 @@ int main(int argc, char** argv) {
       double np[2] = {0, 0};
       for (int prec = 0; prec < 2; ++prec) {
         const auto& t = prec ? cw.t32 : cw.t64;
-        const auto& w = prec ? cw.w32 : cw.w64;
-        check(w.size() == t.size(), "one work estimate per tile", X, Y);
         for (size_t ti = 0; ti < t.size(); ++ti) {
           const Tile& tl = t[ti];
           const int nkt = tl.nket & 0xff, lg = (tl.nket >> 8) & 7;
@@ -127,8 +125,8 @@ int main(int argc, char** argv) {
             mask[idx] |= bits;
           }
         }
-        // bra runs: a partition of the tiles into runs of one bra, at most
-        // the cap long, heaviest first
+        // bra runs: a partition of the tiles, in order, into runs of one
+        // bra at most the cap long
         const int run = caps[prec] > 0 ? caps[prec] & (PQ_CAP_STAGE_D - 1) : 0;
         const auto& g = prec ? cw.g32 : cw.g64;
         if (run <= 0) {
@@ -136,19 +134,17 @@ int main(int argc, char** argv) {
           continue;
         }
         std::vector<int> seen(t.size(), 0);
-        double last = 1e300;
+        int next = 0;
         for (size_t r = 0; r < g.size(); ++r) {
           if (!check(g[r].y >= 1 && g[r].y <= run && g[r].x >= 0 && g[r].x + g[r].y <= (int)t.size(),
                      "run inside the tile list and at most the cap long", X, Y, (long)r))
             break;
-          double wsum = 0;
+          check(g[r].x == next, "runs in tile order", X, Y, (long)r, g[r].x);
+          next = g[r].x + g[r].y;
           for (int i = g[r].x; i < g[r].x + g[r].y; ++i) {
             ++seen[i];
-            wsum += w[i];
             check(t[i].bra == t[g[r].x].bra, "one bra per run", X, Y, (long)r, i);
           }
-          check(wsum <= last * (1 + 1e-6), "runs in order of decreasing work", X, Y, (long)r);
-          last = wsum;
         }
         for (size_t i = 0; i < t.size(); ++i) check(seen[i] == 1, "every tile in one run", X, Y, (long)i);
       }

@@ -6,7 +6,7 @@ Schwarz bounds and primitive screening of realistic shape, every class's
 tiles must cover each kept (bra, ket) shell quartet exactly once, whole
 (primitive ranges partitioning the bra's contraction), in the bucket its
 bound selects, with the screened quartets absent, and the bra runs must
-partition the tiles by bra, heaviest first; any out-of-bounds access or
+partition the tiles, in order, by bra; any out-of-bounds access or
 undefined behaviour fails the run."""
 import math
 import os
