@@ -517,6 +517,9 @@ def main():
     ap.add_argument("--batch-molecules", type=int, default=512,
                     help="molecules of the batched workload (0 skips it)")
     ap.add_argument("--streams", type=int, default=4, help="engine streams for the class kernels (0-4)")
+    ap.add_argument("--no-isolated", action="store_true",
+                    help="skip timing the class kernels one at a time (keeps a kernel-trace profile's averages to "
+                         "the concurrent launches)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run one 256 MiB device copy (reference for rocprofv3 FETCH/WRITE_SIZE)")
     ap.add_argument("--classes", action="store_true", help="print per-class kernel times")
@@ -605,6 +608,17 @@ def main():
                      cc["prim_quartets_fp64"], cc["tiles_fp64"], t * 1e3,
                      cc["flops_fp64"] / t / 1e12 if t > 0 else 0,
                      cc["fquartets_fp64"] / t / 1e9 if t > 0 else 0), file=sys.stderr)
+    # the same kernels one at a time on the caller's stream: each launch with
+    # the chip to itself, beside its duration inside the concurrent build
+    iso = None
+    if not args.no_isolated:
+        eng.set_streams(0)
+        iso = np.zeros(43)
+        for _ in range(5):
+            _, _, ms = eng.fock_jk_timed(D)
+            iso += np.array(ms)
+        iso /= 5
+        eng.set_streams(args.streams)
     k = int(np.argmax(acc[:42:2]))
     kname = kernel_name(cls[k]["class"], caps[k][0] if caps else None, eng.nbf)
     traffic = recorded_traffic("%s h2o8_631ss_fp64" % kname)
@@ -620,6 +634,10 @@ def main():
           "%.2f TFLOP/s; all kernels %.2f TFLOP/s"
           % (dt * 1e3, "eager" if args.eager else "graph replay", dt_eager * 1e3, graph_err, lname, ktime * 1e3,
              achieved, total_flops / (acc[42] * 1e-3) / 1e12), file=sys.stderr, flush=True)
+    if iso is not None:
+        print("bench: one class at a time: dominant kernel %.3f ms alone (%.3f of FP64 peak), all classes %.3f ms"
+              % (iso[2 * k], c["flops_fp64"] / (iso[2 * k] * 1e-3) / 1e12 / FP64_PEAK_TFLOPS, iso[42]),
+              file=sys.stderr, flush=True)
     if args.copy_calibration:
         src = torch.ones(32 * 1024 * 1024, dtype=torch.float64, device="cuda")
         dst = torch.empty_like(src)
@@ -666,7 +684,13 @@ def main():
                      "frac": achieved / FP64_PEAK_TFLOPS, "traffic": traffic,
                      "kernel": "%s, %s FP64 J/K" % (kname, lname), "launch_ms": ktime * 1e3,
                      "all_kernels_tflops": total_flops / (acc[42] * 1e-3) / 1e12,
-                     "counted_fp64_tflops": hw_flops / ktime / 1e12 if hw_flops else None},
+                     "counted_fp64_tflops": hw_flops / ktime / 1e12 if hw_flops else None,
+                     "isolated": None if iso is None else {
+                         "launch_ms": iso[2 * k],
+                         "frac": c["flops_fp64"] / (iso[2 * k] * 1e-3) / 1e12 / FP64_PEAK_TFLOPS,
+                         "serialized_build_ms": iso[42],
+                         "note": "the same kernel launched alone (class kernels one at a time on one stream); "
+                                 "frac above is its duration inside the concurrent build"}},
         "cpu_baseline": cpu,
         "cpu_baseline_all_cores": cpu_all,
     }

@@ -53,6 +53,8 @@ struct HgpCls {
   static constexpr int NE = ncum(LAB), NF = ncum(LCD);
   static constexpr int E0 = ncum(LA - 1), F0 = ncum(LC - 1);  // first target triples
   static constexpr int NEA = NE - E0, NFC = NF - F0;
+  // targets, and their count rounded up to the 8 the rolled variant adds per batch
+  static constexpr int NT = NEA * NFC, NTP = (NT + 7) / 8 * 8;
   static constexpr int NA = ncart(LA), NB = ncart(LB), NC = ncart(LC), ND = ncart(LD);
   static constexpr bool need(int e, int f) { return f == 0 || tri_order(e) >= LA - (LCD - tri_order(f)); }
   static constexpr int nm(int e, int f) {
@@ -100,7 +102,7 @@ struct HgpSteps {
   static constexpr int NAB = H::NA * H::NB, NCD = H::NC * H::ND, MAXT = 4;
   HgpStep bra[NBRA > 0 ? NBRA : 1];
   HgpStep ket[NKET];
-  short tgt[H::NEA * H::NFC];
+  short tgt[H::NTP];  // entries past NT stay 0 (they read V[0] into acc's padding)
   unsigned char nbt[NAB], nkt[NCD];
   HgpTerm bt[NAB][MAXT], kt[NCD][MAXT];
 
@@ -174,21 +176,29 @@ __constant__ HgpSteps<LA, LB, LC, LD> kHgpSteps = HgpSteps<LA, LB, LC, LD>::make
 
 __device__ __forceinline__ double pick3(const double (&v)[3], int d) { return d == 0 ? v[0] : (d == 1 ? v[1] : v[2]); }
 
-// The vertical recurrence of hgp_vrr driven by the step tables.
+// The vertical recurrence of hgp_vrr driven by the step tables.  The table
+// lives in per-lane private memory, so each step first loads the whole order
+// range of its operands (unconditionally: V is padded so that the reads past
+// an entry's last order stay inside the array) and then computes and stores
+// its nm orders: the loads of one step are in flight together instead of one
+// order at a time.  acc (NTP entries) takes the targets in batches of 8 for
+// the same reason.  (Loading the orders two at a time, from entries aligned
+// to even offsets, measured no faster.)
 template <int LA, int LB, int LC, int LD>
 __device__ __forceinline__ void hgp_vrr_rolled(const PairInfo& bp, const PrimPair& b, const PairInfo& kp,
                                                const PrimPair& q, double* acc) {
   using H = HgpCls<LA, LB, LC, LD>;
   using S = HgpSteps<LA, LB, LC, LD>;
   const S& T = kHgpSteps<LA, LB, LC, LD>;
+  constexpr int L = H::L, LCD = H::LCD;
   const PrimQ s = prim_setup(bp, b, kp, q);
   double u[H::NR], w[H::NR];
   rys_roots<H::NR, double>(s.X, u, w);
-  double V[H::SIZE];
+  double V[H::SIZE + L + 1];
 #pragma unroll
   for (int r = 0; r < H::NR; ++r) w[r] *= s.pref;
 #pragma unroll
-  for (int m = 0; m <= H::L; ++m) {
+  for (int m = 0; m <= L; ++m) {
     double f = 0.0;
 #pragma unroll
     for (int r = 0; r < H::NR; ++r) {
@@ -201,31 +211,59 @@ __device__ __forceinline__ void hgp_vrr_rolled(const PairInfo& bp, const PrimPai
   const double WP[3] = {-s.qfac * s.PQ[0], -s.qfac * s.PQ[1], -s.qfac * s.PQ[2]};
   const double WQ[3] = {s.pfac * s.PQ[0], s.pfac * s.PQ[1], s.pfac * s.PQ[2]};
   const double i2pq = 0.5 * s.rpq;
+  // bra steps: nm = L - |e| + 1 <= L orders, operands up to order nm
 #pragma unroll 1
   for (int n = 0; n < S::NBRA; ++n) {
     const HgpStep st = T.bra[n];
     const double pa = pick3(PA, st.dir), wp = pick3(WP, st.dir), c2 = st.c1 * s.i2p;
-#pragma unroll 1
-    for (int m = 0; m < st.nm; ++m) {
-      double v = pa * V[st.o1 + m] + wp * V[st.o1 + m + 1];
-      if (st.c1) v += c2 * (V[st.o2 + m] - s.qfac * V[st.o2 + m + 1]);
+    double x1[L + 1], x2[L + 1];
+#pragma unroll
+    for (int m = 0; m <= L; ++m) x1[m] = V[st.o1 + m];
+    if (st.c1) {
+#pragma unroll
+      for (int m = 0; m <= L; ++m) x2[m] = V[st.o2 + m];
+    }
+#pragma unroll
+    for (int m = 0; m < L; ++m) {
+      if (m >= st.nm) break;
+      double v = pa * x1[m] + wp * x1[m + 1];
+      if (st.c1) v += c2 * (x2[m] - s.qfac * x2[m + 1]);
       V[st.o + m] = v;
     }
   }
+  // ket steps: nm = LCD - |f| + 1 <= LCD orders
 #pragma unroll 1
   for (int n = 0; n < S::NKET; ++n) {
     const HgpStep st = T.ket[n];
     const double qc = pick3(QC, st.dir), wq = pick3(WQ, st.dir), c2 = st.c1 * s.i2q, c3 = st.c2 * i2pq;
-#pragma unroll 1
-    for (int m = 0; m < st.nm; ++m) {
-      double v = qc * V[st.o1 + m] + wq * V[st.o1 + m + 1];
-      if (st.c1) v += c2 * (V[st.o2 + m] - s.pfac * V[st.o2 + m + 1]);
-      if (st.c2) v += c3 * V[st.oe + m + 1];
+    double x1[LCD + 1], x2[LCD + 1], xe[LCD > 0 ? LCD : 1];
+#pragma unroll
+    for (int m = 0; m <= LCD; ++m) x1[m] = V[st.o1 + m];
+    if (st.c1) {
+#pragma unroll
+      for (int m = 0; m <= LCD; ++m) x2[m] = V[st.o2 + m];
+    }
+    if (st.c2) {
+#pragma unroll
+      for (int m = 0; m < LCD; ++m) xe[m] = V[st.oe + m + 1];
+    }
+#pragma unroll
+    for (int m = 0; m < LCD; ++m) {
+      if (m >= st.nm) break;
+      double v = qc * x1[m] + wq * x1[m + 1];
+      if (st.c1) v += c2 * (x2[m] - s.pfac * x2[m + 1]);
+      if (st.c2) v += c3 * xe[m];
       V[st.o + m] = v;
     }
   }
 #pragma unroll 1
-  for (int t = 0; t < H::NEA * H::NFC; ++t) acc[t] += V[T.tgt[t]];
+  for (int t0 = 0; t0 < H::NTP; t0 += 8) {
+    double x[8];
+#pragma unroll
+    for (int j = 0; j < 8; ++j) x[j] = V[T.tgt[t0 + j]];
+#pragma unroll
+    for (int j = 0; j < 8; ++j) acc[t0 + j] += x[j];
+  }
 }
 
 // One primitive quartet: the vertical recurrence, targets added into acc

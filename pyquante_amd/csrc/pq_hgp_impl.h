@@ -62,9 +62,10 @@ __device__ __forceinline__ void hgp_tile(const EngineView& ev, const Tile& tl, c
   // a lane holding part of the primitive sum adds its block into the zeroed
   // array; a whole quartet is stored
   const bool part = tl.prim != 0 || tn.ks > 1;
-  double acc[H::NEA * H::NFC];
+  constexpr int NACC = H::SIZE > kHgpUnrollMax ? H::NTP : H::NT;
+  double acc[NACC];
 #pragma unroll
-  for (int c = 0; c < H::NEA * H::NFC; ++c) acc[c] = 0.0;
+  for (int c = 0; c < NACC; ++c) acc[c] = 0.0;
 #pragma unroll 1
   for (int k = k0; k < k1; ++k) {
     const PrimPair q = ev.prims[kp.pp_off + k];
@@ -101,13 +102,16 @@ __device__ __forceinline__ void hgp_tile(const EngineView& ev, const Tile& tl, c
         const int c = cd / H::ND, d = cd % H::ND;
         const int fi = bp.bfA + a, fj = bp.bfB + bb, fk = kp.bfA + c, fl = kp.bfB + d;
         if (!store_canonical(bp, kp, same, fi, fj, fk, fl)) continue;
+        // every term slot (unused ones have cb = 0 and read acc[idx * NFC]),
+        // so that the acc loads of one output are in flight together
+        constexpr int MTB = LB == 0 ? 1 : (LB == 1 ? 2 : S::MAXT), MTK = LD == 0 ? 1 : (LD == 1 ? 2 : S::MAXT);
         double v = 0.0;
-#pragma unroll 1
-        for (int x = 0; x < T.nbt[ab]; ++x) {
+#pragma unroll
+        for (int x = 0; x < MTB; ++x) {
           const HgpTerm tb = T.bt[ab][x];
           double r = 0.0;
-#pragma unroll 1
-          for (int y = 0; y < T.nkt[cd]; ++y) {
+#pragma unroll
+          for (int y = 0; y < MTK; ++y) {
             const HgpTerm tk = T.kt[cd][y];
             r += tk.cb * CDp[0][tk.p0] * CDp[1][tk.p1] * CDp[2][tk.p2] * acc[tb.idx * H::NFC + tk.idx];
           }
