@@ -1,10 +1,12 @@
-"""RHF nuclear gradients and forces on the GPU engine.
+"""Hartree-Fock nuclear gradients and forces on the GPU engine.
 
-Mirrors PyQuante/force.py:31-70 (hf_force / rhf_force: forces -dE/dR_A
-stored on each atom), with every term -- 2 tr(D dH), the two-electron
-derivative contracted with D on the fly, -2 tr(Q dS) and dV_nn -- computed
-by pq_forces.  D = C_occ C_occ^T and Q (W) = C_occ diag(e_occ) C_occ^T
-(Wavefunction.py:143-160).
+Mirrors PyQuante/force.py:31-108 (hf_force / rhf_force / uhf_force /
+fixedocc_uhf_force: forces -dE/dR_A stored on each atom).  For RHF every
+term -- 2 tr(D dH), the two-electron derivative contracted with D on the
+fly, -2 tr(Q dS) and dV_nn -- is computed by pq_forces, with
+D = C_occ C_occ^T and Q (W) = C_occ diag(e_occ) C_occ^T
+(Wavefunction.py:143-160); the unrestricted and fixed-occupation forces run
+the same kernels on the spin densities (pq_forces_uhf).
 
 RHFEnergy is a torch.autograd.Function: forward runs the SCF at the given
 coordinates and returns E, keeping the engine and the converged D and W;
@@ -68,6 +70,42 @@ class UHFWavefunction(object):
         return (ca * self.orbe[0][:self.nalpha]) @ ca.T, (cb * self.orbe[1][:self.nbeta]) @ cb.T
 
 
+def _vec(x, like):
+    """x as a vector of like's kind (numpy array or device tensor)."""
+    if isinstance(like, torch.Tensor):
+        return torch.as_tensor(np.asarray(x, dtype=np.float64), device=like.device)
+    return np.asarray(x, dtype=np.float64)
+
+
+class FixedOccWavefunction(object):
+    """Unrestricted wavefunction with fixed, possibly fractional, occupations
+    (Wavefunction.py:161-199, the fixedocc branch): orbitals orbs_a, orbs_b
+    (columns), energies orbe_a, orbe_b, occupation numbers occs_a, occs_b of
+    the first len(occs) orbitals of each spin.  mk_auger_Qmatrix weights the
+    beta orbitals with the alpha energies orbe_a, as the reference does
+    (Wavefunction.py:197), so that forces match its fixedocc_uhf_force."""
+
+    restricted = False
+    unrestricted = False
+    fixedocc = True
+
+    def __init__(self, orbs_a, orbs_b, orbe_a, orbe_b, occs_a, occs_b):
+        self.orbs_a, self.orbs_b = orbs_a, orbs_b
+        self.orbe_a, self.orbe_b = orbe_a, orbe_b
+        self.occs_a, self.occs_b = occs_a, occs_b
+
+    def mk_auger_dens(self):
+        fa, fb = _vec(self.occs_a, self.orbs_a), _vec(self.occs_b, self.orbs_b)
+        ca, cb = self.orbs_a[:, :len(fa)], self.orbs_b[:, :len(fb)]
+        return (ca * fa) @ ca.T, (cb * fb) @ cb.T
+
+    def mk_auger_Qmatrix(self):
+        fa, fb = _vec(self.occs_a, self.orbs_a), _vec(self.occs_b, self.orbs_b)
+        ca, cb = self.orbs_a[:, :len(fa)], self.orbs_b[:, :len(fb)]
+        ea = self.orbe_a
+        return (ca * (ea[:len(fa)] * fa)) @ ca.T, (cb * (ea[:len(fb)] * fb)) @ cb.T
+
+
 def scf(engine, mol, ConvCriteria=1e-10, MaxIter=100, DoAveraging=True, precision=0, graph=False):
     """RHF on an engine (hartree_fock.rhf semantics, DIIS on by default here
     for tight convergence).  Returns (E, orbe, orbs, D, W) as device tensors.
@@ -119,9 +157,24 @@ def rhf_force(mol, wf, bset, engine=None):
     eng = engine if engine is not None else Engine(bset, mol)
     if eng.cut is None:
         eng.build()
-    D = torch.as_tensor(np.asarray(wf.mkdens(), dtype=np.float64), device=eng.device)
-    W = torch.as_tensor(np.asarray(wf.mkQmatrix(), dtype=np.float64), device=eng.device)
+    D = torch.as_tensor(wf.mkdens(), dtype=torch.float64, device=eng.device)
+    W = torch.as_tensor(wf.mkQmatrix(), dtype=torch.float64, device=eng.device)
     g = eng.gradient(D, W).cpu().numpy()
+    for atom, gi in zip(mol.atoms, g):
+        atom.forces = -gi
+    return -g
+
+
+def _spin_force(mol, dens, qmat, bset, engine):
+    """-dE/dR from spin densities (Da, Db) and energy-weighted densities
+    (Qa, Qb) by pq_forces_uhf; stored as atom.forces."""
+    eng = engine if engine is not None else Engine(bset, mol)
+    if eng.cut is None:
+        eng.build()
+    dev = eng.device
+    Da, Db = (torch.as_tensor(x, dtype=torch.float64, device=dev) for x in dens)
+    Qa, Qb = (torch.as_tensor(x, dtype=torch.float64, device=dev) for x in qmat)
+    g = eng.gradient_uhf(Da, Db, Qa, Qb).cpu().numpy()
     for atom, gi in zip(mol.atoms, g):
         atom.forces = -gi
     return -g
@@ -131,16 +184,13 @@ def uhf_force(mol, wf, bset, engine=None):
     """force.py:72-89: sets atom.forces = -dE/dR_A for an unrestricted
     wavefunction (spin densities and energy-weighted densities of
     wf.mkdens / wf.mkQmatrix)."""
-    eng = engine if engine is not None else Engine(bset, mol)
-    if eng.cut is None:
-        eng.build()
-    dev = eng.device
-    Da, Db = (torch.as_tensor(np.asarray(x, dtype=np.float64), device=dev) for x in wf.mkdens())
-    Qa, Qb = (torch.as_tensor(np.asarray(x, dtype=np.float64), device=dev) for x in wf.mkQmatrix())
-    g = eng.gradient_uhf(Da, Db, Qa, Qb).cpu().numpy()
-    for atom, gi in zip(mol.atoms, g):
-        atom.forces = -gi
-    return -g
+    return _spin_force(mol, wf.mkdens(), wf.mkQmatrix(), bset, engine)
+
+
+def fixedocc_uhf_force(mol, wf, bset, engine=None):
+    """force.py:91-108: the unrestricted forces with the fixed-occupation
+    densities wf.mk_auger_dens() and wf.mk_auger_Qmatrix()."""
+    return _spin_force(mol, wf.mk_auger_dens(), wf.mk_auger_Qmatrix(), bset, engine)
 
 
 def uhf_gradient(mol, basis_data="6-31g**", ConvCriteria=1e-11, MaxIter=400, schwarz_cut=1e-12):
@@ -162,13 +212,19 @@ def uhf_gradient(mol, basis_data="6-31g**", ConvCriteria=1e-11, MaxIter=400, sch
 
 
 def hf_force(mol, wf, bname):
-    """force.py:31-46: forces of a restricted or unrestricted wavefunction
-    (the reference's fixed-occupation branch has no counterpart here)."""
-    if wf.restricted:
-        return rhf_force(mol, wf, BasisSet(mol, bname))
-    if wf.unrestricted:
-        return uhf_force(mol, wf, BasisSet(mol, bname))
-    raise NotImplementedError("fixed-occupation wavefunctions have no GPU gradient here")
+    """force.py:31-46: the forces of the branch each of wf's flags
+    (restricted, unrestricted, fixedocc) selects, in that order, each
+    overwriting atom.forces; returns the last (None when no flag is set,
+    as the reference then computes nothing)."""
+    bset = BasisSet(mol, bname)
+    f = None
+    if getattr(wf, "restricted", False):
+        f = rhf_force(mol, wf, bset)
+    if getattr(wf, "unrestricted", False):
+        f = uhf_force(mol, wf, bset)
+    if getattr(wf, "fixedocc", False):
+        f = fixedocc_uhf_force(mol, wf, bset)
+    return f
 
 
 class RHFEnergy(torch.autograd.Function):
