@@ -179,9 +179,10 @@ int pq_engine_set_fused(pq_engine *eng, int on);
  * configuration J and K are bit-identical from call to call whatever the
  * order of execution (streams, graph replay), as the reference's sequential
  * getJ/getK loops are (Ints.py:195-223).  Bra-run caps and the LDS limit
- * select the kernels, whose integrals may differ in the last bits.  Single-density FP64 builds only (PQ_E_ARG
- * otherwise); costs extra atomics and three times the LDS per accumulator.
- * 0 (default) restores the FP64-atomic build. */
+ * select the kernels, whose integrals may differ in the last bits.  Covers
+ * one- and two-density builds (pq_fock_jk_multi) and the FP32 quartets of
+ * mixed-precision builds; costs extra atomics and three times the LDS per
+ * accumulator.  0 (default) restores the FP64-atomic build. */
 int pq_engine_set_deterministic(pq_engine *eng, int on);
 
 /* Dynamic LDS the J/K launches may use, bytes: 0 restores the device's

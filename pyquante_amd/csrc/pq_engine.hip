@@ -1324,11 +1324,9 @@ static int fock_jk_impl(pq_engine* eng, int nd, const double* const* Ds, double*
     EngineView kview = eng->view_s();
     double *Jacc = eng->d_Jacc, *Kacc = eng->d_Kacc;
     if (det) {
-      if (nd != 1 || precision != PQ_PREC_FP64) {
-        rc = fail(PQ_E_ARG, "the deterministic build digests one FP64 density");
-        break;
-      }
-      const size_t need = 6 * (size_t)nn + 3 * (size_t)eng->jp_size;
+      // per density three limb arrays each of J, K and the ket J:
+      // [J d0 | J d1 | K d0 | K d1 | ket J d0 | ket J d1]
+      const size_t need = (size_t)nd * (6 * (size_t)nn + 3 * (size_t)eng->jp_size);
       if (eng->fx_cap < need) {
         dev_free(eng->d_fx);
         eng->fx_cap = 0;
@@ -1337,9 +1335,9 @@ static int fock_jk_impl(pq_engine* eng, int nd, const double* const* Ds, double*
         ++eng->gen;
       }
       Jacc = eng->d_fx;
-      Kacc = eng->d_fx + 3 * nn;
-      kview.jpair = eng->d_fx + 6 * nn;
-      PQ_TRY(launch_zero3(Jacc, 3 * nn, Kacc, 3 * nn, kview.jpair, 3 * eng->jp_size, st));
+      Kacc = eng->d_fx + 3 * nd * nn;
+      kview.jpair = eng->d_fx + 6 * nd * nn;
+      PQ_TRY(launch_zero3(Jacc, 3 * nd * nn, Kacc, 3 * nd * nn, kview.jpair, 3 * nd * eng->jp_size, st));
     } else {
       PQ_TRY(launch_zero3(Jacc, nd * nn, Kacc, nd * nn, eng->d_jpair, nd * eng->jp_size, st));
     }
@@ -1398,7 +1396,8 @@ static int fock_jk_impl(pq_engine* eng, int nd, const double* const* Ds, double*
         hipStream_t ks = nside > 0 ? eng->side[next] : st;
         if (nside > 0) next = (next + 1) % nside;
         if (ms) PQ_TRY(mark(ks));
-        PQ_TRY(launch_eri_class(la, lb, lc, ld, det ? MODE_JKD : nd == 2 ? MODE_JK2 : MODE_JK, prec, kview,
+        const int mode = det ? (nd == 2 ? MODE_JK2D : MODE_JKD) : (nd == 2 ? MODE_JK2 : MODE_JK);
+        PQ_TRY(launch_eri_class(la, lb, lc, ld, mode, prec, kview,
                                 prec ? c.d_t32 : c.d_t64,
                                 (int)tl.size(), eng->d_sorted_s[c.Y], D_dev, Jacc, Kacc, nullptr,
                                 nullptr, ks, prec ? c.d_g32 : c.d_g64, (int)(prec ? c.g32 : c.g64).size(),
@@ -1418,10 +1417,16 @@ static int fock_jk_impl(pq_engine* eng, int nd, const double* const* Ds, double*
       jmap.ndf[y] = ncart(kPairL[y][1]);
     }
     if (det) {
-      PQ_TRY(launch_jpair_scatter_fx(kview, jmap, reinterpret_cast<long long*>(Jacc), st));
-      PQ_TRY(launch_symmetrize_fx(eng->nmol, eng->maxn, eng->nmol == 1 ? nullptr : eng->d_mol_moff, eng->d_mol_n,
-                                  nn, reinterpret_cast<const long long*>(Jacc),
-                                  reinterpret_cast<const long long*>(Kacc), Js[0], Ks[0], st));
+      for (int d = 0; d < nd && rc == PQ_OK; ++d) {
+        EngineView dv = kview;
+        dv.jpair = kview.jpair + (size_t)d * 3 * eng->jp_size;
+        long long* Jfx = reinterpret_cast<long long*>(Jacc + (size_t)d * 3 * nn);
+        const long long* Kfx = reinterpret_cast<const long long*>(Kacc + (size_t)d * 3 * nn);
+        PQ_TRY(launch_jpair_scatter_fx(dv, jmap, Jfx, st));
+        PQ_TRY(launch_symmetrize_fx(eng->nmol, eng->maxn, eng->nmol == 1 ? nullptr : eng->d_mol_moff, eng->d_mol_n,
+                                    nn, Jfx, Kfx, Js[d], Ks[d], st));
+      }
+      if (rc != PQ_OK) break;
     } else {
       PQ_TRY(launch_jpair_scatter(eng->view_s(), jmap, nd, eng->d_Jacc, st));
     }

@@ -14,7 +14,7 @@ namespace pq {
                                               int ngroups, int gflags);
 #define PQ_DECL(A, B, C_, D_) \
   PQ_DECL1(A, B, C_, D_, 0) PQ_DECL1(A, B, C_, D_, 1) PQ_DECL1(A, B, C_, D_, 2) PQ_DECL1(A, B, C_, D_, 5) \
-  PQ_DECL1(A, B, C_, D_, 6)
+  PQ_DECL1(A, B, C_, D_, 6) PQ_DECL1(A, B, C_, D_, 7) PQ_DECL1(A, B, C_, D_, 8)
 PQ_DECL(0, 0, 0, 0)
 PQ_DECL(1, 0, 0, 0)
 PQ_DECL(1, 0, 1, 0)
@@ -303,7 +303,9 @@ hipError_t launch_eri_class(int la, int lb, int lc, int ld, int mode, int fp32, 
     if (mode == MODE_JKD && !fp32)                                                                 \
       return launch_cls_##A##B##C_##D_##_m6(ev, tiles, ntiles, kets, D, Jacc, Kacc, ints, qout, st, groups, ngroups, gflags); \
     if (mode == MODE_JKD)                                                                          \
-      return hipErrorInvalidValue;                                                                 \
+      return launch_cls_##A##B##C_##D_##_m7(ev, tiles, ntiles, kets, D, Jacc, Kacc, ints, qout, st, groups, ngroups, gflags); \
+    if (mode == MODE_JK2D)                                                                         \
+      return launch_cls_##A##B##C_##D_##_m8(ev, tiles, ntiles, kets, D, Jacc, Kacc, ints, qout, st, groups, ngroups, gflags); \
     return launch_diag_##A##B##C_##D_(ev, tiles, ntiles, kets, D, Jacc, Kacc, ints, qout, st);      \
   }
   PQ_CASE(0, 0, 0, 0)

@@ -2,8 +2,9 @@
 // kernels per translation unit (PQ_LA..PQ_LD and PQ_MODE given on the
 // compiler command line: 0 J/K FP64, 1 J/K FP32, 2 store, 3 Schwarz
 // diagonal, 4 nuclear gradient, 5 J/K of two densities FP64, 6 deterministic
-// J/K FP64 with fixed-point accumulators), so the
-// instantiations compile in parallel.
+// J/K FP64 with fixed-point accumulators, 7 the same for FP32 quartets, 8
+// deterministic J/K of two densities), so the instantiations compile in
+// parallel.
 #include "pq_eri_impl.h"
 #include "pq_grad_impl.h"
 
@@ -37,6 +38,12 @@ hipError_t PQ_CAT(PQ_LA, PQ_LB, PQ_LC, PQ_LD, PQ_MODE)(const EngineView& ev, con
 #elif PQ_MODE == 6
   return launch_one<PQ_LA, PQ_LB, PQ_LC, PQ_LD, double, MODE_JKD>(ev, tiles, ntiles, kets, D, Jacc, Kacc, ints,
                                                                   qout, st, groups, ngroups, gflags);
+#elif PQ_MODE == 7
+  return launch_one<PQ_LA, PQ_LB, PQ_LC, PQ_LD, float, MODE_JKD>(ev, tiles, ntiles, kets, D, Jacc, Kacc, ints,
+                                                                 qout, st, groups, ngroups, gflags);
+#elif PQ_MODE == 8
+  return launch_one<PQ_LA, PQ_LB, PQ_LC, PQ_LD, double, MODE_JK2D>(ev, tiles, ntiles, kets, D, Jacc, Kacc, ints,
+                                                                   qout, st, groups, ngroups, gflags);
 #else
   return launch_one<PQ_LA, PQ_LB, PQ_LC, PQ_LD, double, MODE_DIAG>(ev, tiles, ntiles, kets, D, Jacc, Kacc, ints,
                                                                    qout, st, nullptr, 0, 0);

@@ -193,7 +193,7 @@ __device__ __forceinline__ void hgp_jk_block(const EngineView& ev, const PairInf
 // b), each row flushed as soon as it is complete; then the K rows of each b
 // (from the D rows of a).
 // Adds x into accumulator element p: an FP64 atomic (global memory, or LDS
-// when GLB is false), or in MODE_JKD the three fixed-point limbs of x into
+// when GLB is false), or in the fixed-point modes the three limbs of x into
 // the int64 sums at p, p + stride and p + 2 stride.
 //
 // The atomics go through pointers of their own address space (LDS or global
@@ -211,7 +211,7 @@ __device__ __forceinline__ void atomic_add_as(T* p, T x, bool glb) {
 
 template <int MODE, bool GLB>
 __device__ __forceinline__ void acc_add(double* p, int64_t stride, double x) {
-  if constexpr (MODE == MODE_JKD) {
+  if constexpr (mode_limbs(MODE) == 3) {
     const FxLimbs l = fx_split(x);
     unsigned long long* q = reinterpret_cast<unsigned long long*>(p);
     if (l.l0) atomic_add_as(q, (unsigned long long)l.l0, GLB);
@@ -223,12 +223,12 @@ __device__ __forceinline__ void acc_add(double* p, int64_t stride, double x) {
 }
 
 // Flushes accumulator element i of a block's LDS array (limbs stride lstride
-// apart in MODE_JKD) into global element p (limbs gstride apart); zero
+// apart in the fixed-point modes) into global element p (limbs gstride apart); zero
 // elements are skipped.
 template <int MODE>
 __device__ __forceinline__ void acc_flush(const double* lds, int64_t i, int64_t lstride, double* p,
                                           int64_t gstride) {
-  if constexpr (MODE == MODE_JKD) {
+  if constexpr (mode_limbs(MODE) == 3) {
     const long long* l = reinterpret_cast<const long long*>(lds);
     const long long s0 = l[i], s1 = l[i + lstride], s2 = l[i + 2 * lstride];
     unsigned long long* q = reinterpret_cast<unsigned long long*>(p);
@@ -497,15 +497,18 @@ __device__ __forceinline__ void lane_body(const EngineView& ev, const Tile& tl, 
     const double f = active ? quartet_degeneracy(bp, kp, bra, ket) : 0.0;
     // one block of integrals, digested once per density (stacked n x n
     // matrices; LDS K rows and J sums stacked the same way)
+    // (accumulators: NL limb arrays per density)
     const size_t nn = (size_t)ev.msize;
     constexpr int Y = pair_class_of(LC, LD);
+    constexpr int NL = mode_limbs(MODE);
 #pragma unroll
     for (int d = 0; d < mode_ndens(MODE); ++d)
       digest_lane<LA, LB, LC, LD, A0, NAS, B0, NBS, T, MODE, RUN>(
-          ev, bp, kp, out, f, active, D + d * nn, Jacc + d * nn, Kacc + d * nn,
-          krow ? krow + d * (C::NA + C::NB) * ev.nbf : nullptr, jsum ? jsum + d * 64 * C::NC * C::ND : nullptr,
-          drow ? drow + d * (C::NA + C::NB) * ev.nbf : nullptr, jrow ? jrow + d * C::NA * C::NB : nullptr,
-          ev.jpair + d * ev.jp_size + ev.jp_off[Y], ev.jp_n[Y], kpos,
+          ev, bp, kp, out, f, active, D + d * nn, Jacc + d * NL * nn, Kacc + d * NL * nn,
+          krow ? krow + d * NL * (C::NA + C::NB) * ev.nbf : nullptr,
+          jsum ? jsum + d * NL * 64 * C::NC * C::ND : nullptr,
+          drow ? drow + d * (C::NA + C::NB) * ev.nbf : nullptr, jrow ? jrow + d * NL * C::NA * C::NB : nullptr,
+          ev.jpair + d * NL * ev.jp_size + ev.jp_off[Y], ev.jp_n[Y], kpos,
           jab_acc ? jab_acc + d * C::NA * C::NB : nullptr);
   } else if constexpr (MODE == MODE_STORE) {
     if (!active) return;
@@ -661,7 +664,6 @@ __device__ __forceinline__ void group_body(const EngineView& ev, const Tile* __r
   else if constexpr (NT > 0) stage_rys_table<C::NR, T>(pq_lds);
   constexpr int ND = mode_ndens(MODE);
   constexpr int NL = mode_limbs(MODE);  // accumulator words per element
-  static_assert(NL == 1 || ND == 1, "fixed-point limbs are for single-density builds");
   double* krow = pq_lds + NT;
   for (int i = threadIdx.x; i < ND * NL * (C::NA + C::NB) * n; i += blockDim.x) krow[i] = 0.0;
   const PairInfo bp = ld_uniform(ev.pairs + tiles[g.x].bra);
@@ -698,26 +700,27 @@ __device__ __forceinline__ void group_body(const EngineView& ev, const Tile* __r
         ev, tl, tn.active, tl.bra, ket, D, Jacc, Kacc, nullptr, nullptr, krow, nullptr, rtab, tn.kc, tn.ks, tn.nk,
         drow, jrow, tl.ket0 + tn.slot, RJ ? jab_run : nullptr);
   }
+  constexpr int NAB = C::NA * C::NB;
   if constexpr (RJ) {
 #pragma unroll
-    for (int i = 0; i < ND * C::NA * C::NB; ++i) {
+    for (int i = 0; i < ND * NAB; ++i) {
       const double s = wave_sum(jab_run[i]);
-      if (lane == 0) acc_add<MODE, false>(&jrow[i], C::NA * C::NB, 2.0 * s);
+      if (lane == 0) acc_add<MODE, false>(&jrow[(i / NAB) * NL * NAB + i % NAB], NAB, 2.0 * s);
     }
   }
   __syncthreads();
-  for (int i = threadIdx.x; i < ND * C::NA * C::NB; i += blockDim.x) {
-    const int d = i / (C::NA * C::NB), ab = i - d * (C::NA * C::NB);
-    acc_flush<MODE>(jrow, i, C::NA * C::NB,
-                    &Jacc[(size_t)d * ev.msize + bp.moff + (bp.bfA + ab / C::NB) * ms + bp.bfB + ab % C::NB],
+  for (int i = threadIdx.x; i < ND * NAB; i += blockDim.x) {
+    const int d = i / NAB, ab = i - d * NAB;
+    acc_flush<MODE>(jrow, d * NL * NAB + ab, NAB,
+                    &Jacc[(size_t)d * NL * ev.msize + bp.moff + (bp.bfA + ab / C::NB) * ms + bp.bfB + ab % C::NB],
                     ev.msize);
   }
   for (int d = 0; d < ND; ++d)
     for (int r = 0; r < C::NA + C::NB; ++r) {
       const int grow = r < C::NA ? bp.bfA + r : bp.bfB + (r - C::NA);
       for (int col = threadIdx.x; col < ms; col += blockDim.x)
-        acc_flush<MODE>(krow, d * region + r * ms + col, region,
-                        &Kacc[(size_t)d * ev.msize + bp.moff + grow * ms + col], ev.msize);
+        acc_flush<MODE>(krow, d * NL * region + r * ms + col, region,
+                        &Kacc[(size_t)d * NL * ev.msize + bp.moff + grow * ms + col], ev.msize);
     }
 }
 
@@ -767,7 +770,6 @@ __global__ __launch_bounds__(256) PQ_WPE_ATTR void k_eri_lane(EngineView ev, con
   double* jsum = nullptr;
   constexpr int ND = mode_ndens(MODE);
   constexpr int NL = mode_limbs(MODE);  // accumulator words per element
-  static_assert(NL == 1 || ND == 1, "fixed-point limbs are for single-density builds");
   if (lds_mode) {
     krow = pq_lds;
     if (C::SPLIT) jsum = pq_lds + ND * NL * (C::NA + C::NB) * n;
@@ -818,8 +820,8 @@ __global__ __launch_bounds__(256) PQ_WPE_ATTR void k_eri_lane(EngineView ev, con
     for (int r = 0; r < C::NA + C::NB; ++r) {
       const int grow = r < C::NA ? bp.bfA + r : bp.bfB + (r - C::NA);
       for (int col = threadIdx.x; col < ms; col += blockDim.x)
-        acc_flush<MODE>(krow, d * region + r * ms + col, region,
-                        &Kacc[(size_t)d * ev.msize + bp.moff + grow * ms + col], ev.msize);
+        acc_flush<MODE>(krow, d * NL * region + r * ms + col, region,
+                        &Kacc[(size_t)d * NL * ev.msize + bp.moff + grow * ms + col], ev.msize);
     }
   if constexpr (C::SPLIT != 0) {
     constexpr int NCD = C::NC * C::ND;
@@ -831,8 +833,8 @@ __global__ __launch_bounds__(256) PQ_WPE_ATTR void k_eri_lane(EngineView ev, con
         const int cd = i >> 6, l = i & 63;
         const TileLane tl_l = tile_lane(tl.nket, l);
         if (!tl_l.active) continue;
-        acc_flush<MODE>(jsum, d * 64 * NCD + l * NCD + cd, 64 * NCD,
-                        &ev.jpair[d * ev.jp_size + ev.jp_off[Y] + cd * ev.jp_n[Y] + tl.ket0 + tl_l.slot],
+        acc_flush<MODE>(jsum, d * NL * 64 * NCD + l * NCD + cd, 64 * NCD,
+                        &ev.jpair[d * NL * ev.jp_size + ev.jp_off[Y] + cd * ev.jp_n[Y] + tl.ket0 + tl_l.slot],
                         ev.jp_size);
       }
   }

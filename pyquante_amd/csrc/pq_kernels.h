@@ -6,16 +6,17 @@
 
 namespace pq {
 
-enum { MODE_JK = 0, MODE_STORE = 1, MODE_DIAG = 2, MODE_JK2 = 3, MODE_JKD = 4 };
+enum { MODE_JK = 0, MODE_STORE = 1, MODE_DIAG = 2, MODE_JK2 = 3, MODE_JKD = 4, MODE_JK2D = 5 };
 // J/K modes and the number of densities they digest (MODE_JK2: two stacked
-// n x n densities into two stacked J/K accumulators).  MODE_JKD is the
-// deterministic single-density build: every accumulator element (global J/K,
-// position-indexed ket J, LDS K rows and J sums) is three int64 fixed-point
-// limbs (pq_fixed.h) at a stride of one array apart, so the sums do not depend
-// on the order of the atomics.
-constexpr bool mode_is_jk(int m) { return m == MODE_JK || m == MODE_JK2 || m == MODE_JKD; }
-constexpr int mode_ndens(int m) { return m == MODE_JK2 ? 2 : 1; }
-constexpr int mode_limbs(int m) { return m == MODE_JKD ? 3 : 1; }
+// n x n densities into two stacked J/K accumulators).  MODE_JKD and
+// MODE_JK2D are the deterministic builds of one and two densities: every
+// accumulator element (global J/K, position-indexed ket J, LDS K rows and J
+// sums) is three int64 fixed-point limbs (pq_fixed.h) at a stride of one
+// array apart, so the sums do not depend on the order of the atomics; the
+// limbs of density d start NL arrays after those of density d - 1.
+constexpr bool mode_is_jk(int m) { return m == MODE_JK || m == MODE_JK2 || m == MODE_JKD || m == MODE_JK2D; }
+constexpr int mode_ndens(int m) { return m == MODE_JK2 || m == MODE_JK2D ? 2 : 1; }
+constexpr int mode_limbs(int m) { return m == MODE_JKD || m == MODE_JK2D ? 3 : 1; }
 
 // classes with at most this many Cartesian integrals run one lane per quartet
 constexpr int kLaneMaxComp = 108;

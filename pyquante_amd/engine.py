@@ -161,8 +161,8 @@ class Engine(object):
         """Bit-reproducible J/K (pq_engine_set_deterministic): exact integer
         fixed-point accumulation instead of FP64 atomics, so every build of
         the same density with the same caps gives the same bits, whatever the
-        execution order (streams, graph replay).  Single-density FP64
-        builds."""
+        execution order (streams, graph replay); one or two densities, FP64
+        or mixed precision."""
         check(lib.pq_engine_set_deterministic(self.handle, 1 if on else 0))
 
     def set_lds_limit(self, nbytes=0):

@@ -144,14 +144,47 @@ def test_batch_engine_deterministic():
     assert torch.equal(J1, J0) and torch.equal(K1, K0)
 
 
-def test_deterministic_rejects_two_densities_and_mixed():
+def test_deterministic_two_densities_and_mixed_precision():
+    """The deterministic build of two densities in one pass and of
+    mixed-precision work lists (FP32 quartets, the same limb accumulation):
+    bit-identical from call to call and across 0, 1 and 4 side streams; the
+    two-density pass within 1e-12 of two single deterministic builds and
+    1e-11 of the FP64-atomic pass; the mixed build within 1e-10 of the
+    atomic mixed build, with its energy functional within 1e-6 Eh of the
+    FP64 one."""
     from pyquante_amd import _abi
-    eng = _engine((1, 1, 1), 5)
+    eng = _engine((2, 1, 1), 5)
     eng.build(1e-12, 0.0)
+    Da, Db = _density(eng.nbf, 16), _density(eng.nbf, 17)
+    atomic = eng.fock_jk_multi([Da, Db])
     eng.set_deterministic(True)
-    D = _density(eng.nbf, 16)
-    with pytest.raises(PQError):
-        eng.fock_jk_multi([D, D])
+    single = [eng.fock_jk(Da), eng.fock_jk(Db)]
+    ref = eng.fock_jk_multi([Da, Db])
+    for n in (0, 1, 4):
+        eng.set_streams(n)
+        for _ in range(2):
+            got = eng.fock_jk_multi([Da, Db])
+            for (J, K), (J0, K0) in zip(got, ref):
+                assert torch.equal(J, J0) and torch.equal(K, K0)
+    eng.set_streams(4)
+    for (J, K), (Js, Ks), (Ja, Ka) in zip(ref, single, atomic):
+        assert torch.equal(J, J.T) and torch.equal(K, K.T)
+        assert _close(J, Js, 1e-12) and _close(K, Ks, 1e-12)
+        assert _close(J, Ja) and _close(K, Ka)
+    # mixed precision
+    E64 = float((Da * (2 * single[0][0] - single[0][1])).sum())
+    eng.set_deterministic(False)
     eng.build(1e-12, 1e-4)
-    with pytest.raises(PQError):
-        eng.fock_jk(D, precision=_abi.PQ_PREC_MIXED)
+    assert eng.stats()["quartets_fp32"] > 0
+    Jm, Km = eng.fock_jk(Da, precision=_abi.PQ_PREC_MIXED)
+    eng.set_deterministic(True)
+    Jd, Kd = eng.fock_jk(Da, precision=_abi.PQ_PREC_MIXED)
+    for n in (0, 1, 4):
+        eng.set_streams(n)
+        J, K = eng.fock_jk(Da, precision=_abi.PQ_PREC_MIXED)
+        assert torch.equal(J, Jd) and torch.equal(K, Kd)
+    print("deterministic mixed vs atomic mixed: %.2e %.2e" % (torch.max(torch.abs(Jd - Jm)).item(),
+                                                            torch.max(torch.abs(Kd - Km)).item()), flush=True)
+    assert _close(Jd, Jm, 1e-10) and _close(Kd, Km, 1e-10)
+    Em = float((Da * (2 * Jd - Kd)).sum())
+    assert abs(Em - E64) < 1e-6, (Em, E64)
