@@ -195,3 +195,31 @@ def test_cgbf_list_norms_are_checked():
     bad[2].pnorms[0] *= 1.01
     with pytest.raises(ValueError, match="primitive norm"):
         shells_of(bad)
+
+
+def test_wavefunction_density_matrices():
+    """Densities of the force module's wavefunctions (Wavefunction.py:143-199)
+    on random orbitals: the fixed-occupation ones with integer occupations
+    equal the unrestricted ones, with the alpha orbital energies weighting
+    the beta energy-weighted density as the reference's mk_auger_Qmatrix
+    does; fractional occupations scale each orbital's outer product; and the
+    closed-shell density is the unrestricted alpha one."""
+    from pyquante_amd.force import FixedOccWavefunction, UHFWavefunction, Wavefunction
+    rng = np.random.default_rng(7)
+    n = 9
+    ca, cb = rng.standard_normal((n, n)), rng.standard_normal((n, n))
+    ea, eb = np.sort(rng.standard_normal(n)), np.sort(rng.standard_normal(n))
+    fx = FixedOccWavefunction(ca, cb, ea, eb, [1.0] * 4, [1.0] * 3)
+    u = UHFWavefunction((ca, cb), (ea, ea), 4, 3)
+    for x, y in zip(fx.mk_auger_dens() + fx.mk_auger_Qmatrix(), u.mkdens() + u.mkQmatrix()):
+        assert np.max(np.abs(x - y)) < 1e-13
+    occ = [1.0, 0.5, 0.25]
+    Da, _ = FixedOccWavefunction(ca, cb, ea, eb, occ, occ).mk_auger_dens()
+    ref = sum(o * np.outer(ca[:, i], ca[:, i]) for i, o in enumerate(occ))
+    assert np.max(np.abs(Da - ref)) < 1e-13
+    Qa, Qb = FixedOccWavefunction(ca, cb, ea, eb, occ, occ).mk_auger_Qmatrix()
+    refb = sum(ea[i] * o * np.outer(cb[:, i], cb[:, i]) for i, o in enumerate(occ))
+    assert np.max(np.abs(Qb - refb)) < 1e-13
+    w = Wavefunction(ca, ea, 4)
+    assert np.max(np.abs(w.mkdens() - u.mkdens()[0])) < 1e-13
+    assert np.max(np.abs(w.mkQmatrix() - u.mkQmatrix()[0])) < 1e-13
