@@ -4,10 +4,11 @@ All molecules' shells go into one engine (pq_engine_create_batch): each
 class kernel is one launch over the flat (molecule, class, tile) work list of
 the whole batch, and D, J, K, S, T, V and W are packed per molecule (molecule
 m's n_m x n_m block at offset moff[m]).  The SCF runs for all molecules at
-once: the Fock build is one HIP-graph replay over the packed density, and the
-eigensolves are one batched call on the blocks padded to the largest
-molecule (padding rows carry an identity overlap and a large diagonal, so
-their orbitals sort last and are never occupied).
+once: the Fock build is one HIP-graph replay over the packed density, the
+matrix algebra runs on the blocks padded to the largest molecule (padding
+rows carry an identity overlap and a large diagonal, so their orbitals sort
+last and are never occupied), and the eigensolves run batched per group of
+equal-size molecules on their unpadded blocks.
 """
 import ctypes
 
@@ -96,20 +97,37 @@ def _chunked(fn, *xs):
     return torch.cat(outs, 0)
 
 
-def _batched_diis_fock(Fs, Es, F, err, nkeep=8):
+class _DiisState(object):
+    """History of one batched DIIS: Fock matrices, error vectors and the Gram
+    matrix of the errors, one row added per iteration (the stored errors are
+    not multiplied again)."""
+
+    def __init__(self):
+        self.Fs, self.Es, self.G = [], [], None
+
+
+def _batched_diis_fock(st, F, err, nkeep=8):
     """Pulay DIIS per molecule on padded batches: returns the extrapolated
     Fock matrices (nmol, nmax, nmax)."""
-    Fs.append(F)
-    Es.append(err)
-    if len(Fs) > nkeep:
-        Fs.pop(0)
-        Es.pop(0)
-    k = len(Fs)
+    nm = F.shape[0]
+    if st.G is None:
+        st.G = torch.zeros((nm, nkeep, nkeep), dtype=F.dtype, device=F.device)
+    if len(st.Fs) == nkeep:  # drop the oldest entry and its Gram row and column
+        st.Fs.pop(0)
+        st.Es.pop(0)
+        st.G[:, :-1, :-1] = st.G[:, 1:, 1:].clone()
+    e = err.reshape(nm, -1)
+    st.Fs.append(F)
+    st.Es.append(e)
+    k = len(st.Fs)
+    for j in range(k):
+        g = (st.Es[j] * e).sum(1)
+        st.G[:, k - 1, j] = g
+        st.G[:, j, k - 1] = g
     if k < 2:
         return F
-    E = torch.stack(Es, 1).flatten(2)                       # (nmol, k, nmax^2)
-    B = -torch.ones((F.shape[0], k + 1, k + 1), dtype=F.dtype, device=F.device)
-    B[:, :k, :k] = E @ E.transpose(1, 2)
+    B = -torch.ones((nm, k + 1, k + 1), dtype=F.dtype, device=F.device)
+    B[:, :k, :k] = st.G[:, :k, :k]
     B[:, k, k] = 0.0
     rhs = torch.zeros((F.shape[0], k + 1), dtype=F.dtype, device=F.device)
     rhs[:, k] = -1.0
@@ -117,7 +135,10 @@ def _batched_diis_fock(Fs, Es, F, err, nkeep=8):
     scale = B[:, :k, :k].diagonal(dim1=1, dim2=2).abs().max(1).values.clamp_min(1e-300)
     B[:, :k, :k] /= scale[:, None, None]
     c = _chunked(torch.linalg.solve, B, rhs)[:, :k]
-    return torch.einsum("mk,mkij->mij", c, torch.stack(Fs, 1))
+    out = c[:, 0, None, None] * st.Fs[0]
+    for j in range(1, k):
+        out += c[:, j, None, None] * st.Fs[j]
+    return out
 
 
 def batch_rhf(be, ConvCriteria=1e-8, MaxIter=60, precision=_abi.PQ_PREC_FP64, graph=True, diis=True,
@@ -136,14 +157,30 @@ def batch_rhf(be, ConvCriteria=1e-8, MaxIter=60, precision=_abi.PQ_PREC_FP64, gr
     occ = (torch.arange(be.nmax, device=be.device)[None, :] < nocc[:, None]).to(torch.float64)
     enuke = torch.as_tensor([m.get_enuke() for m in be.mols], dtype=torch.float64, device=be.device)
 
+    # the eigensolves run per group of equal-size molecules on their n x n
+    # blocks (the padding block of X^T F X is diagonal and decoupled: its
+    # eigenvectors are unit vectors, its eigenvalues the padded diagonal,
+    # which sorts after every real one)
+    groups = []
+    for n in sorted(set(int(x) for x in be.mol_nbf)):
+        idx = torch.as_tensor(np.nonzero(be.mol_nbf == n)[0], device=be.device)
+        groups.append((n, idx))
+    eye_pad = torch.eye(be.nmax, dtype=Sp.dtype, device=Sp.device)
+
     def solve(Fp):
-        e, v = _chunked(torch.linalg.eigh, X.transpose(1, 2) @ Fp @ X)
-        return e, X @ v
+        A = X.transpose(1, 2) @ Fp @ X
+        e = torch.diagonal(A, dim1=1, dim2=2).clone()
+        V = eye_pad.expand_as(A).clone()
+        for n, idx in groups:
+            eg, vg = _chunked(torch.linalg.eigh, A[idx, :n, :n])
+            e[idx, :n] = eg
+            V[idx, :n, :n] = vg
+        return e, X @ V
 
     orbe, C = solve(hp)
     eold = torch.zeros_like(enuke)
     fg = None
-    Fs, Es = [], []
+    diis_state = _DiisState()
     it = 0
     energy = eold
     for it in range(1, MaxIter + 1):
@@ -159,7 +196,7 @@ def batch_rhf(be, ConvCriteria=1e-8, MaxIter=60, precision=_abi.PQ_PREC_FP64, gr
         energy = (Dp * (hp + Fp)).sum((1, 2)) + enuke
         if diis:
             err = Fp @ Dp @ Sp - Sp @ Dp @ Fp
-            Fp = _batched_diis_fock(Fs, Es, Fp, err)
+            Fp = _batched_diis_fock(diis_state, Fp, err)
         orbe, C = solve(Fp)
         if float((energy - eold).abs().max()) < ConvCriteria:
             break
