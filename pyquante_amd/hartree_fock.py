@@ -33,12 +33,24 @@ def chol_orth(S):
     return torch.linalg.inv(L).T
 
 
+# Largest order solved by the host's LAPACK instead of the GPU eigensolver:
+# small symmetric eigenproblems are latency-bound on the GPU (measured on
+# MI355X: 4.3 ms at n = 200 against 2.6 ms on the host with the copies; the
+# two meet near n = 400).
+HOST_EIGH_MAX = 320
+
+
 def geigh(H, S, X=None):
     """Generalised eigenproblem H C = S C e by Cholesky orthogonalisation
     (LA2.py:47-75).  Returns (eigenvalues, eigenvectors as columns)."""
     if X is None:
         X = chol_orth(S)
-    e, v = torch.linalg.eigh(X.T @ H @ X)
+    A = X.T @ H @ X
+    if A.shape[0] <= HOST_EIGH_MAX:
+        w, v = np.linalg.eigh(A.cpu().numpy())
+        e, v = torch.as_tensor(w, device=A.device), torch.as_tensor(v, device=A.device)
+    else:
+        e, v = torch.linalg.eigh(A)
     return e, X @ v
 
 
@@ -58,32 +70,62 @@ def get_energy(h, F, D, enuke=0.0):
 
 
 class DIIS(object):
-    """Pulay DIIS on the commutator error FDS - SDF (Convergence.py:73-136)."""
+    """Pulay DIIS on the commutator error FDS - SDF (Convergence.py:73-136),
+    every stored error in the system as the reference keeps them.  The
+    errors and Fock matrices live in growing device buffers; the Gram matrix
+    of the errors gains one row per call (one matrix-vector product), and
+    the small (k+1) x (k+1) system is solved on the host, as the reference's
+    LAPACK solve is."""
 
     def __init__(self, S):
-        self.Fs, self.Errs, self.S = [], [], S
+        self.S = S
         self.maxerr = None
+        self.k = 0
+        self._E = self._F = None
+        self._G = np.zeros((0, 0))
+
+    def _reserve(self, err, F):
+        cap = 0 if self._E is None else self._E.shape[0]
+        if self.k < cap:
+            return
+        ncap = max(8, 2 * cap)
+        E = torch.empty((ncap, err.numel()), dtype=err.dtype, device=err.device)
+        Fb = torch.empty((ncap,) + tuple(F.shape), dtype=F.dtype, device=F.device)
+        if cap:
+            E[:cap] = self._E
+            Fb[:cap] = self._F
+        self._E, self._F = E, Fb
 
     def getF(self, F, D):
         err = (F @ D @ self.S - self.S @ D @ F).reshape(-1)
-        self.maxerr = float(err.abs().max())
-        self.Fs.append(F)
-        self.Errs.append(err)
-        nit = len(self.Errs)
-        E = torch.stack(self.Errs)
-        a = torch.zeros((nit + 1, nit + 1), dtype=torch.float64, device=F.device)
-        a[:nit, :nit] = E @ E.T
+        self._reserve(err, F)
+        k = self.k
+        self._E[k] = err
+        self._F[k] = F
+        nit = k + 1
+        self.k = nit
+        # new Gram row and the largest error element, one copy to the host
+        row = torch.cat([self._E[:nit] @ err, err.abs().max().reshape(1)]).cpu().numpy()
+        self.maxerr = float(row[-1])
+        G = np.zeros((nit, nit))
+        G[:k, :k] = self._G
+        G[k, :] = row[:nit]
+        G[:, k] = row[:nit]
+        self._G = G
+        a = np.zeros((nit + 1, nit + 1))
+        a[:nit, :nit] = G
         a[nit, :nit] = -1.0
         a[:nit, nit] = -1.0
-        b = torch.zeros(nit + 1, dtype=torch.float64, device=F.device)
+        b = np.zeros(nit + 1)
         b[nit] = -1.0
         try:
-            c = torch.linalg.solve(a, b)
-            if not bool(torch.isfinite(c).all()):
-                raise RuntimeError("singular DIIS system")
-        except RuntimeError:
+            c = np.linalg.solve(a, b)
+        except np.linalg.LinAlgError:
             return F
-        return torch.einsum("i,ijk->ijk", c[:nit], torch.stack(self.Fs)).sum(0)
+        if not np.all(np.isfinite(c)):
+            return F
+        cd = torch.as_tensor(c[:nit], dtype=F.dtype, device=F.device)
+        return torch.tensordot(cd, self._F[:nit], dims=1)
 
 
 def rhf(atoms, **opts):
