@@ -258,3 +258,34 @@ def test_diis_incremental_gram_matches_full_system():
         ref = sum(c[i] * Fs[i] for i in range(k))
         assert float((out - ref).abs().max()) < 1e-10, it
         assert abs(d.maxerr - float(e.abs().max())) < 1e-14
+
+
+def test_batched_diis_matches_per_molecule_systems():
+    """batch._batched_diis_fock (Gram matrix one row per call, the last 8
+    entries, host solve) on CPU tensors: each molecule's extrapolated Fock
+    matrix equals the full DIIS system of its own last 8 errors."""
+    import torch
+    from pyquante_amd.batch import _batched_diis_fock, _DiisState
+    rng = np.random.default_rng(3)
+    nm, n = 5, 4
+    st = _DiisState()
+    hist = []
+    for it in range(11):  # past the 8-entry window
+        F = torch.as_tensor(rng.standard_normal((nm, n, n)))
+        err = torch.as_tensor(rng.standard_normal((nm, n, n)))
+        out = _batched_diis_fock(st, F, err)
+        hist = (hist + [(F, err)])[-8:]
+        k = len(hist)
+        if k < 2:
+            assert torch.equal(out, F)
+            continue
+        for m in range(nm):
+            E = np.array([h[1][m].reshape(-1).numpy() for h in hist])
+            B = -np.ones((k + 1, k + 1))
+            B[:k, :k] = E @ E.T
+            B[k, k] = 0.0
+            r = np.zeros(k + 1)
+            r[k] = -1.0
+            c = np.linalg.solve(B, r)[:k]
+            ref = sum(c[j] * hist[j][0][m].numpy() for j in range(k))
+            assert np.abs(out[m].numpy() - ref).max() < 1e-9, (it, m)
