@@ -1,6 +1,7 @@
 """GPU: the reference's own SCF test cases (Tests/he.py, ne.py, h2.py,
 ohm.py, he_mp2.py, lih_mp2.py), run through the drop-in rhf and MP2 with the
-reference's defaults (6-31G**, ConvCriteria 1e-4, MaxIter 20, no DIIS), each
+reference's defaults (6-31G**, ConvCriteria 1e-4, MaxIter 20, no DIIS), and
+the HF energy of H2 in Tests/MiniSweet.py, each
 against the golden energy and tolerance its file states (1e-4 where it states
 none): a single atom (every shell pair on one centre), a two-centre s/p
 molecule, an anion, and MP2 on top of the SCF."""
@@ -37,6 +38,13 @@ def test_h2_rhf():
     en = _rhf(Molecule("h2", atomlist=[(1, (0, 0, 0)), (1, (1.0, 0, 0))]))
     print("H2 %.10f" % en, flush=True)
     assert abs(en - (-1.082098)) < 1e-6          # Tests/h2.py:15,33
+
+
+def test_h2_rhf_minisweet():
+    from pyquante_amd.molecule import Molecule
+    en = _rhf(Molecule("H2", atomlist=[(1, (0.35, 0, 0)), (1, (-0.35, 0, 0))], units="Angs"))
+    print("H2 (0.7 A) %.10f" % en, flush=True)
+    assert round(en - (-1.130501), 4) == 0       # Tests/MiniSweet.py:25 (assertAlmostEqual, 4 places)
 
 
 def test_ohm_rhf():
