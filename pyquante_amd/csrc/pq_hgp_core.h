@@ -74,6 +74,15 @@ struct HgpCls {
 
 constexpr int kHgpUnrollMax = 40;
 
+// Step-table classes whose contracted targets are few enough to stay in
+// registers (the accumulation and the horizontal recurrence then use
+// compile-time indices; the recurrence table itself stays in private memory):
+// (pp|ps), (ds|ps) and (dd|ss).  Measured: get2ints 23.4 -> 20.4 ms on
+// (H2O)_8 6-31G**; with (ds|ds) or (dp|ps) as well the shared launch needs
+// 253 registers and falls to one wave per SIMD.
+template <int LA, int LB, int LC, int LD>
+constexpr bool kHgpAccInRegs = HgpCls<LA, LB, LC, LD>::SIZE > kHgpUnrollMax && HgpCls<LA, LB, LC, LD>::NT <= 31;
+
 // Step tables of the rolled variant.  A VRR step writes nm orders at o from
 // the entries at o1 (and o2, oe); dir is the recurrence direction, c1 the
 // power of the lower entry along it (bra e_i - 1, ket f_j - 1), c2 the bra
@@ -177,11 +186,11 @@ __constant__ HgpSteps<LA, LB, LC, LD> kHgpSteps = HgpSteps<LA, LB, LC, LD>::make
 __device__ __forceinline__ double pick3(const double (&v)[3], int d) { return d == 0 ? v[0] : (d == 1 ? v[1] : v[2]); }
 
 // The vertical recurrence of hgp_vrr driven by the step tables.  The table
-// lives in per-lane private memory, so each step first loads the whole order
-// range of its operands (unconditionally: V is padded so that the reads past
-// an entry's last order stay inside the array) and then computes and stores
-// its nm orders: the loads of one step are in flight together instead of one
-// order at a time.  acc (NTP entries) takes the targets in batches of 8 for
+// lives in per-lane private memory, so each step first loads the orders of
+// its operands that it uses (behind uniform branches, so that the loads are
+// still issued back to back; V is padded so that the reads stay inside it)
+// and then computes and stores its nm orders: the loads of one step are in
+// flight together instead of one order at a time.  acc (NTP entries) takes the targets in batches of 8 for
 // the same reason.  (Loading the orders two at a time, from entries aligned
 // to even offsets, measured no faster.)
 template <int LA, int LB, int LC, int LD>
@@ -218,10 +227,12 @@ __device__ __forceinline__ void hgp_vrr_rolled(const PairInfo& bp, const PrimPai
     const double pa = pick3(PA, st.dir), wp = pick3(WP, st.dir), c2 = st.c1 * s.i2p;
     double x1[L + 1], x2[L + 1];
 #pragma unroll
-    for (int m = 0; m <= L; ++m) x1[m] = V[st.o1 + m];
+    for (int m = 0; m <= L; ++m)
+      if (m <= st.nm) x1[m] = V[st.o1 + m];
     if (st.c1) {
 #pragma unroll
-      for (int m = 0; m <= L; ++m) x2[m] = V[st.o2 + m];
+      for (int m = 0; m <= L; ++m)
+        if (m <= st.nm) x2[m] = V[st.o2 + m];
     }
 #pragma unroll
     for (int m = 0; m < L; ++m) {
@@ -238,14 +249,17 @@ __device__ __forceinline__ void hgp_vrr_rolled(const PairInfo& bp, const PrimPai
     const double qc = pick3(QC, st.dir), wq = pick3(WQ, st.dir), c2 = st.c1 * s.i2q, c3 = st.c2 * i2pq;
     double x1[LCD + 1], x2[LCD + 1], xe[LCD > 0 ? LCD : 1];
 #pragma unroll
-    for (int m = 0; m <= LCD; ++m) x1[m] = V[st.o1 + m];
+    for (int m = 0; m <= LCD; ++m)
+      if (m <= st.nm) x1[m] = V[st.o1 + m];
     if (st.c1) {
 #pragma unroll
-      for (int m = 0; m <= LCD; ++m) x2[m] = V[st.o2 + m];
+      for (int m = 0; m <= LCD; ++m)
+        if (m <= st.nm) x2[m] = V[st.o2 + m];
     }
     if (st.c2) {
 #pragma unroll
-      for (int m = 0; m < LCD; ++m) xe[m] = V[st.oe + m + 1];
+      for (int m = 0; m < LCD; ++m)
+        if (m < st.nm) xe[m] = V[st.oe + m + 1];
     }
 #pragma unroll
     for (int m = 0; m < LCD; ++m) {
@@ -256,13 +270,20 @@ __device__ __forceinline__ void hgp_vrr_rolled(const PairInfo& bp, const PrimPai
       V[st.o + m] = v;
     }
   }
+  if constexpr (kHgpAccInRegs<LA, LB, LC, LD>) {
+    // few targets: every index into acc a compile-time constant, so acc
+    // stays in registers
+#pragma unroll
+    for (int t = 0; t < H::NT; ++t) acc[t] += V[T.tgt[t]];
+  } else {
 #pragma unroll 1
-  for (int t0 = 0; t0 < H::NTP; t0 += 8) {
-    double x[8];
+    for (int t0 = 0; t0 < H::NTP; t0 += 8) {
+      double x[8];
 #pragma unroll
-    for (int j = 0; j < 8; ++j) x[j] = V[T.tgt[t0 + j]];
+      for (int j = 0; j < 8; ++j) x[j] = V[T.tgt[t0 + j]];
 #pragma unroll
-    for (int j = 0; j < 8; ++j) acc[t0 + j] += x[j];
+      for (int j = 0; j < 8; ++j) acc[t0 + j] += x[j];
+    }
   }
 }
 

@@ -62,7 +62,7 @@ __device__ __forceinline__ void hgp_tile(const EngineView& ev, const Tile& tl, c
   // a lane holding part of the primitive sum adds its block into the zeroed
   // array; a whole quartet is stored
   const bool part = tl.prim != 0 || tn.ks > 1;
-  constexpr int NACC = H::SIZE > kHgpUnrollMax ? H::NTP : H::NT;
+  constexpr int NACC = H::SIZE > kHgpUnrollMax && !kHgpAccInRegs<LA, LB, LC, LD> ? H::NTP : H::NT;
   double acc[NACC];
 #pragma unroll
   for (int c = 0; c < NACC; ++c) acc[c] = 0.0;
@@ -89,7 +89,7 @@ __device__ __forceinline__ void hgp_tile(const EngineView& ev, const Tile& tl, c
     CDp[d][2] = CD[d] * CD[d];
   }
   const bool same = bra == ket;
-  if constexpr (H::SIZE > kHgpUnrollMax) {
+  if constexpr (H::SIZE > kHgpUnrollMax && !kHgpAccInRegs<LA, LB, LC, LD>) {
     using S = HgpSteps<LA, LB, LC, LD>;
     const S& T = kHgpSteps<LA, LB, LC, LD>;
     constexpr int NABS = S::NAB / kHgpSplit<LA, LB, LC, LD>;
