@@ -193,9 +193,12 @@ __device__ __forceinline__ double pick3(const double (&v)[3], int d) { return d 
 // flight together instead of one order at a time.  acc (NTP entries) takes the targets in batches of 8 for
 // the same reason.  (Loading the orders two at a time, from entries aligned
 // to even offsets, measured no faster.)
+// V: the caller's table (HgpCls::SIZE + L + 1 entries); with accumulate
+// false the targets stay in V (a quartet of one primitive quartet, whose
+// horizontal recurrence then reads them there).
 template <int LA, int LB, int LC, int LD>
 __device__ __forceinline__ void hgp_vrr_rolled(const PairInfo& bp, const PrimPair& b, const PairInfo& kp,
-                                               const PrimPair& q, double* acc) {
+                                               const PrimPair& q, double* acc, double* V, bool accumulate) {
   using H = HgpCls<LA, LB, LC, LD>;
   using S = HgpSteps<LA, LB, LC, LD>;
   const S& T = kHgpSteps<LA, LB, LC, LD>;
@@ -203,7 +206,6 @@ __device__ __forceinline__ void hgp_vrr_rolled(const PairInfo& bp, const PrimPai
   const PrimQ s = prim_setup(bp, b, kp, q);
   double u[H::NR], w[H::NR];
   rys_roots<H::NR, double>(s.X, u, w);
-  double V[H::SIZE + L + 1];
 #pragma unroll
   for (int r = 0; r < H::NR; ++r) w[r] *= s.pref;
 #pragma unroll
@@ -275,7 +277,7 @@ __device__ __forceinline__ void hgp_vrr_rolled(const PairInfo& bp, const PrimPai
     // stays in registers
 #pragma unroll
     for (int t = 0; t < H::NT; ++t) acc[t] += V[T.tgt[t]];
-  } else {
+  } else if (accumulate) {
 #pragma unroll 1
     for (int t0 = 0; t0 < H::NTP; t0 += 8) {
       double x[8];

@@ -62,10 +62,18 @@ __device__ __forceinline__ void hgp_tile(const EngineView& ev, const Tile& tl, c
   // a lane holding part of the primitive sum adds its block into the zeroed
   // array; a whole quartet is stored
   const bool part = tl.prim != 0 || tn.ks > 1;
-  constexpr int NACC = H::SIZE > kHgpUnrollMax && !kHgpAccInRegs<LA, LB, LC, LD> ? H::NTP : H::NT;
+  // RS: the step-table classes whose targets live in private memory; a
+  // quartet of one primitive quartet (single) leaves them in the recurrence
+  // table V, and the horizontal recurrence reads them there
+  constexpr bool RS = H::SIZE > kHgpUnrollMax && !kHgpAccInRegs<LA, LB, LC, LD>;
+  constexpr int NACC = RS ? H::NTP : H::NT;
+  const bool single = RS && i1 - i0 == 1 && k1 - k0 == 1;
   double acc[NACC];
+  double V[RS || H::SIZE > kHgpUnrollMax ? H::SIZE + H::L + 1 : 1];
+  if (!single) {
 #pragma unroll
-  for (int c = 0; c < NACC; ++c) acc[c] = 0.0;
+    for (int c = 0; c < NACC; ++c) acc[c] = 0.0;
+  }
 #pragma unroll 1
   for (int k = k0; k < k1; ++k) {
     const PrimPair q = ev.prims[kp.pp_off + k];
@@ -74,7 +82,7 @@ __device__ __forceinline__ void hgp_tile(const EngineView& ev, const Tile& tl, c
       if constexpr (H::SIZE <= kHgpUnrollMax)
         hgp_vrr<LA, LB, LC, LD>(bp, ev.prims[bp.pp_off + i], kp, q, acc);
       else
-        hgp_vrr_rolled<LA, LB, LC, LD>(bp, ev.prims[bp.pp_off + i], kp, q, acc);
+        hgp_vrr_rolled<LA, LB, LC, LD>(bp, ev.prims[bp.pp_off + i], kp, q, acc, V, !single);
     }
   }
   // powers of A - B and C - D for the horizontal recurrence
@@ -89,11 +97,15 @@ __device__ __forceinline__ void hgp_tile(const EngineView& ev, const Tile& tl, c
     CDp[d][2] = CD[d] * CD[d];
   }
   const bool same = bra == ket;
-  if constexpr (H::SIZE > kHgpUnrollMax && !kHgpAccInRegs<LA, LB, LC, LD>) {
+  if constexpr (RS) {
     using S = HgpSteps<LA, LB, LC, LD>;
     const S& T = kHgpSteps<LA, LB, LC, LD>;
     constexpr int NABS = S::NAB / kHgpSplit<LA, LB, LC, LD>;
     const int ab0 = split * NABS;
+    // the contracted targets: acc, or V through the target table
+    auto hrr = [&](auto from_v) {
+      constexpr bool FV = decltype(from_v)::value;
+      auto tgt = [&](int t) -> double { return FV ? V[T.tgt[t]] : acc[t]; };
 #pragma unroll 1
     for (int ab = ab0; ab < ab0 + NABS; ++ab) {
       const int a = ab / H::NB, bb = ab % H::NB;
@@ -113,7 +125,7 @@ __device__ __forceinline__ void hgp_tile(const EngineView& ev, const Tile& tl, c
 #pragma unroll
           for (int y = 0; y < MTK; ++y) {
             const HgpTerm tk = T.kt[cd][y];
-            r += tk.cb * CDp[0][tk.p0] * CDp[1][tk.p1] * CDp[2][tk.p2] * acc[tb.idx * H::NFC + tk.idx];
+            r += tk.cb * CDp[0][tk.p0] * CDp[1][tk.p1] * CDp[2][tk.p2] * tgt(tb.idx * H::NFC + tk.idx);
           }
           v += tb.cb * ABp[0][tb.p0] * ABp[1][tb.p1] * ABp[2][tb.p2] * r;
         }
@@ -123,6 +135,9 @@ __device__ __forceinline__ void hgp_tile(const EngineView& ev, const Tile& tl, c
         else *dst = v;
       }
     }
+    };
+    if (single) hrr(std::true_type{});
+    else hrr(std::false_type{});
     return;
   }
   double out[H::NA * H::NB * H::NC * H::ND];
