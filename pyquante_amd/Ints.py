@@ -81,6 +81,8 @@ class Integrals(object):
         self._dev = None
         self._host = None
         self._built_cut = None
+        # the last two direct builds: (precision, generation, D copy, J, K)
+        self._recent = []
 
     def __len__(self):
         return self._len
@@ -107,12 +109,24 @@ class Integrals(object):
         return a if dtype is None else a.astype(dtype)
 
     def jk(self, D, want_j=True, want_k=True, precision=_abi.PQ_PREC_FP64):
+        """(J, K) of a symmetric device density (None for the one not
+        wanted).  A direct build computes both; the last two builds are kept
+        with a copy of their density, so that getJ and getK of one density
+        (as the reference's UHF loop calls them, hartree_fock.py:233-236)
+        cost one build: a later call whose density equals a kept one
+        element for element returns copies of its J and K."""
         if self.method == "hgp":
             return digest_stored(self.device_array(), D, want_j, want_k)
         if self._built_cut != schwarz_cut or self.engine.fp32cut:
             self.engine.build(schwarz_cut, 0.0)
             self._built_cut = schwarz_cut
-        return self.engine.fock_jk(D, want_j, want_k, precision)
+        gen = (self.engine.generation(), getattr(self.engine, "deterministic", False))
+        for p, g, Dk, J, K in self._recent:
+            if p == precision and g == gen and torch.equal(Dk, D):
+                return (J.clone() if want_j else None), (K.clone() if want_k else None)
+        J, K = self.engine.fock_jk(D, True, True, precision)
+        self._recent = [(precision, gen, D.clone(), J, K)] + self._recent[:1]
+        return (J.clone() if want_j else None), (K.clone() if want_k else None)
 
 
 def ijkl2intindex(i, j, k, l):

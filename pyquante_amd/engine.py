@@ -164,6 +164,7 @@ class Engine(object):
         execution order (streams, graph replay); one or two densities, FP64
         or mixed precision."""
         check(lib.pq_engine_set_deterministic(self.handle, 1 if on else 0))
+        self.deterministic = bool(on)
 
     def set_lds_limit(self, nbytes=0):
         """Dynamic LDS limit of the J/K launches (pq_engine_set_lds_limit;

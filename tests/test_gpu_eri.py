@@ -389,3 +389,46 @@ def _unique_ijkl(n):
                     if i * (i + 1) // 2 + j <= k * (k + 1) // 2 + l:
                         out.append((i, j, k, l))
     return out
+
+
+def test_getj_getk_of_one_density_share_a_build():
+    """getJ and getK of one density (the reference's UHF loop calls them
+    separately, hartree_fock.py:233-236) cost one direct build: Integrals
+    keeps its last two builds and answers a density equal to a kept one
+    from them.  Results equal a fresh get2JmK; a changed density, or one
+    edited in place, is rebuilt; returned matrices are copies."""
+    from pyquante_amd.Ints import getints, get2JmK, getJ, getK
+    from pyquante_amd.basis import BasisSet
+    mol = _h2o_631ss()
+    bs = BasisSet(mol, "6-31g**")
+    _, _, Ints = getints(bs, mol)
+    calls = []
+    real = Ints.engine.fock_jk
+
+    def counting(*a, **k):
+        calls.append(1)
+        return real(*a, **k)
+
+    Ints.engine.fock_jk = counting
+    rng = np.random.default_rng(3)
+    n = len(bs.bfs)
+    A = rng.standard_normal((n, 6))
+    Da = A @ A.T
+    B = rng.standard_normal((n, 6))
+    Db = B @ B.T
+    Ja, Jb = getJ(Ints, Da), getJ(Ints, Db)
+    Ka, Kb = getK(Ints, Da), getK(Ints, Db)
+    assert len(calls) == 2
+    Ga = get2JmK(Ints, Da)
+    assert len(calls) == 2 and np.max(np.abs(Ga - (2 * Ja - Ka))) < 1e-12
+    Jt = getJ(Ints, torch.as_tensor(Da, device="cuda"))  # a device density: a device J
+    assert len(calls) == 2 and np.max(np.abs(Jt.cpu().numpy() - Ja)) < 1e-12
+    Jt += 1.0  # a copy: the kept J is untouched
+    assert np.max(np.abs(getJ(Ints, Da) - Ja)) < 1e-12
+    Da[1, 2] += 0.01
+    Da[2, 1] += 0.01
+    J2, K2 = getJ(Ints, Da), getK(Ints, Da)
+    assert len(calls) == 3
+    Ints.engine.fock_jk = real
+    Jf, Kf = real(torch.as_tensor(Da, device="cuda"))
+    assert np.max(np.abs(J2 - Jf.cpu().numpy())) < 1e-12 and np.max(np.abs(K2 - Kf.cpu().numpy())) < 1e-12
