@@ -122,24 +122,27 @@ __device__ __forceinline__ void eri_block(const EngineView& ev, const PairInfo& 
 }
 
 // ---------------------------------------------------------------------
-// The J/K classes of total angular momentum 2 ((ps|ps), (pp|ss), (ds|ss)):
-// Head-Gordon-Pople recurrences instead of Rys quadrature.  Per primitive
-// quartet the Boys values F_0..F_2 come from the grid table (staged in LDS
-// by the bra-run kernels; 13 doubles and 30 FMAs) and the Obara-Saika
-// vertical recurrence builds the [e0|f0] targets (tables of 15-18 entries
-// in registers); those are summed over the contraction and the horizontal
-// recurrence runs once per contracted block.  Two-root Rys quadrature reads
-// 48 table coefficients and spends 44 FMAs on the roots alone.  Measured on
-// (H2O)_8 (kernel time inside the concurrent build): (pp|ss) 0.206 ->
-// 0.148 ms, (ds|ss) 0.102 -> 0.093, (ps|ps) 0.309 -> 0.303.  With one root
-// (L <= 1) the two cost the same per primitive quartet and the Rys kernels
-// hold fewer registers ((ss|ss) 0.117 against 0.128 ms), so they stay Rys.
-// Same block layout as eri_block; FP64 only (the FP32 tiles of mixed
-// builds keep the Rys path).
+// The J/K classes of total angular momentum 2 and 3 ((ps|ps), (pp|ss),
+// (ds|ss); (pp|ps), (ds|ps), (dp|ss)): Head-Gordon-Pople recurrences instead
+// of Rys quadrature.  Per primitive quartet the Boys values F_0..F_L come
+// from the grid table (staged in LDS by the bra-run kernels; L + 11 doubles
+// and a Taylor sum per order) and the Obara-Saika vertical recurrence builds
+// the [e0|f0] targets (tables of 15-52 entries, every index a compile-time
+// constant, so they stay in registers); those are summed over the
+// contraction and the horizontal recurrence runs once per contracted block.
+// Two-root Rys quadrature reads 48 table coefficients and spends 44 FMAs on
+// the roots alone.  Measured on (H2O)_8 6-31G**, kernel time inside the
+// concurrent build: (pp|ss) 0.206 -> 0.148 ms, (ds|ss) 0.102 -> 0.093,
+// (ps|ps) 0.309 -> 0.303; one class at a time: (pp|ps) 0.214 -> 0.166 ms,
+// (ds|ps) 0.083 -> 0.064, (dp|ss) 0.099 -> 0.077.  With one root (L <= 1)
+// the two cost the same per primitive quartet and the Rys kernels hold fewer
+// registers ((ss|ss) 0.117 against 0.128 ms), so they stay Rys; from L = 4
+// the tables outgrow the register file.  Same block layout as eri_block;
+// FP64 only (the FP32 tiles of mixed builds keep the Rys path).
 // ---------------------------------------------------------------------
 template <int LA, int LB, int LC, int LD, typename T, int MODE>
-constexpr bool kHgpJK = mode_is_jk(MODE) && std::is_same<T, double>::value && LA + LB + LC + LD == 2 &&
-                        HgpCls<LA, LB, LC, LD>::SIZE <= kHgpUnrollMax;
+constexpr bool kHgpJK = mode_is_jk(MODE) && std::is_same<T, double>::value &&
+                        (LA + LB + LC + LD == 2 || LA + LB + LC + LD == 3);
 
 template <int LA, int LB, int LC, int LD, bool LT>
 __device__ __forceinline__ void hgp_jk_block(const EngineView& ev, const PairInfo& bp, const PairInfo& kp, int i0,

@@ -70,16 +70,22 @@ struct HgpCls {
     return s;
   }
   static constexpr int SIZE = off(NE, 0);
+  static_assert(off(0, 0) == 0, "the [00|00] orders start the table");
 };
 
-constexpr int kHgpUnrollMax = 40;
+// Classes up to this many table entries run the unrolled recurrence (tables
+// in registers): the s/p classes up to (pp|ps), and (ds|ps).  With 70
+// ((dd|ss) as well) the shared store launch needs 256 registers; at 52 it
+// takes (H2O)_8 get2ints from 19.7 to 16.9 ms against 40.
+constexpr int kHgpUnrollMax = 52;
 
 // Step-table classes whose contracted targets are few enough to stay in
 // registers (the accumulation and the horizontal recurrence then use
 // compile-time indices; the recurrence table itself stays in private memory):
-// (pp|ps), (ds|ps) and (dd|ss).  Measured: get2ints 23.4 -> 20.4 ms on
-// (H2O)_8 6-31G**; with (ds|ds) or (dp|ps) as well the shared launch needs
-// 253 registers and falls to one wave per SIMD.
+// (dd|ss).  Measured with (pp|ps) and (ds|ps) in this form too, before they
+// were unrolled: get2ints 23.4 -> 20.4 ms on (H2O)_8 6-31G**; with (ds|ds) or
+// (dp|ps) as well the shared launch needs 253 registers and falls to one wave
+// per SIMD.
 template <int LA, int LB, int LC, int LD>
 constexpr bool kHgpAccInRegs = HgpCls<LA, LB, LC, LD>::SIZE > kHgpUnrollMax && HgpCls<LA, LB, LC, LD>::NT <= 31;
 
@@ -216,7 +222,7 @@ __device__ __forceinline__ void hgp_vrr_rolled(const PairInfo& bp, const PrimPai
       f += w[r];
       w[r] *= u[r];
     }
-    V[H::off(0, 0) + m] = f;
+    V[m] = f;  // [00|00] entries start the table
   }
   const double PA[3] = {s.PA[0], s.PA[1], s.PA[2]}, QC[3] = {s.QC[0], s.QC[1], s.QC[2]};
   const double WP[3] = {-s.qfac * s.PQ[0], -s.qfac * s.PQ[1], -s.qfac * s.PQ[2]};
@@ -304,7 +310,7 @@ __device__ __forceinline__ void hgp_vrr(const PairInfo& bp, const PrimPair& b, c
     double F[H::L + 1];
     boys_grid_eval<H::L, LT>(s.X, F, btab);
 #pragma unroll
-    for (int m = 0; m <= H::L; ++m) V[H::off(0, 0) + m] = s.pref * F[m];
+    for (int m = 0; m <= H::L; ++m) V[m] = s.pref * F[m];  // [00|00] entries start the table
   } else {
     double u[H::NR], w[H::NR];
     rys_roots<H::NR, double>(s.X, u, w);
@@ -317,7 +323,7 @@ __device__ __forceinline__ void hgp_vrr(const PairInfo& bp, const PrimPair& b, c
         f += w[r];
         w[r] *= u[r];
       }
-      V[H::off(0, 0) + decltype(M)::value] = f;
+      V[decltype(M)::value] = f;
     });
   }
   // W - P = -q/(p+q) (P - Q),  W - Q = p/(p+q) (P - Q)
@@ -371,8 +377,10 @@ __device__ __forceinline__ void hgp_vrr(const PairInfo& bp, const PrimPair& b, c
   });
   static_for<H::E0, H::NE>([&](auto E) {
     static_for<H::F0, H::NF>([&](auto F) {
-      constexpr int e = decltype(E)::value, f = decltype(F)::value;
-      acc[(e - H::E0) * H::NFC + (f - H::F0)] += V[H::off(e, f)];
+      // offsets as constants: off() left to the optimiser is a loop over the
+      // table that, rolled, puts V in private memory
+      constexpr int e = decltype(E)::value, f = decltype(F)::value, o = H::off(e, f);
+      acc[(e - H::E0) * H::NFC + (f - H::F0)] += V[o];
     });
   });
 }
