@@ -1164,9 +1164,11 @@ int pq_get2ints_hgp(pq_engine* eng, double* out_dev, uint64_t len, int compat_nr
     // runtime allocates per hardware queue; spread over the side streams'
     // queues as well, it outgrew the runtime's scratch limit and the next
     // large-scratch dispatch failed (HSA_STATUS_ERROR_OUT_OF_RESOURCES)
-    // all classes share one launch (heaviest class first, so that its
-    // long-running waves start first); the per-class launches remain for a
-    // list beyond the launch's segment table
+    // the step-table classes and the small unrolled ones share one launch
+    // (heaviest class first, so that its long-running waves start first);
+    // the large unrolled classes (hgp_multi_class false: 1-2 waves per SIMD
+    // of registers, no private memory) run in launches of their own ahead of
+    // it, as does a list beyond the launch's segment table
     HgpMulti mm{};
     int64_t nb = 0;
     for (auto it = eng->hgp_work.rbegin(); it != eng->hgp_work.rend(); ++it) {

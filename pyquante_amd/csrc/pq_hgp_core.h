@@ -79,15 +79,19 @@ struct HgpCls {
 // takes (H2O)_8 get2ints from 19.7 to 16.9 ms against 40.
 constexpr int kHgpUnrollMax = 52;
 
-// Step-table classes whose contracted targets are few enough to stay in
-// registers (the accumulation and the horizontal recurrence then use
-// compile-time indices; the recurrence table itself stays in private memory):
-// (dd|ss).  Measured with (pp|ps) and (ds|ps) in this form too, before they
-// were unrolled: get2ints 23.4 -> 20.4 ms on (H2O)_8 6-31G**; with (ds|ds) or
-// (dp|ps) as well the shared launch needs 253 registers and falls to one wave
-// per SIMD.
+// Classes above kHgpUnrollMax and up to kHgpAloneMax entries ((dd|ss),
+// (dp|ps), (ds|pp), (ds|ds), (pp|pp), (dd|ps), (dp|ds), (dp|pp)) run the
+// unrolled recurrence too, each in a launch of its own: unrolled they need
+// 224-256 registers (1-2 waves per SIMD; (dp|pp) spills 310 of them), which
+// inside the shared launch would set the occupancy of every class, but their
+// recurrence tables leave private memory for registers.  Measured on (H2O)_8
+// 6-31G** get2ints: 16.9 -> 8.7 ms (with (dd|ps), (dp|ds) and (dp|pp) kept
+// on step tables 11.8 ms).
+constexpr int kHgpAloneMax = 300;
 template <int LA, int LB, int LC, int LD>
-constexpr bool kHgpAccInRegs = HgpCls<LA, LB, LC, LD>::SIZE > kHgpUnrollMax && HgpCls<LA, LB, LC, LD>::NT <= 31;
+constexpr bool kHgpUnrolled = HgpCls<LA, LB, LC, LD>::SIZE <= kHgpAloneMax;
+template <int LA, int LB, int LC, int LD>
+constexpr bool kHgpAlone = HgpCls<LA, LB, LC, LD>::SIZE > kHgpUnrollMax && kHgpUnrolled<LA, LB, LC, LD>;
 
 // Step tables of the rolled variant.  A VRR step writes nm orders at o from
 // the entries at o1 (and o2, oe); dir is the recurrence direction, c1 the
@@ -278,12 +282,7 @@ __device__ __forceinline__ void hgp_vrr_rolled(const PairInfo& bp, const PrimPai
       V[st.o + m] = v;
     }
   }
-  if constexpr (kHgpAccInRegs<LA, LB, LC, LD>) {
-    // few targets: every index into acc a compile-time constant, so acc
-    // stays in registers
-#pragma unroll
-    for (int t = 0; t < H::NT; ++t) acc[t] += V[T.tgt[t]];
-  } else if (accumulate) {
+  if (accumulate) {
 #pragma unroll 1
     for (int t0 = 0; t0 < H::NTP; t0 += 8) {
       double x[8];

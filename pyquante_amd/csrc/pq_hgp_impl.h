@@ -19,14 +19,15 @@
 // F_m(X) = sum_r w_r t_r^(2m) holds exactly for m < 2 (L/2+1), which covers
 // every m <= L, and every term is positive.
 //
-// Up to kHgpUnrollMax table entries (the s/p classes up to (pp|ps) and
-// (ds|ps)) every index of the recurrences is a compile-time constant
-// (static_for over the Cartesian triples), and the tables stay in registers.
-// The larger classes walk step tables in constant memory instead (uniform
-// loads), with the recurrence table in per-lane private memory (at most 3,366
-// doubles): unrolled, their straight-line code spills and outgrows the
-// instruction cache, and ran 10-40x slower per tile (measured), and (dd|dd)
-// took 12 minutes to compile.
+// Up to kHgpAloneMax table entries (every class but (dp|dp), (dd|pp),
+// (dd|ds), (dd|dp) and (dd|dd)) every index of the recurrences is a
+// compile-time constant (static_for over the Cartesian triples), and the
+// tables stay in registers; the classes above kHgpUnrollMax among them run in
+// launches of their own.  The five larger classes walk step tables in
+// constant memory instead (uniform loads), with the recurrence table in
+// per-lane private memory (at most 3,366 doubles): unrolled, (dp|pp) at 285
+// entries already spills 310 registers, and (dd|dd) took 12 minutes to
+// compile.
 #pragma once
 #include <type_traits>
 
@@ -65,11 +66,11 @@ __device__ __forceinline__ void hgp_tile(const EngineView& ev, const Tile& tl, c
   // RS: the step-table classes whose targets live in private memory; a
   // quartet of one primitive quartet (single) leaves them in the recurrence
   // table V, and the horizontal recurrence reads them there
-  constexpr bool RS = H::SIZE > kHgpUnrollMax && !kHgpAccInRegs<LA, LB, LC, LD>;
+  constexpr bool RS = !kHgpUnrolled<LA, LB, LC, LD>;
   constexpr int NACC = RS ? H::NTP : H::NT;
   const bool single = RS && i1 - i0 == 1 && k1 - k0 == 1;
   double acc[NACC];
-  double V[RS || H::SIZE > kHgpUnrollMax ? H::SIZE + H::L + 1 : 1];
+  double V[kHgpUnrolled<LA, LB, LC, LD> ? 1 : H::SIZE + H::L + 1];
   if (!single) {
 #pragma unroll
     for (int c = 0; c < NACC; ++c) acc[c] = 0.0;
@@ -79,7 +80,7 @@ __device__ __forceinline__ void hgp_tile(const EngineView& ev, const Tile& tl, c
     const PrimPair q = ev.prims[kp.pp_off + k];
 #pragma unroll 1
     for (int i = i0; i < i1; ++i) {
-      if constexpr (H::SIZE <= kHgpUnrollMax)
+      if constexpr (kHgpUnrolled<LA, LB, LC, LD>)
         hgp_vrr<LA, LB, LC, LD>(bp, ev.prims[bp.pp_off + i], kp, q, acc);
       else
         hgp_vrr_rolled<LA, LB, LC, LD>(bp, ev.prims[bp.pp_off + i], kp, q, acc, V, !single);

@@ -26,7 +26,8 @@ __global__ __launch_bounds__(64) void k_hgp_multi(EngineView ev, HgpMulti m, dou
   switch (g.code) {
 #define PQ_HGP_CASE(A, B, C_, D_)                                   \
   case hgp_code(A, B, C_, D_):                                      \
-    hgp_tile<A, B, C_, D_>(ev, tl, g.kets, ints, part);             \
+    if constexpr (!kHgpAlone<A, B, C_, D_>)                         \
+      hgp_tile<A, B, C_, D_>(ev, tl, g.kets, ints, part);           \
     break;
     PQ_HGP_CLASSES(PQ_HGP_CASE)
 #undef PQ_HGP_CASE
@@ -39,7 +40,7 @@ bool hgp_multi_class(int la, int lb, int lc, int ld, int* split) {
 #define PQ_HGP_Q(A, B, C_, D_)                             \
   if (la == A && lb == B && lc == C_ && ld == D_) {        \
     *split = kHgpSplit<A, B, C_, D_>;                      \
-    return true;                                           \
+    return !kHgpAlone<A, B, C_, D_>;                       \
   }
   PQ_HGP_CLASSES(PQ_HGP_Q)
 #undef PQ_HGP_Q
