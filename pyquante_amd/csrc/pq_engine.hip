@@ -1139,6 +1139,9 @@ int build_functions(pq_engine* e) {
 }
 }  // namespace
 
+static hipError_t fork_side(pq_engine* eng, hipStream_t st);
+static hipError_t join_side(pq_engine* eng, hipStream_t st);
+
 int pq_get2ints_hgp(pq_engine* eng, double* out_dev, uint64_t len, int compat_nr, void* stream) {
   if (!eng || !out_dev) return fail(PQ_E_ARG, "null argument");
   if (eng->nmol > 1) return fail(PQ_E_STATE, "stored integrals need a single-molecule engine");
@@ -1167,8 +1170,11 @@ int pq_get2ints_hgp(pq_engine* eng, double* out_dev, uint64_t len, int compat_nr
     // the step-table classes and the small unrolled ones share one launch
     // (heaviest class first, so that its long-running waves start first);
     // the large unrolled classes (hgp_multi_class false: 1-2 waves per SIMD
-    // of registers, no private memory) run in launches of their own ahead of
-    // it, as does a list beyond the launch's segment table
+    // of registers) run in launches of their own, as does a list beyond the
+    // launch's segment table.  Those of them with no private memory go to
+    // the side streams, beside the shared launch; the others stay on st.
+    PQ_HIP(fork_side(eng, st));
+    int next = 0;
     HgpMulti mm{};
     int64_t nb = 0;
     for (auto it = eng->hgp_work.rbegin(); it != eng->hgp_work.rend(); ++it) {
@@ -1181,11 +1187,17 @@ int pq_get2ints_hgp(pq_engine* eng, double* out_dev, uint64_t len, int compat_nr
         mm.seg[mm.nseg++] = HgpSeg{c.d_t64, eng->d_sorted[c.Y], hgp_code(la, lb, lc, ld), split, (int)nb, 0};
         nb += blocks * split;
       } else {
+        hipStream_t ks = st;
+        if (eng->nside > 0 && hgp_class_private_bytes(la, lb, lc, ld) == 0) {
+          ks = eng->side[next];
+          next = (next + 1) % eng->nside;
+        }
         PQ_HIP(launch_hgp_class(la, lb, lc, ld, eng->view(), c.d_t64, (int)c.t64.size(), eng->d_sorted[c.Y],
-                                out_dev, st));
+                                out_dev, ks));
       }
     }
     PQ_HIP(launch_hgp_multi(eng->view(), mm, (int)nb, out_dev, st));
+    PQ_HIP(join_side(eng, st));
     return PQ_OK;
   }
   int rc = build_functions(eng);

@@ -336,7 +336,8 @@ hipError_t launch_eri_class(int la, int lb, int lc, int ld, int mode, int fp32, 
 
 #define PQ_HDECL(A, B, C_, D_)                                                                          \
   hipError_t launch_hgp_##A##B##C_##D_(const EngineView& ev, const Tile* tiles, int ntiles, const int* kets, \
-                                       double* ints, hipStream_t st);
+                                       double* ints, hipStream_t st);                                     \
+  size_t hgp_private_##A##B##C_##D_();
 PQ_HDECL(0, 0, 0, 0)
 PQ_HDECL(1, 0, 0, 0)
 PQ_HDECL(1, 0, 1, 0)
@@ -387,6 +388,34 @@ hipError_t launch_hgp_class(int la, int lb, int lc, int ld, const EngineView& ev
   PQ_HCASE(2, 2, 2, 2)
 #undef PQ_HCASE
   return hipErrorInvalidValue;
+}
+
+size_t hgp_class_private_bytes(int la, int lb, int lc, int ld) {
+#define PQ_PCASE(A, B, C_, D_) \
+  if (la == A && lb == B && lc == C_ && ld == D_) return hgp_private_##A##B##C_##D_();
+  PQ_PCASE(0, 0, 0, 0)
+  PQ_PCASE(1, 0, 0, 0)
+  PQ_PCASE(1, 0, 1, 0)
+  PQ_PCASE(1, 1, 0, 0)
+  PQ_PCASE(1, 1, 1, 0)
+  PQ_PCASE(1, 1, 1, 1)
+  PQ_PCASE(2, 0, 0, 0)
+  PQ_PCASE(2, 0, 1, 0)
+  PQ_PCASE(2, 0, 1, 1)
+  PQ_PCASE(2, 0, 2, 0)
+  PQ_PCASE(2, 1, 0, 0)
+  PQ_PCASE(2, 1, 1, 0)
+  PQ_PCASE(2, 1, 1, 1)
+  PQ_PCASE(2, 1, 2, 0)
+  PQ_PCASE(2, 1, 2, 1)
+  PQ_PCASE(2, 2, 0, 0)
+  PQ_PCASE(2, 2, 1, 0)
+  PQ_PCASE(2, 2, 1, 1)
+  PQ_PCASE(2, 2, 2, 0)
+  PQ_PCASE(2, 2, 2, 1)
+  PQ_PCASE(2, 2, 2, 2)
+#undef PQ_PCASE
+  return SIZE_MAX;
 }
 
 hipError_t launch_grad_class(int la, int lb, int lc, int ld, const EngineView& ev, const Tile* tiles, int ntiles,

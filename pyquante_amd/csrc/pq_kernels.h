@@ -38,6 +38,8 @@ hipError_t launch_eri_class(int la, int lb, int lc, int ld, int mode, int fp32, 
 // (the tiles and kets of MODE_STORE), writing the flat unique-ERI array.
 hipError_t launch_hgp_class(int la, int lb, int lc, int ld, const EngineView& ev, const Tile* tiles, int ntiles,
                             const int* kets, double* ints, hipStream_t st);
+// Per-lane private memory of that class's kernel (SIZE_MAX if unknown).
+size_t hgp_class_private_bytes(int la, int lb, int lc, int ld);
 // The HGP classes run in one launch (hgp_multi_class: the class is in it;
 // split = its output parts per tile): segment i covers blocks [b0, next b0)
 // of class code = hgp_code(la, lb, lc, ld), block b0 + tile * split + part.
